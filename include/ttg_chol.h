@@ -1,0 +1,159 @@
+/*
+ * ttg_chol.h -- C ABI of the MI355X (gfx950) tiled-Cholesky kernel library
+ * (libttg_chol.so).
+ *
+ * This is the drop-in boundary below the POTRF/TRSM/SYRK/GEMM device-task
+ * bodies of examples/potrf/potrf.h (devreal/ttg).  Each per-tile entry point
+ * replaces one vendor call of the reference (cited below); the batched, tiled
+ * and solver entry points are the launch-grouped forms the runtime and the
+ * drivers use.
+ *
+ * Conventions (all entry points):
+ *   - Column-major storage.  Every pointer argument named A, B, C, L, X,
+ *     tiles, d_* is a DEVICE pointer; descriptors passed in host arrays are
+ *     staged by the library.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *     Calls are stream-ordered and never synchronise the host, except the
+ *     drivers that document a host read-back.
+ *   - The return value is a launch status: TTG_CHOL_OK (0) or a negative
+ *     TTG_CHOL_E* code; ttg_chol_last_error() describes the last failure of
+ *     the calling thread.  Numerical failure (a non-positive pivot) is
+ *     reported LAPACK-style through a device int `d_info`: 0, or the 1-based
+ *     index of the first pivot that is not positive (NaN included).
+ *   - Per-tile kernels accept any size up to TTG_CHOL_MAX_TILE; the tiled
+ *     drivers use the "lower tile-packed" layout: NT = ceil(N/NB) and tile
+ *     (m, n), n <= m, starts at tiles + tile_index(m, n) * NB * NB with
+ *     tile_index(m, n) = n*NT - n*(n-1)/2 + (m - n), each tile NB x NB with
+ *     leading dimension NB (the PaRSEC sym_block_cyclic tile layout of
+ *     examples/potrf/pmw.h:52-59 for a 1 x 1 process grid).
+ */
+#ifndef TTG_CHOL_H
+#define TTG_CHOL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TTG_CHOL_OK 0
+#define TTG_CHOL_EINVAL (-1)   /* bad argument (size, leading dimension, pointer) */
+#define TTG_CHOL_EHIP (-2)     /* the HIP runtime reported an error */
+#define TTG_CHOL_ENOTSUP (-3)  /* size or option outside what the kernels support */
+#define TTG_CHOL_ENOMEM (-4)   /* device allocation of a plan or workspace failed */
+#define TTG_CHOL_ESTATE (-5)   /* call out of order (e.g. group_end without begin) */
+
+/* Largest tile order handled by the single-tile POTRF/TRSM kernels. */
+#define TTG_CHOL_MAX_TILE 512
+
+/* ---------------------------------------------------------------- */
+/* library                                                           */
+/* ---------------------------------------------------------------- */
+
+/* Version string of the library. */
+const char *ttg_chol_version(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char *ttg_chol_last_error(void);
+/* Number of HIP devices visible to the library (replaces
+ * ttg::device::num_devices() of ttg/ttg/device/device.h:150). */
+int ttg_chol_num_devices(void);
+
+/* ---------------------------------------------------------------- */
+/* per-tile operations (FP64)                                        */
+/* ---------------------------------------------------------------- */
+
+/* In-place lower Cholesky of the n x n tile A:  A = L L^T.
+ * Replaces hipsolverDpotrf(h, LOWER, n, A, lda, work, lwork, devInfo)
+ * (examples/potrf/potrf.h:68-72) and its workspace query (:45-48): no
+ * workspace is needed.  *d_info receives 0 or info_offset + the 1-based
+ * local index of the first non-positive pivot; d_info may be NULL.  The
+ * strict upper triangle is not referenced. */
+int ttg_chol_dpotrf_tile(void *stream, int n, double *A, int lda, int *d_info, int info_offset);
+
+/* B := B * L^{-T}  (side Right, Lower, Trans, NonUnit, alpha = 1) for the
+ * m x n tile B and the n x n lower triangular L.
+ * Replaces hipblasDtrsm(h, RIGHT, LOWER, T, NON_UNIT, m, n, &1, L, ldl, B, ldb)
+ * (examples/potrf/potrf.h:299-304). */
+int ttg_chol_dtrsm_rlt(void *stream, int m, int n, const double *L, int ldl, double *B, int ldb);
+
+/* C := alpha A A^T + beta C on the lower triangle of the n x n tile C, A n x k.
+ * Replaces hipblasDsyrk(h, LOWER, N, n, k, &alpha, A, lda, &beta, C, ldc)
+ * (examples/potrf/potrf.h:429-434; the graph uses alpha = -1, beta = 1). */
+int ttg_chol_dsyrk_ln(void *stream, int n, int k, double alpha, const double *A, int lda, double beta,
+                      double *C, int ldc);
+
+/* C := alpha A B^T + beta C,  A m x k, B n x k, C m x n.
+ * Replaces hipblasDgemm(h, N, T, m, n, k, &alpha, A, lda, B, ldb, &beta, C, ldc)
+ * (examples/potrf/potrf.h:555-561; the graph uses alpha = -1, beta = 1). */
+int ttg_chol_dgemm_nt(void *stream, int m, int n, int k, double alpha, const double *A, int lda,
+                      const double *B, int ldb, double beta, double *C, int ldc);
+
+/* *d_result := ||x||_2 of n contiguous doubles.  Replaces hipblasDnrm2
+ * (examples/potrf/potrf.h:85, DEBUG_TILES_VALUES builds). */
+int ttg_chol_dnrm2(void *stream, int n, const double *x, double *d_result);
+
+/* ---------------------------------------------------------------- */
+/* launch grouping                                                   */
+/* ---------------------------------------------------------------- */
+
+/* Between group_begin(stream) and group_end(stream), the per-tile
+ * operations above issued by the calling thread on `stream` are recorded
+ * instead of launched; group_end launches them as one batched kernel per
+ * operation kind (POTRF, TRSM, SYRK+GEMM), in that order.  Operations in one
+ * group must be independent (no tile written by one op and read or written
+ * by another).  This is how the task runtime fuses every ready device task
+ * of a scheduling round into a few launches (no counterpart in the
+ * reference, which issues one vendor call per task). */
+int ttg_chol_group_begin(void *stream);
+int ttg_chol_group_end(void *stream);
+/* Number of kernel launches issued by the last group_end on this thread. */
+int ttg_chol_group_last_launches(void);
+
+/* ---------------------------------------------------------------- */
+/* whole-matrix drivers (lower tile-packed layout)                   */
+/* ---------------------------------------------------------------- */
+
+/* Flags for the tiled drivers. */
+#define TTG_CHOL_F_LOOKAHEAD 0x1u /* overlap POTRF/TRSM(k+1) with the update of step k */
+#define TTG_CHOL_F_GRAPH 0x2u     /* capture the launch sequence once in a hipGraph and replay it */
+#define TTG_CHOL_F_DEFAULT (TTG_CHOL_F_LOOKAHEAD | TTG_CHOL_F_GRAPH)
+
+/* FP64 tiled Cholesky of the N x N SPD matrix held in `tiles`, in place.
+ * Runs the task graph of examples/potrf/potrf.h:670-764 with every panel
+ * step's TRSMs fused into one launch and its SYRK+GEMM updates fused into
+ * one launch.  Padding of edge tiles (N % NB != 0) is set by the driver.
+ * *d_info = 0 or the global 1-based index of the first failing pivot (the
+ * remaining steps then become no-ops on the device). */
+int ttg_chol_dpotrf_tiled(void *stream, int N, int NB, double *tiles, int *d_info, unsigned flags);
+
+/* Solve A X = B with the factor produced by ttg_chol_dpotrf_tiled
+ * (B is N x nrhs, leading dimension ldb, overwritten by X).
+ * (No counterpart in the reference: POTRS for the iterative refinement.) */
+int ttg_chol_dpotrs_tiled(void *stream, int N, int NB, const double *tiles, int nrhs, double *B, int ldb);
+
+/* R := B - A X, with A symmetric given by its lower tiles (FP64). */
+int ttg_chol_dresidual_tiled(void *stream, int N, int NB, const double *tiles, int nrhs, const double *X,
+                             int ldx, const double *B, int ldb, double *R, int ldr);
+
+/* ---------------------------------------------------------------- */
+/* generators                                                        */
+/* ---------------------------------------------------------------- */
+
+/* PLGSY of examples/potrf/plgsy.h:7-34 / core_plgsy.h:63-150 on the device:
+ * every lower tile of the N x N symmetric random matrix with `bump` on the
+ * diagonal, bit-identical to the reference generator. */
+int ttg_chol_plgsy_tiled(void *stream, int N, int NB, double bump, unsigned long long seed, double *tiles);
+
+/* A = B B^T / N + alpha I into the lower tiles, with B(i,j) = k/127, k an
+ * integer in [-127, 127] drawn from the PLASMA LCG stream position i + j*N.
+ * B B^T is formed exactly (integer data in FP64 MFMA), so A is bit-identical
+ * to the CPU construction.  `work` must hold N*N doubles (the integer B).
+ * (No counterpart in the reference: the benchmark's SPD construction.) */
+int ttg_chol_spd_tiled(void *stream, int N, int NB, double alpha, unsigned long long seed, double *tiles,
+                       double *work);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TTG_CHOL_H */

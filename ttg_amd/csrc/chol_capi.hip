@@ -1,0 +1,313 @@
+// extern "C" entry points of libttg_chol: per-tile operations, launch
+// grouping and the tiled FP64 driver.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "chol_internal.hpp"
+#include "chol_kernels.hpp"
+
+namespace ttgc {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+void clear_error() { g_err[0] = 0; }
+
+constexpr int kDinvBlock = 32 * 32;  // doubles per 32 x 32 inverse block
+
+static int dinv_doubles(int n) { return ((n + 31) / 32) * kDinvBlock; }
+
+// ---------------------------------------------------------------------------
+// per-stream library workspaces (descriptor staging and TRSM inverses)
+// ---------------------------------------------------------------------------
+struct StreamWork {
+  DevBuf ops;   // staged descriptors of grouped launches
+  DevBuf dinv;  // inverses of diagonal blocks for stand-alone TRSMs
+  // pinned host staging of the descriptors; `copied` marks when the last
+  // asynchronous copy out of it has finished
+  char *host = nullptr;
+  size_t host_bytes = 0;
+  hipEvent_t copied = nullptr;
+  bool pending = false;
+  hipError_t host_reserve(size_t n) {
+    if (pending) {
+      hipError_t e = hipEventSynchronize(copied);
+      if (e != hipSuccess) return e;
+      pending = false;
+    }
+    if (!copied) {
+      hipError_t e = hipEventCreateWithFlags(&copied, hipEventDisableTiming);
+      if (e != hipSuccess) return e;
+    }
+    if (n <= host_bytes) return hipSuccess;
+    if (host) {
+      hipError_t e = hipHostFree(host);
+      if (e != hipSuccess) return e;
+      host = nullptr;
+      host_bytes = 0;
+    }
+    size_t want = n < 65536 ? 65536 : 2 * n;
+    hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&host), want, hipHostMallocDefault);
+    if (e == hipSuccess) host_bytes = want;
+    return e;
+  }
+};
+static std::mutex g_ws_mu;
+static std::map<hipStream_t, std::unique_ptr<StreamWork>> g_ws;
+
+static StreamWork &stream_work(hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  auto &p = g_ws[s];
+  if (!p) p.reset(new StreamWork);
+  return *p;
+}
+
+// ---------------------------------------------------------------------------
+// launch grouping state (per calling thread)
+// ---------------------------------------------------------------------------
+struct UpdKey {
+  double alpha, beta;
+  bool operator<(const UpdKey &o) const { return std::tie(alpha, beta) < std::tie(o.alpha, o.beta); }
+};
+struct Group {
+  bool active = false;
+  hipStream_t stream = nullptr;
+  std::vector<PotrfOp> potrf;
+  std::vector<TrsmOp<double>> trsm;
+  std::vector<PotrfOp> trsm_dinv;  // dinv jobs for the TRSMs (A = L tile)
+  std::map<UpdKey, std::vector<UpdOp<double>>> upd;
+  int last_launches = 0;
+};
+static thread_local Group g_group;
+
+static bool grouping(hipStream_t s) { return g_group.active && g_group.stream == s; }
+
+static bool upd_fast(const UpdOp<double> &op) {
+  constexpr int bk = update_bk<double>();
+  return op.m % 128 == 0 && op.n % 128 == 0 && op.k % bk == 0 && op.lda % 2 == 0 && op.ldb % 2 == 0 &&
+         op.ldc % 2 == 0 && ((reinterpret_cast<uintptr_t>(op.A) | reinterpret_cast<uintptr_t>(op.B) |
+                              reinterpret_cast<uintptr_t>(op.C)) &
+                             15) == 0;
+}
+
+}  // namespace ttgc
+
+using namespace ttgc;
+
+extern "C" {
+
+const char *ttg_chol_version(void) { return "ttg_chol 0.1 (gfx950)"; }
+
+const char *ttg_chol_last_error(void) { return ttgc::g_err; }
+
+int ttg_chol_num_devices(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int ttg_chol_dpotrf_tile(void *stream, int n, double *A, int lda, int *d_info, int info_offset) {
+  clear_error();
+  if (n < 0 || n > TTG_CHOL_MAX_TILE || lda < std::max(1, n) || (n > 0 && !A)) {
+    set_error("dpotrf_tile: bad argument (n=%d lda=%d)", n, lda);
+    return TTG_CHOL_EINVAL;
+  }
+  hipStream_t s = as_stream(stream);
+  PotrfOp op{A, nullptr, d_info, n, lda, info_offset, 1};
+  if (n == 0) {
+    if (!d_info) return TTG_CHOL_OK;
+    op.n = 0;  // the kernel stores 0 into *d_info
+  }
+  if (grouping(s)) {
+    g_group.potrf.push_back(op);
+    return TTG_CHOL_OK;
+  }
+  TTGC_HIP(launch_potrf(s, nullptr, 1, op, nullptr));
+  return TTG_CHOL_OK;
+}
+
+int ttg_chol_dtrsm_rlt(void *stream, int m, int n, const double *L, int ldl, double *B, int ldb) {
+  clear_error();
+  if (m < 0 || n < 0 || n > TTG_CHOL_MAX_TILE || ldl < std::max(1, n) || ldb < std::max(1, m) ||
+      (m > 0 && n > 0 && (!L || !B))) {
+    set_error("dtrsm_rlt: bad argument (m=%d n=%d ldl=%d ldb=%d)", m, n, ldl, ldb);
+    return TTG_CHOL_EINVAL;
+  }
+  if (m == 0 || n == 0) return TTG_CHOL_OK;
+  hipStream_t s = as_stream(stream);
+  TrsmOp<double> op{L, B, nullptr, m, n, ldl, ldb};
+  PotrfOp dj{const_cast<double *>(L), nullptr, nullptr, n, ldl, 0, 0};
+  if (grouping(s)) {
+    g_group.trsm.push_back(op);
+    g_group.trsm_dinv.push_back(dj);
+    return TTG_CHOL_OK;
+  }
+  StreamWork &w = stream_work(s);
+  TTGC_HIP(w.dinv.reserve(sizeof(double) * dinv_doubles(n)));
+  dj.dinv = static_cast<double *>(w.dinv.ptr);
+  op.dinv = dj.dinv;
+  TTGC_HIP(launch_dinv(s, nullptr, 1, dj));
+  TTGC_HIP(launch_trsm<double>(s, nullptr, 1, op, m, nullptr));
+  return TTG_CHOL_OK;
+}
+
+static int update_entry(const char *name, hipStream_t s, int m, int n, int k, double alpha, const double *A,
+                        int lda, const double *B, int ldb, double beta, double *C, int ldc, int lower) {
+  if (m < 0 || n < 0 || k < 0 || lda < std::max(1, m) || ldb < std::max(1, n) || ldc < std::max(1, m) ||
+      (m > 0 && n > 0 && (!C || (k > 0 && (!A || !B))))) {
+    set_error("%s: bad argument (m=%d n=%d k=%d lda=%d ldb=%d ldc=%d)", name, m, n, k, lda, ldb, ldc);
+    return TTG_CHOL_EINVAL;
+  }
+  if (m == 0 || n == 0) return TTG_CHOL_OK;
+  if (k == 0) {
+    // C := beta C ; expressed as an update with a zero contribution
+    alpha = 0.0;
+  }
+  UpdOp<double> op{A, B, C, m, n, k, lda, ldb, ldc, lower};
+  if (grouping(s)) {
+    g_group.upd[UpdKey{alpha, beta}].push_back(op);
+    return TTG_CHOL_OK;
+  }
+  TTGC_HIP(launch_update<double>(s, nullptr, 1, op, m, n, alpha, beta, upd_fast(op), nullptr));
+  return TTG_CHOL_OK;
+}
+
+int ttg_chol_dsyrk_ln(void *stream, int n, int k, double alpha, const double *A, int lda, double beta, double *C,
+                      int ldc) {
+  clear_error();
+  return update_entry("dsyrk_ln", as_stream(stream), n, n, k, alpha, A, lda, A, lda, beta, C, ldc, 1);
+}
+
+int ttg_chol_dgemm_nt(void *stream, int m, int n, int k, double alpha, const double *A, int lda, const double *B,
+                      int ldb, double beta, double *C, int ldc) {
+  clear_error();
+  return update_entry("dgemm_nt", as_stream(stream), m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 0);
+}
+
+int ttg_chol_group_begin(void *stream) {
+  clear_error();
+  if (g_group.active) {
+    set_error("group_begin: a group is already open on this thread");
+    return TTG_CHOL_ESTATE;
+  }
+  g_group.active = true;
+  g_group.stream = as_stream(stream);
+  g_group.potrf.clear();
+  g_group.trsm.clear();
+  g_group.trsm_dinv.clear();
+  g_group.upd.clear();
+  return TTG_CHOL_OK;
+}
+
+int ttg_chol_group_last_launches(void) { return g_group.last_launches; }
+
+int ttg_chol_group_end(void *stream) {
+  clear_error();
+  if (!g_group.active || g_group.stream != as_stream(stream)) {
+    set_error("group_end: no group open on this stream");
+    return TTG_CHOL_ESTATE;
+  }
+  Group &g = g_group;
+  g.active = false;
+  hipStream_t s = g.stream;
+  StreamWork &w = stream_work(s);
+  // one staging buffer holds every descriptor array of the group
+  size_t off = 0;
+  auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+  size_t o_potrf = off;
+  off = align(off + g.potrf.size() * sizeof(PotrfOp));
+  size_t o_dinvops = off;
+  off = align(off + g.trsm_dinv.size() * sizeof(PotrfOp));
+  size_t o_trsm = off;
+  off = align(off + g.trsm.size() * sizeof(TrsmOp<double>));
+  std::vector<size_t> o_upd;
+  for (auto &kv : g.upd) {
+    o_upd.push_back(off);
+    off = align(off + kv.second.size() * sizeof(UpdOp<double>));
+  }
+  size_t dinv_off = off;
+  size_t dinv_bytes = 0;
+  for (auto &d : g.trsm_dinv) dinv_bytes += sizeof(double) * dinv_doubles(d.n);
+  off = align(off + dinv_bytes);
+  if (off == 0) {
+    g.last_launches = 0;
+    return TTG_CHOL_OK;
+  }
+  TTGC_HIP(w.ops.reserve(off));
+  char *base = static_cast<char *>(w.ops.ptr);
+  // inverse-block workspace for the TRSMs lives in the same buffer
+  {
+    size_t p = dinv_off;
+    for (size_t i = 0; i < g.trsm.size(); ++i) {
+      g.trsm_dinv[i].dinv = reinterpret_cast<double *>(base + p);
+      g.trsm[i].dinv = g.trsm_dinv[i].dinv;
+      p += sizeof(double) * dinv_doubles(g.trsm_dinv[i].n);
+    }
+  }
+  TTGC_HIP(w.host_reserve(dinv_off));
+  char *host = w.host;
+  if (!g.potrf.empty()) memcpy(host + o_potrf, g.potrf.data(), g.potrf.size() * sizeof(PotrfOp));
+  if (!g.trsm_dinv.empty())
+    memcpy(host + o_dinvops, g.trsm_dinv.data(), g.trsm_dinv.size() * sizeof(PotrfOp));
+  if (!g.trsm.empty()) memcpy(host + o_trsm, g.trsm.data(), g.trsm.size() * sizeof(TrsmOp<double>));
+  {
+    size_t i = 0;
+    for (auto &kv : g.upd) {
+      memcpy(host + o_upd[i], kv.second.data(), kv.second.size() * sizeof(UpdOp<double>));
+      ++i;
+    }
+  }
+  if (dinv_off) {
+    TTGC_HIP(hipMemcpyAsync(base, host, dinv_off, hipMemcpyHostToDevice, s));
+    TTGC_HIP(hipEventRecord(w.copied, s));
+    w.pending = true;
+  }
+  int launches = 0;
+  PotrfOp none{};
+  if (!g.potrf.empty()) {
+    TTGC_HIP(launch_potrf(s, reinterpret_cast<PotrfOp *>(base + o_potrf), (int)g.potrf.size(), none, nullptr));
+    ++launches;
+  }
+  if (!g.trsm.empty()) {
+    TTGC_HIP(launch_dinv(s, reinterpret_cast<PotrfOp *>(base + o_dinvops), (int)g.trsm_dinv.size(), none));
+    int max_m = 0;
+    for (auto &t : g.trsm) max_m = std::max(max_m, t.m);
+    TrsmOp<double> tnone{};
+    TTGC_HIP(launch_trsm<double>(s, reinterpret_cast<TrsmOp<double> *>(base + o_trsm), (int)g.trsm.size(), tnone,
+                                 max_m, nullptr));
+    launches += 2;
+  }
+  {
+    size_t i = 0;
+    for (auto &kv : g.upd) {
+      int max_m = 0, max_n = 0;
+      bool fast = true;
+      for (auto &u : kv.second) {
+        max_m = std::max(max_m, u.m);
+        max_n = std::max(max_n, u.n);
+        fast = fast && upd_fast(u);
+      }
+      UpdOp<double> unone{};
+      TTGC_HIP(launch_update<double>(s, reinterpret_cast<UpdOp<double> *>(base + o_upd[i]), (int)kv.second.size(),
+                                     unone, max_m, max_n, kv.first.alpha, kv.first.beta, fast, nullptr));
+      ++launches;
+      ++i;
+    }
+  }
+  g.last_launches = launches;
+  return TTG_CHOL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,207 @@
+// Device-side building blocks of the tiled Cholesky kernels for gfx950
+// (CDNA4, 64-lane wavefronts): MFMA wrappers, the LDS-staged double-buffered
+// GEMM core shared by the update, TRSM and POTRF kernels, and the
+// XCD-aware block remap.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ttgc {
+
+constexpr int kWave = 64;
+constexpr int kXcds = 8;  // MI355X: 8 XCDs, workgroups are dealt round-robin
+
+typedef double d2_t __attribute__((ext_vector_type(2)));
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+// ----------------------------------------------------------------------------
+// MFMA 16x16x4 (FP64 and FP32).  Operand layout (64 lanes, l = lane):
+//   A fragment 16 x 4:  lane l holds A[i = l % 16][k = l / 16]
+//   B fragment 4 x 16:  lane l holds B[k = l / 16][j = l % 16]
+//   D 16 x 16:          lane l, element r holds D[i = 4*(l/16) + r][j = l % 16]
+// ----------------------------------------------------------------------------
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<double> {
+  using acc_t = d4_t;
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+
+template <>
+struct Mfma<float> {
+  using acc_t = f4_t;
+  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename T>
+struct Vec16;  // a 16-byte vector of T
+template <>
+struct Vec16<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+  static constexpr int n = 2;
+};
+template <>
+struct Vec16<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+  static constexpr int n = 4;
+};
+
+// Workgroup b of a grid of G is placed on XCD b % 8.  Return a logical index
+// such that consecutive logical indices run on the same XCD (and share its
+// L2), while the map stays a bijection on [0, G).
+__device__ __forceinline__ int xcd_remap(int b, int G) {
+  const int q = G / kXcds, r = G % kXcds;
+  const int x = b % kXcds, pos = b / kXcds;
+  return x * q + (x < r ? x : r) + pos;
+}
+
+// Make this workgroup's global stores visible to its own later global loads
+// (the POTRF and TRSM kernels re-read data they wrote).
+__device__ __forceinline__ void wg_sync_global() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+  __syncthreads();
+}
+
+// ----------------------------------------------------------------------------
+// Stage loader: ROWS x COLS block of a column-major matrix into registers,
+// then into an LDS array laid out [COLS][LD] (k-major, rows contiguous).
+// FAST: 16-byte loads, no bounds checks (caller guarantees alignment and
+// full blocks).  Otherwise scalar predicated loads with zero fill.
+// ----------------------------------------------------------------------------
+template <typename T, int ROWS, int COLS, int NTH, int LD>
+struct StageLoader {
+  static constexpr int V = Vec16<T>::n;
+  using vec_t = typename Vec16<T>::type;
+  static constexpr int VPC = ROWS / V;  // vectors per column
+  static_assert(ROWS % V == 0, "rows must be a multiple of the vector width");
+  static_assert(NTH % VPC == 0, "threads must cover whole columns");
+  static constexpr int CPP = NTH / VPC;  // columns per pass
+  static_assert(COLS % CPP == 0, "columns must be a multiple of columns per pass");
+  static constexpr int PASSES = COLS / CPP;
+
+  vec_t r[PASSES];
+
+  template <bool FAST>
+  __device__ __forceinline__ void load(const T *__restrict__ base, long ld, int row0, int col0, int rlim,
+                                       int clim, int tid) {
+    const int v = tid % VPC, c = tid / VPC;
+    const int row = row0 + v * V;
+#pragma unroll
+    for (int p = 0; p < PASSES; ++p) {
+      const int col = col0 + c + p * CPP;
+      if constexpr (FAST) {
+        r[p] = *reinterpret_cast<const vec_t *>(base + row + (long)col * ld);
+      } else {
+#pragma unroll
+        for (int e = 0; e < V; ++e)
+          r[p][e] = (row + e < rlim && col < clim) ? base[row + e + (long)col * ld] : T(0);
+      }
+    }
+  }
+
+  __device__ __forceinline__ void store(T (*s)[LD], int tid) const {
+    const int v = tid % VPC, c = tid / VPC;
+#pragma unroll
+    for (int p = 0; p < PASSES; ++p) *reinterpret_cast<vec_t *>(&s[c + p * CPP][v * V]) = r[p];
+  }
+};
+
+// ----------------------------------------------------------------------------
+// GEMM core: acc (BM x BN block, spread over WM x WN waves) += A * B^T where
+// A is rows [i0, i0+BM) and B is rows [j0, j0+BN) of two column-major
+// matrices, summed over k in [kbeg, kend).  Both operands are staged through
+// LDS in BK-wide slabs, double buffered with a register prefetch of the next
+// slab while MFMAs consume the current one.
+// ----------------------------------------------------------------------------
+template <typename T, int BM, int BN, int BK, int WM, int WN>
+struct GemmCore {
+  static constexpr int NW = WM * WN;
+  static constexpr int NTH = NW * kWave;
+  static constexpr int TM = BM / WM, TN = BN / WN;
+  static constexpr int MI = TM / 16, NI = TN / 16;
+  static_assert(TM % 16 == 0 && TN % 16 == 0, "wave tile must be a multiple of 16");
+  static_assert(BK % 4 == 0, "BK must be a multiple of the MFMA k (4)");
+  static constexpr int PAD = 32 / sizeof(T);
+  static constexpr int LDA_S = BM + PAD, LDB_S = BN + PAD;
+  using acc_t = typename Mfma<T>::acc_t;
+
+  struct Smem {
+    T a[2][BK][LDA_S];
+    T b[2][BK][LDB_S];
+  };
+
+  acc_t acc[MI][NI];
+
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+#pragma unroll
+      for (int j = 0; j < NI; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+  }
+
+  __device__ __forceinline__ void compute(const Smem &sm, int buf, int wm, int wn, int lane) {
+    const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int ks = 0; ks < BK / 4; ++ks) {
+      T a[MI], b[NI];
+#pragma unroll
+      for (int i = 0; i < MI; ++i) a[i] = sm.a[buf][ks * 4 + lk][wm * TM + i * 16 + lr];
+#pragma unroll
+      for (int j = 0; j < NI; ++j) b[j] = sm.b[buf][ks * 4 + lk][wn * TN + j * 16 + lr];
+#pragma unroll
+      for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NI; ++j) acc[i][j] = Mfma<T>::mma(a[i], b[j], acc[i][j]);
+    }
+  }
+
+  // FAST: every slab is a full, 16-byte aligned block (no predication).
+  // mlim/nlim: row limits of A and B; kend: column limit of both.
+  template <bool FAST>
+  __device__ __forceinline__ void run(Smem &sm, const T *__restrict__ A, long lda, const T *__restrict__ B,
+                                      long ldb, int i0, int j0, int kbeg, int kend, int mlim, int nlim) {
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    const int wm = wave / WN, wn = wave % WN;
+    StageLoader<T, BM, BK, NTH, LDA_S> la;
+    StageLoader<T, BN, BK, NTH, LDB_S> lb;
+    const int nk = (kend - kbeg + BK - 1) / BK;
+    if (nk <= 0) return;
+    la.template load<FAST>(A, lda, i0, kbeg, mlim, kend, tid);
+    lb.template load<FAST>(B, ldb, j0, kbeg, nlim, kend, tid);
+    la.store(sm.a[0], tid);
+    lb.store(sm.b[0], tid);
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+      const int cur = kt & 1;
+      const bool more = kt + 1 < nk;
+      if (more) {
+        const int k0 = kbeg + (kt + 1) * BK;
+        la.template load<FAST>(A, lda, i0, k0, mlim, kend, tid);
+        lb.template load<FAST>(B, ldb, j0, k0, nlim, kend, tid);
+      }
+      compute(sm, cur, wm, wn, lane);
+      if (more) {
+        la.store(sm.a[cur ^ 1], tid);
+        lb.store(sm.b[cur ^ 1], tid);
+      }
+      __syncthreads();
+    }
+  }
+
+  // Row / column (relative to the block origin) of accumulator element r of
+  // MFMA tile (i, j) held by this lane.
+  static __device__ __forceinline__ int row_of(int wm, int i, int lane, int r) {
+    return wm * TM + i * 16 + 4 * (lane >> 4) + r;
+  }
+  static __device__ __forceinline__ int col_of(int wn, int j, int lane) { return wn * TN + j * 16 + (lane & 15); }
+};
+
+}  // namespace ttgc

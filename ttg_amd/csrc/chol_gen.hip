@@ -1,0 +1,234 @@
+// Matrix generators on the device (tile-packed lower layout):
+//   PLGSY (examples/potrf/core_plgsy.h:63-150), bit-identical to the
+//   reference, and the SPD construction A = B B^T / N + alpha I with integer
+//   B, formed exactly on FP64 MFMA.  Also the small utility kernels of the
+//   tiled drivers (edge-tile padding, info reset, diagonal shift, nrm2).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "chol_internal.hpp"
+#include "chol_kernels.hpp"
+#include "chol_tiled.hpp"
+
+namespace ttgc {
+
+__device__ __forceinline__ unsigned long long rnd64_jump(unsigned long long n, unsigned long long seed) {
+  // random.h:24-41 (PLASMA LCG skip-ahead by binary powering)
+  unsigned long long a_k = 6364136223846793005ULL, c_k = 1ULL, ran = seed;
+  for (; n; n >>= 1) {
+    if (n & 1ULL) ran = a_k * ran + c_k;
+    c_k *= (a_k + 1ULL);
+    a_k *= a_k;
+  }
+  return ran;
+}
+
+// 0.5f - ran * 5.4210108624275222e-20f in single precision (core_plgsy.h:86),
+// each operation rounded on its own.
+__device__ __forceinline__ double plgsy_value(unsigned long long ran) {
+#pragma clang fp contract(off)
+  const float r = __ull2float_rn(ran);
+  const float p = __fmul_rn(r, 5.4210108624275222e-20f);
+  return (double)__fsub_rn(0.5f, p);
+}
+
+__device__ __forceinline__ double spd_bint(unsigned long long ran) {
+  return (double)((int)((((ran >> 32) * 255ULL) >> 32)) - 127);
+}
+
+__device__ __forceinline__ void tile_of_index(long t, int nt, int &m, int &n) {
+  // inverse of tile_index(m, n) = n*nt - n(n-1)/2 + (m - n)
+  int lo = 0, hi = nt - 1;
+  while (lo < hi) {  // largest n with start(n) <= t
+    const int mid = (lo + hi + 1) / 2;
+    const long st = (long)mid * nt - (long)mid * (mid - 1) / 2;
+    if (st <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  n = lo;
+  m = n + (int)(t - ((long)n * nt - (long)n * (n - 1) / 2));
+}
+
+// One wave per tile column: lane l produces rows l, l+64, ... (coalesced).
+__global__ __launch_bounds__(256) void plgsy_kernel(int N, int NB, int nt, double bump, unsigned long long seed,
+                                                    double *tiles) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64, lane = threadIdx.x & 63;
+  const long ncols = (long)nt * (nt + 1) / 2 * NB;
+  if (wave >= ncols) return;
+  const long t = wave / NB;
+  const int jj = wave % NB;
+  int m, n;
+  tile_of_index(t, nt, m, n);
+  const int mb = min(NB, N - m * NB), nb = min(NB, N - n * NB);
+  if (jj >= nb) return;
+  double *col = tiles + t * (long)NB * NB + (long)jj * NB;
+  const long gj = (long)n * NB + jj;
+  const unsigned long long N64 = (unsigned long long)N;
+  for (int ii = lane; ii < mb; ii += 64) {
+    const long gi = (long)m * NB + ii;
+    double v;
+    if (gi >= gj) {
+      v = plgsy_value(rnd64_jump((unsigned long long)gi + (unsigned long long)gj * N64, seed));
+      if (gi == gj) v += bump;
+    } else {  // upper part of a diagonal tile mirrors the lower part
+      v = plgsy_value(rnd64_jump((unsigned long long)gj + (unsigned long long)gi * N64, seed));
+    }
+    col[ii] = v;
+  }
+}
+
+// Integer-valued B (N x N, column-major, stored as doubles).
+__global__ __launch_bounds__(256) void spd_b_kernel(long N, unsigned long long seed, double *B) {
+  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const int lane = threadIdx.x & 63;
+  if (wave >= N) return;
+  const long j = wave;
+  // lane l starts at row l; all lanes advance by 64 rows = 64 LCG steps
+  unsigned long long ran = rnd64_jump((unsigned long long)(j * N + lane), seed);
+  unsigned long long a64 = 6364136223846793005ULL, c64 = 1ULL;
+  {  // affine map of 64 steps: (a, c)^64 by squaring
+    unsigned long long a = a64, c = c64;
+    for (int s = 0; s < 6; ++s) {
+      c = c * (a + 1ULL);
+      a = a * a;
+    }
+    a64 = a;
+    c64 = c;
+  }
+  for (long i = lane; i < N; i += 64) {
+    B[i + j * N] = spd_bint(ran);
+    ran = a64 * ran + c64;
+  }
+}
+
+__global__ void add_diag_kernel(int N, int NB, int nt, double alpha, double *tiles) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= N) return;
+  const int t = (int)(g / NB), r = (int)(g % NB);
+  double *d = tiles + tile_index(t, t, nt) * (long)NB * NB;
+  d[r + (long)r * NB] = __dadd_rn(d[r + (long)r * NB], alpha);
+}
+
+// Reset *d_info and set the padding of the edge tiles (rows/cols >= N of the
+// last tile row) to zero, with ones on the padded diagonal, so every tile op
+// of the factorization can run on full NB x NB tiles.
+__global__ void prepare_kernel(int N, int NB, int nt, double *tiles, int *d_info) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g == 0 && d_info) *d_info = 0;
+  const int rem = N - (nt - 1) * NB;  // valid rows of the last tile row
+  if (rem == NB) return;
+  // nt tiles in the last tile row, NB x NB each
+  const long per = (long)NB * NB;
+  if (g >= per * nt) return;
+  const int n = (int)(g / per);
+  const long e = g % per;
+  const int r = (int)(e % NB), c = (int)(e / NB);
+  double *t = tiles + tile_index(nt - 1, n, nt) * per;
+  if (n < nt - 1) {
+    if (r >= rem) t[e] = 0.0;
+  } else {
+    if (r >= rem || c >= rem) t[e] = (r == c) ? 1.0 : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(1024) void nrm2_kernel(int n, const double *x, double *out) {
+  __shared__ double part[1024];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += x[i] * x[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sqrt(part[0]);
+}
+
+hipError_t launch_prepare(hipStream_t s, int N, int NB, double *tiles, int *d_info) {
+  const int nt = (N + NB - 1) / NB;
+  const long work = (long)NB * NB * nt;
+  const long blocks = std::max(1L, (work + 255) / 256);
+  hipLaunchKernelGGL(prepare_kernel, dim3((unsigned)blocks), dim3(256), 0, s, N, NB, nt, tiles, d_info);
+  return hipGetLastError();
+}
+
+}  // namespace ttgc
+
+using namespace ttgc;
+
+extern "C" {
+
+int ttg_chol_plgsy_tiled(void *stream, int N, int NB, double bump, unsigned long long seed, double *tiles) {
+  clear_error();
+  if (N < 0 || NB <= 0 || (N > 0 && !tiles)) {
+    set_error("plgsy_tiled: bad argument (N=%d NB=%d)", N, NB);
+    return TTG_CHOL_EINVAL;
+  }
+  if (N == 0) return TTG_CHOL_OK;
+  const int nt = (N + NB - 1) / NB;
+  const long waves = (long)nt * (nt + 1) / 2 * NB;
+  const long blocks = (waves + 3) / 4;
+  hipLaunchKernelGGL(plgsy_kernel, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), N, NB, nt, bump, seed,
+                     tiles);
+  TTGC_HIP(hipGetLastError());
+  return TTG_CHOL_OK;
+}
+
+int ttg_chol_spd_tiled(void *stream, int N, int NB, double alpha, unsigned long long seed, double *tiles,
+                       double *work) {
+  clear_error();
+  if (N < 0 || NB <= 0 || (N > 0 && (!tiles || !work))) {
+    set_error("spd_tiled: bad argument (N=%d NB=%d)", N, NB);
+    return TTG_CHOL_EINVAL;
+  }
+  if (N == 0) return TTG_CHOL_OK;
+  hipStream_t s = as_stream(stream);
+  const int nt = (N + NB - 1) / NB;
+  hipLaunchKernelGGL(spd_b_kernel, dim3((unsigned)(((long)N + 3) / 4)), dim3(256), 0, s, (long)N, seed, work);
+  TTGC_HIP(hipGetLastError());
+  // lower tiles of B B^T, exact; scaled once by 1/(127^2 N) in the epilogue
+  std::vector<UpdOp<double>> fast, slow;
+  for (int n = 0; n < nt; ++n)
+    for (int m = n; m < nt; ++m) {
+      const int mb = std::min(NB, N - m * NB), nb = std::min(NB, N - n * NB);
+      UpdOp<double> op{work + (long)m * NB, work + (long)n * NB, tiles + tile_index(m, n, nt) * (long)NB * NB,
+                       mb, nb, N, N, N, NB, m == n ? 1 : 0};
+      const bool f = mb % 128 == 0 && nb % 128 == 0 && N % update_bk<double>() == 0 && N % 2 == 0 && NB % 2 == 0;
+      (f ? fast : slow).push_back(op);
+    }
+  const double c = 1.0 / (127.0 * 127.0 * (double)N);
+  DevBuf ops;
+  const size_t bytes = sizeof(UpdOp<double>) * (fast.size() + slow.size());
+  TTGC_HIP(ops.reserve(bytes));
+  auto *d = static_cast<UpdOp<double> *>(ops.ptr);
+  if (!fast.empty()) TTGC_HIP(hipMemcpy(d, fast.data(), sizeof(UpdOp<double>) * fast.size(), hipMemcpyHostToDevice));
+  if (!slow.empty())
+    TTGC_HIP(hipMemcpy(d + fast.size(), slow.data(), sizeof(UpdOp<double>) * slow.size(), hipMemcpyHostToDevice));
+  UpdOp<double> none{};
+  TTGC_HIP(launch_update<double>(s, d, (int)fast.size(), none, NB, NB, c, 0.0, true, nullptr));
+  TTGC_HIP(launch_update<double>(s, d + fast.size(), (int)slow.size(), none, NB, NB, c, 0.0, false, nullptr));
+  hipLaunchKernelGGL(add_diag_kernel, dim3((unsigned)(((long)N + 255) / 256)), dim3(256), 0, s, N, NB, nt, alpha,
+                     tiles);
+  TTGC_HIP(hipGetLastError());
+  // the descriptor buffer is freed when this call returns: wait for its users
+  TTGC_HIP(hipStreamSynchronize(s));
+  (void)hipFree(ops.ptr);
+  return TTG_CHOL_OK;
+}
+
+int ttg_chol_dnrm2(void *stream, int n, const double *x, double *d_result) {
+  clear_error();
+  if (n < 0 || !d_result || (n > 0 && !x)) {
+    set_error("dnrm2: bad argument (n=%d)", n);
+    return TTG_CHOL_EINVAL;
+  }
+  hipLaunchKernelGGL(nrm2_kernel, dim3(1), dim3(1024), 0, as_stream(stream), n, x, d_result);
+  TTGC_HIP(hipGetLastError());
+  return TTG_CHOL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,388 @@
+// Tile kernels of the tiled Cholesky for gfx950.
+//
+//   potrf_kernel  : one workgroup factors one n x n tile (n <= 512) in place,
+//                   blocked by 32: a 32 x 32 diagonal block is factored and
+//                   inverted in LDS, the panel below it is multiplied by the
+//                   inverse, and the trailing lower triangle is updated with
+//                   FP64 MFMA from the LDS-resident panel.  Also emits the
+//                   inverses of the 32 x 32 diagonal blocks for the TRSM.
+//   trsm_kernel   : B := B L^{-T} by 64-row stripes; each workgroup walks the
+//                   32-wide column blocks left to right:  X_j = (B_j -
+//                   X_{<j} L_{j,<j}^T) inv(L_jj)^T, the GEMM part on MFMA.
+//   update_kernel : batched C := alpha A B^T + beta C (GEMM, or the lower
+//                   triangle for SYRK), 128 x 128 blocks per workgroup, LDS
+//                   double-buffered, XCD-aware block order.
+#include "chol_device.hpp"
+#include "chol_kernels.hpp"
+
+namespace ttgc {
+
+constexpr int kPB = 32;                       // POTRF/TRSM inner block
+constexpr int kMaxTile = 512;                 // largest single-tile POTRF/TRSM
+constexpr int kPanelLd = kMaxTile - kPB + 4;  // LDS panel row stride
+
+// ---------------------------------------------------------------------------
+// POTRF
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void potrf_kernel(const PotrfOp *__restrict__ ops, PotrfOp one,
+                                                    const int *__restrict__ d_abort) {
+  if (d_abort && *d_abort) return;
+  const PotrfOp op = ops ? ops[blockIdx.x] : one;
+  __shared__ double Ds[kPB][kPB + 1];
+  __shared__ double Di[kPB][kPB + 1];
+  __shared__ __align__(16) double Ps[kPB][kPanelLd];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int n = op.n;
+  const long lda = op.lda;
+  double *__restrict__ A = op.A;
+  int fail = 0;
+  const int nblk = (n + kPB - 1) / kPB;
+
+  for (int jb = 0; jb < nblk; ++jb) {
+    const int c0 = jb * kPB, bs = min(kPB, n - c0);
+    // 1. diagonal block, lower part, identity padding beyond n
+    for (int e = tid; e < kPB * kPB; e += 256) {
+      const int r = e & 31, c = e >> 5;
+      double v;
+      if (r < bs && c < bs)
+        v = (r >= c) ? A[c0 + r + (long)(c0 + c) * lda] : 0.0;
+      else
+        v = (r == c) ? 1.0 : 0.0;
+      Ds[r][c] = v;
+    }
+    __syncthreads();
+
+    // 2. right-looking Cholesky of the 32 x 32 block (dpotf2 order)
+    {
+      const int i = tid & 31;
+      for (int j = 0; j < kPB; ++j) {
+        const double djj = Ds[j][j];
+        if (!(djj > 0.0)) {  // also catches NaN; the value is the same in every thread
+          fail = c0 + j + 1;
+          break;
+        }
+        const double d = sqrt(djj), rinv = 1.0 / d;
+        const double lij = Ds[i][j] * rinv;
+        double upd[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int l = (tid >> 5) + 8 * q;
+          upd[q] = (l > j && i >= l) ? lij * (Ds[l][j] * rinv) : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int l = (tid >> 5) + 8 * q;
+          if (l > j && i >= l) Ds[i][l] -= upd[q];
+        }
+        if (tid < kPB) {
+          if (tid > j)
+            Ds[tid][j] *= rinv;
+          else if (tid == j)
+            Ds[j][j] = d;
+        }
+        __syncthreads();
+      }
+    }
+    if (fail) break;
+
+    // 3. Di = inv(L_D) (one column per thread), L_D back to global
+    if (tid < kPB) {
+      const int c = tid;
+      double x[kPB];
+#pragma unroll
+      for (int i = 0; i < kPB; ++i) {
+        double v;
+        if (i < c) {
+          v = 0.0;
+        } else if (i == c) {
+          v = 1.0 / Ds[c][c];
+        } else {
+          double s = 0.0;
+#pragma unroll
+          for (int k = 0; k < i; ++k)
+            if (k >= c) s += Ds[i][k] * x[k];
+          v = -s / Ds[i][i];
+        }
+        x[i] = v;
+        Di[i][c] = v;
+      }
+    }
+    for (int e = tid; e < kPB * kPB; e += 256) {
+      const int r = e & 31, c = e >> 5;
+      if (r >= c && r < bs && c < bs) A[c0 + r + (long)(c0 + c) * lda] = Ds[r][c];
+    }
+    __syncthreads();
+    if (op.dinv)
+      for (int e = tid; e < kPB * kPB; e += 256) op.dinv[jb * kPB * kPB + e] = Di[e & 31][e >> 5];
+
+    // 4. panel rows below the block: P = A(:, c0:c0+32) * Di^T
+    const int R = n - c0 - kPB;
+    for (int rr = tid; rr < R; rr += 256) {
+      const long i = c0 + kPB + rr;
+      double a[kPB];
+#pragma unroll
+      for (int k = 0; k < kPB; ++k) a[k] = A[i + (long)(c0 + k) * lda];
+#pragma unroll
+      for (int c = 0; c < kPB; ++c) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k <= c; ++k) s += a[k] * Di[c][k];
+        A[i + (long)(c0 + c) * lda] = s;
+        Ps[c][rr] = s;
+      }
+    }
+    __syncthreads();
+
+    // 5. trailing lower update  A22 -= P P^T  on 16 x 16 blocks
+    if (R > 0) {
+      const int nb = (R + 15) / 16, nblocks = nb * (nb + 1) / 2;
+      const int lr = lane & 15, lk = lane >> 4;
+      for (int q = wave; q < nblocks; q += 4) {
+        int bi = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
+        while ((bi + 1) * (bi + 2) / 2 <= q) ++bi;
+        while (bi * (bi + 1) / 2 > q) --bi;
+        const int bj = q - bi * (bi + 1) / 2;
+        const int col = c0 + kPB + bj * 16 + lr;
+        const int rowb = c0 + kPB + bi * 16 + 4 * lk;
+        double old[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = rowb + r;
+          old[r] = (row < n && col < n) ? A[row + (long)col * lda] : 0.0;
+        }
+        d4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < kPB / 4; ++ks) {
+          const double av = Ps[ks * 4 + lk][bi * 16 + lr];
+          const double bv = Ps[ks * 4 + lk][bj * 16 + lr];
+          acc = Mfma<double>::mma(av, bv, acc);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = rowb + r;
+          if (row < n && col < n && (bi != bj || row >= col)) A[row + (long)col * lda] = old[r] - acc[r];
+        }
+      }
+    }
+    wg_sync_global();
+  }
+  if (tid == 0 && op.info) {
+    if (fail)
+      *op.info = op.info_offset + fail;
+    else if (op.write_zero)
+      *op.info = 0;
+  }
+}
+
+// inv(L_jj) of every 32 x 32 diagonal block of an already factored tile.
+__global__ __launch_bounds__(256) void dinv_kernel(const PotrfOp *__restrict__ ops, PotrfOp one) {
+  const PotrfOp op = ops ? ops[blockIdx.x] : one;
+  const int n = op.n;
+  const long lda = op.lda;
+  const int nblk = (n + kPB - 1) / kPB;
+  for (int t = threadIdx.x; t < nblk * kPB; t += blockDim.x) {
+    const int jb = t / kPB, c = t % kPB, c0 = jb * kPB, bs = min(kPB, n - c0);
+    const double *L = op.A + c0 + (long)c0 * lda;
+    double x[kPB];
+#pragma unroll
+    for (int i = 0; i < kPB; ++i) {
+      double v;
+      if (c >= bs) {
+        v = (i == c) ? 1.0 : 0.0;
+      } else if (i < c || i >= bs) {
+        v = 0.0;
+      } else if (i == c) {
+        v = 1.0 / L[c + (long)c * lda];
+      } else {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < i; ++k)
+          if (k >= c) s += L[i + (long)k * lda] * x[k];
+        v = -s / L[i + (long)i * lda];
+      }
+      x[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < kPB; ++i) op.dinv[jb * kPB * kPB + i + c * kPB] = x[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// TRSM  B := B L^{-T}
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_kernel(const TrsmOp<T> *__restrict__ ops, TrsmOp<T> one, int bpo,
+                                                   const int *__restrict__ d_abort) {
+  if (d_abort && *d_abort) return;
+  const int o = blockIdx.x / bpo, s = blockIdx.x % bpo;
+  const TrsmOp<T> op = ops ? ops[o] : one;
+  const int r0 = s * 64;
+  if (r0 >= op.m) return;
+  using Core = GemmCore<T, 64, 32, 32, 4, 1>;
+  __shared__ __align__(16) typename Core::Smem sm;
+  __shared__ T Ts[kPB][64 + 4];
+  __shared__ T Dis[kPB][kPB + 1];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  constexpr int V = Vec16<T>::n;
+  const bool fast = (op.m % 64 == 0) && (op.n % kPB == 0) && (op.ldb % V == 0) && (op.ldl % V == 0) &&
+                    ((reinterpret_cast<uintptr_t>(op.B) & 15) == 0) &&
+                    ((reinterpret_cast<uintptr_t>(op.L) & 15) == 0);
+  const int nblk = (op.n + kPB - 1) / kPB;
+  for (int j = 0; j < nblk; ++j) {
+    const int c0 = j * kPB;
+    Core core;
+    core.zero();
+    if (c0 > 0) {
+      if (fast)
+        core.template run<true>(sm, op.B, op.ldb, op.L, op.ldl, r0, c0, 0, c0, op.m, op.n);
+      else
+        core.template run<false>(sm, op.B, op.ldb, op.L, op.ldl, r0, c0, 0, c0, op.m, op.n);
+    }
+    for (int e = tid; e < kPB * kPB; e += 256) Dis[e & 31][e >> 5] = op.dinv[j * kPB * kPB + e];
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int rl = wave * 16 + 4 * lk + r, cl = jj * 16 + lr;
+        const int row = r0 + rl, col = c0 + cl;
+        const T bv = (row < op.m && col < op.n) ? op.B[row + (long)col * op.ldb] : T(0);
+        Ts[cl][rl] = bv - core.acc[0][jj][r];
+      }
+    __syncthreads();
+    typename Mfma<T>::acc_t x[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+#pragma unroll
+    for (int ks = 0; ks < kPB / 4; ++ks) {
+      const T av = Ts[ks * 4 + lk][wave * 16 + lr];
+#pragma unroll
+      for (int jj = 0; jj < 2; ++jj) x[jj] = Mfma<T>::mma(av, Dis[jj * 16 + lr][ks * 4 + lk], x[jj]);
+    }
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = r0 + wave * 16 + 4 * lk + r, col = c0 + jj * 16 + lr;
+        if (row < op.m && col < op.n) op.B[row + (long)col * op.ldb] = x[jj][r];
+      }
+    wg_sync_global();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// batched update  C := alpha A B^T + beta C
+// ---------------------------------------------------------------------------
+template <typename T>
+struct UpdCfg;
+template <>
+struct UpdCfg<double> {
+  using Core = GemmCore<double, 128, 128, 16, 2, 2>;
+};
+template <>
+struct UpdCfg<float> {
+  using Core = GemmCore<float, 128, 128, 32, 2, 2>;
+};
+
+template <typename T, bool FAST>
+__global__ __launch_bounds__(256) void update_kernel(const UpdOp<T> *__restrict__ ops, UpdOp<T> one, int bpo, int nbn,
+                                                     T alpha, T beta, const int *__restrict__ d_abort) {
+  if (d_abort && *d_abort) return;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int o = b / bpo, rr = b % bpo, bi = rr / nbn, bj = rr % nbn;
+  const UpdOp<T> op = ops ? ops[o] : one;
+  if (op.lower && bi < bj) return;
+  const int i0 = bi * 128, j0 = bj * 128;
+  if (i0 >= op.m || j0 >= op.n) return;
+  using Core = typename UpdCfg<T>::Core;
+  __shared__ __align__(16) typename Core::Smem sm;
+  Core core;
+  core.zero();
+  core.template run<FAST>(sm, op.A, op.lda, op.B, op.ldb, i0, j0, 0, op.k, op.m, op.n);
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave / 2, wn = wave % 2;
+  const bool diag = op.lower && bi == bj;
+#pragma unroll
+  for (int i = 0; i < Core::MI; ++i)
+#pragma unroll
+    for (int j = 0; j < Core::NI; ++j) {
+      const int col = j0 + Core::col_of(wn, j, lane);
+      const int rowb = i0 + Core::row_of(wm, i, lane, 0);
+      T *p = op.C + rowb + (long)col * op.ldc;
+      if (FAST && !diag) {
+        if (beta == T(0)) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) p[r] = alpha * core.acc[i][j][r];
+        } else {
+          T old[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) old[r] = p[r];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) p[r] = alpha * core.acc[i][j][r] + beta * old[r];
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = rowb + r;
+          if (row < op.m && col < op.n && (!diag || row >= col)) {
+            const T v = alpha * core.acc[i][j][r];
+            p[r] = (beta == T(0)) ? v : v + beta * p[r];
+          }
+        }
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers (ops == nullptr: a single op passed by value in `one`)
+// ---------------------------------------------------------------------------
+hipError_t launch_potrf(hipStream_t s, const PotrfOp *ops, int nops, const PotrfOp &one, const int *d_abort) {
+  if (nops <= 0) return hipSuccess;
+  hipLaunchKernelGGL(potrf_kernel, dim3(nops), dim3(256), 0, s, ops, one, d_abort);
+  return hipGetLastError();
+}
+
+hipError_t launch_dinv(hipStream_t s, const PotrfOp *ops, int nops, const PotrfOp &one) {
+  if (nops <= 0) return hipSuccess;
+  hipLaunchKernelGGL(dinv_kernel, dim3(nops), dim3(256), 0, s, ops, one);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_trsm(hipStream_t s, const TrsmOp<T> *ops, int nops, const TrsmOp<T> &one, int max_m,
+                       const int *d_abort) {
+  if (nops <= 0 || max_m <= 0) return hipSuccess;
+  const int bpo = (max_m + 63) / 64;
+  hipLaunchKernelGGL(trsm_kernel<T>, dim3(nops * bpo), dim3(256), 0, s, ops, one, bpo, d_abort);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_update(hipStream_t s, const UpdOp<T> *ops, int nops, const UpdOp<T> &one, int max_m, int max_n,
+                         T alpha, T beta, bool fast, const int *d_abort) {
+  if (nops <= 0 || max_m <= 0 || max_n <= 0) return hipSuccess;
+  const int nbm = (max_m + 127) / 128, nbn = (max_n + 127) / 128;
+  const int bpo = nbm * nbn;
+  const long grid = (long)nops * bpo;
+  if (grid > 0x7fffffffL) return hipErrorInvalidConfiguration;
+  if (fast)
+    hipLaunchKernelGGL((update_kernel<T, true>), dim3((unsigned)grid), dim3(256), 0, s, ops, one, bpo, nbn, alpha,
+                       beta, d_abort);
+  else
+    hipLaunchKernelGGL((update_kernel<T, false>), dim3((unsigned)grid), dim3(256), 0, s, ops, one, bpo, nbn, alpha,
+                       beta, d_abort);
+  return hipGetLastError();
+}
+
+template hipError_t launch_trsm<double>(hipStream_t, const TrsmOp<double> *, int, const TrsmOp<double> &, int,
+                                        const int *);
+template hipError_t launch_trsm<float>(hipStream_t, const TrsmOp<float> *, int, const TrsmOp<float> &, int,
+                                       const int *);
+template hipError_t launch_update<double>(hipStream_t, const UpdOp<double> *, int, const UpdOp<double> &, int, int,
+                                          double, double, bool, const int *);
+template hipError_t launch_update<float>(hipStream_t, const UpdOp<float> *, int, const UpdOp<float> &, int, int,
+                                         float, float, bool, const int *);
+
+}  // namespace ttgc
