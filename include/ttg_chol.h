@@ -117,6 +117,7 @@ int ttg_chol_group_last_launches(void);
 /* Flags for the tiled drivers. */
 #define TTG_CHOL_F_LOOKAHEAD 0x1u /* overlap POTRF/TRSM(k+1) with the update of step k */
 #define TTG_CHOL_F_GRAPH 0x2u     /* capture the launch sequence once in a hipGraph and replay it */
+#define TTG_CHOL_F_PROFILE 0x4u   /* time every launch with HIP events (launches eagerly, no graph) */
 #define TTG_CHOL_F_DEFAULT (TTG_CHOL_F_LOOKAHEAD | TTG_CHOL_F_GRAPH)
 
 /* FP64 tiled Cholesky of the N x N SPD matrix held in `tiles`, in place.
@@ -152,6 +153,29 @@ int ttg_chol_plgsy_tiled(void *stream, int N, int NB, double bump, unsigned long
  * (No counterpart in the reference: the benchmark's SPD construction.) */
 int ttg_chol_spd_tiled(void *stream, int N, int NB, double alpha, unsigned long long seed, double *tiles,
                        double *work);
+
+/* ---------------------------------------------------------------- */
+/* diagnostics                                                       */
+/* ---------------------------------------------------------------- */
+
+/* Kernel kinds timed by TTG_CHOL_F_PROFILE runs. */
+#define TTG_CHOL_PROF_UPDATE 0 /* batched SYRK+GEMM trailing update */
+#define TTG_CHOL_PROF_TRSM 1   /* batched panel TRSM */
+#define TTG_CHOL_PROF_POTRF 2  /* diagonal-tile POTRF */
+
+/* Sum of the event-timed durations (ms), the launch count and the
+ * algorithmic flops (DPLASMA counts: GEMM 2b^3, SYRK b^2(b+1), TRSM b^3,
+ * POTRF b^3/3 per tile) of every launch of `kind` recorded by profiled
+ * driver calls since the last read.  Waits for those launches to finish.
+ * (No counterpart in the reference.) */
+int ttg_chol_profile_read(int kind, double *ms_total, long long *launches, double *flops);
+
+/* Lane-layout self test of the 16x16x4 MFMA instructions the kernels use
+ * (kind 0: FP64, kind 1: FP32).  `out` (device, 64*64*4 doubles) receives,
+ * for each one-hot B lane m, the four outputs of every lane.  The kernels'
+ * operand/result index maps are checked against it by the test suite.
+ * (No counterpart in the reference.) */
+int ttg_chol_selftest_mfma_layout(void *stream, int kind, double *out);
 
 #ifdef __cplusplus
 }

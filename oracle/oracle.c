@@ -163,34 +163,63 @@ int oracle_spd_bint(long i, long j, long N, unsigned long long seed) {
   return (int)((hi * 255ULL) >> 32) - 127;
 }
 
+static void gemm_nt_acc(int m, int n, int k, double alpha, const double *restrict A, int lda,
+                        const double *restrict B, int ldb, double *restrict C, int ldc);
+static int host_cores(void);
+
+typedef struct {
+  int N, NB, nt, nthreads, tid;
+  double alpha;
+  const double *B;
+  double *tiles;
+} spd_arg;
+
+static void *spd_worker(void *p) {
+  spd_arg *a = (spd_arg *)p;
+  const int N = a->N, NB = a->NB, nt = a->nt;
+  const double c = 1.0 / (127.0 * 127.0 * (double)N);
+  long idx = 0;
+  for (int n = 0; n < nt; n++)
+    for (int m = n; m < nt; m++, idx++) {
+      if (idx % a->nthreads != a->tid) continue;
+      double *t = a->tiles + oracle_tile_index(m, n, nt) * (long)NB * NB;
+      int mb = N - m * NB < NB ? N - m * NB : NB, nb = N - n * NB < NB ? N - n * NB : NB;
+      for (int j = 0; j < nb; j++)
+        for (int i = 0; i < mb; i++) t[i + (long)j * NB] = 0.0;
+      /* integer-valued operands: every product and partial sum is an exact
+       * integer below 2^53, so the blocked kernel's order does not matter */
+      gemm_nt_acc(mb, nb, N, 1.0, a->B + (long)m * NB, N, a->B + (long)n * NB, N, t, NB);
+      for (int j = 0; j < nb; j++)
+        for (int i = 0; i < mb; i++) {
+          volatile double v = t[i + (long)j * NB] * c;
+          if (m == n && i == j) v = v + a->alpha;
+          t[i + (long)j * NB] = v;
+        }
+    }
+  return NULL;
+}
+
 void oracle_spd_tiled(int N, int NB, double alpha, unsigned long long seed, double *tiles) {
   int nt = (N + NB - 1) / NB;
-  /* integer B, column-major, generated column by column from the LCG */
-  int *B = (int *)malloc(sizeof(int) * (size_t)N * N);
+  /* integer B (as doubles), column-major, generated column by column from the LCG */
+  double *B = (double *)malloc(sizeof(double) * (size_t)N * N);
   for (long j = 0; j < N; j++) {
     unsigned long long ran = oracle_rnd64_jump((unsigned long long)(j * N), seed);
     for (long i = 0; i < N; i++) {
-      B[i + j * N] = (int)((((ran >> 32) * 255ULL) >> 32)) - 127;
+      B[i + j * N] = (double)((int)((((ran >> 32) * 255ULL) >> 32)) - 127);
       ran = ORACLE_RND64_A * ran + ORACLE_RND64_C;
     }
   }
-  const double c = 1.0 / (127.0 * 127.0 * (double)N);
-  for (int n = 0; n < nt; n++)
-    for (int m = n; m < nt; m++) {
-      double *t = tile_ptr(tiles, m, n, nt, NB);
-      int mb = tile_dim(N, NB, m), nb = tile_dim(N, NB, n);
-      for (int jj = 0; jj < nb; jj++) {
-        long gj = (long)n * NB + jj;
-        for (int ii = 0; ii < mb; ii++) {
-          long gi = (long)m * NB + ii;
-          long long s = 0;
-          for (long k = 0; k < N; k++) s += (long long)B[gi + k * N] * B[gj + k * N];
-          volatile double v = (double)s * c;
-          if (gi == gj) v = v + alpha;
-          t[ii + (long)jj * NB] = v;
-        }
-      }
-    }
+  int nth = host_cores();
+  pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nth);
+  spd_arg *args = (spd_arg *)malloc(sizeof(spd_arg) * nth);
+  for (int t = 0; t < nth; t++) {
+    args[t] = (spd_arg){N, NB, nt, nth, t, alpha, B, tiles};
+    pthread_create(&th[t], NULL, spd_worker, &args[t]);
+  }
+  for (int t = 0; t < nth; t++) pthread_join(th[t], NULL);
+  free(th);
+  free(args);
   free(B);
 }
 

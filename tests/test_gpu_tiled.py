@@ -1,0 +1,176 @@
+"""Tiled drivers on the GPU: the FP64 factorization of the examples/potrf
+graph with grouped launches, the generators, solves and residuals, against
+the oracle.  FP64 bar: max |gpu - oracle| / max |oracle| <= 1e-12 over the
+lower tiles; bit-exact for the generators."""
+import numpy as np
+import pytest
+import torch
+
+from gpu_util import lower_tiles_rel_err, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+@pytest.mark.parametrize("N,NB", [(160, 32), (1000, 128), (2048, 512)])
+def test_plgsy_bit_exact(dev, oracle, N, NB):
+    import ttg_amd.chol as C
+
+    want = oracle.plgsy_tiles(N, NB, N, 3872)
+    t = torch.full((want.size,), np.nan, dtype=torch.float64, device=dev)
+    C.plgsy_tiled(t, N, NB, N, 3872)
+    got = to_host(t)
+    nt = oracle.nt_of(N, NB)
+    for n in range(nt):
+        for m in range(n, nt):
+            mb, nb = min(NB, N - m * NB), min(NB, N - n * NB)
+            a = oracle.tile_view(got, N, NB, m, n)[:mb, :nb]
+            b = oracle.tile_view(want, N, NB, m, n)[:mb, :nb]
+            assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), (m, n)
+
+
+@pytest.mark.parametrize("N,NB,alpha", [(256, 64, 1.0), (300, 128, 1e-3), (1024, 256, 1.0)])
+def test_spd_bit_exact(dev, oracle, N, NB, alpha):
+    import ttg_amd.chol as C
+
+    want = oracle.spd_tiles(N, NB, alpha, 42)
+    t = torch.zeros(want.size, dtype=torch.float64, device=dev)
+    work = torch.empty(N * N, dtype=torch.float64, device=dev)
+    C.spd_tiled(t, N, NB, alpha, 42, work)
+    got = to_host(t)
+    nt = oracle.nt_of(N, NB)
+    for n in range(nt):
+        for m in range(n, nt):
+            mb, nb = min(NB, N - m * NB), min(NB, N - n * NB)
+            a = oracle.tile_view(got, N, NB, m, n)[:mb, :nb]
+            b = oracle.tile_view(want, N, NB, m, n)[:mb, :nb]
+            if m == n:
+                a, b = np.tril(a), np.tril(b)
+            assert np.array_equal(a, b), (m, n)
+
+
+@pytest.mark.parametrize("N,NB,gen", [(160, 32, "plgsy"), (1000, 128, "spd"), (2048, 256, "spd"),
+                                      (4096, 512, "spd"), (1536, 512, "plgsy")])
+def test_dpotrf_tiled_vs_oracle(dev, oracle, N, NB, gen):
+    import ttg_amd.chol as C
+
+    A0 = oracle.plgsy_tiles(N, NB, N, 3872) if gen == "plgsy" else oracle.spd_tiles(N, NB, 1.0, 5)
+    ref = A0.copy()
+    assert oracle.potrf_tiled(N, NB, ref) == 0
+    t = to_dev(A0, dev)
+    info = torch.full((1,), -3, dtype=torch.int32, device=dev)
+    C.dpotrf_tiled(t, N, NB, info)
+    got = to_host(t)
+    assert int(info.item()) == 0
+    assert lower_tiles_rel_err(oracle, N, NB, got, ref) <= TOL
+    assert oracle.check_dpotrf_tiled(N, NB, got, A0) <= 60  # testing_dpotrf.cc:315-324
+
+
+def test_dpotrf_tiled_flags_bitwise_identical(dev, oracle):
+    """Lookahead and graph capture change only the launch schedule: every tile
+    sees the same kernels in the same order, so the factor is bitwise equal."""
+    import ttg_amd.chol as C
+
+    N, NB = 3072, 512
+    A0 = to_dev(oracle.spd_tiles(N, NB, 1.0, 8), dev)
+    outs = []
+    for flags in (0, C.F_LOOKAHEAD, C.F_GRAPH, C.F_DEFAULT):
+        for rep in range(2):  # a captured graph is replayed on the second call
+            t = A0.clone()
+            info = torch.full((1,), 5, dtype=torch.int32, device=dev)
+            C.dpotrf_tiled(t, N, NB, info, flags=flags)
+            torch.cuda.synchronize()
+            assert int(info.item()) == 0
+            outs.append(t.cpu())
+    for o in outs[1:]:
+        assert torch.equal(o, outs[0])
+
+
+def test_dpotrf_tiled_replay_same_buffer(dev, oracle):
+    import ttg_amd.chol as C
+
+    N, NB = 2048, 512
+    A0 = oracle.spd_tiles(N, NB, 1.0, 9)
+    t = to_dev(A0, dev)
+    info = torch.zeros(1, dtype=torch.int32, device=dev)
+    first = None
+    for _ in range(3):
+        t.copy_(to_dev(A0, dev))
+        C.dpotrf_tiled(t, N, NB, info)
+        got = to_host(t)
+        if first is None:
+            first = got
+        assert np.array_equal(got, first)
+
+
+@pytest.mark.parametrize("N,NB,k,local", [(1024, 256, 2, 17), (2048, 512, 3, 500)])
+def test_dpotrf_tiled_info_nonspd(dev, oracle, N, NB, k, local):
+    import ttg_amd.chol as C
+
+    A0 = oracle.spd_tiles(N, NB, 1.0, 3)
+    oracle.tile_view(A0, N, NB, k, k)[local, local] = -1e6
+    want = oracle.potrf_tiled(N, NB, A0.copy())
+    assert want == k * NB + local + 1
+    t = to_dev(A0, dev)
+    info = torch.zeros(1, dtype=torch.int32, device=dev)
+    C.dpotrf_tiled(t, N, NB, info)
+    assert int(info.item()) == want
+    # a later good factorization on the same plan resets info
+    t.copy_(to_dev(oracle.spd_tiles(N, NB, 1.0, 4), dev))
+    C.dpotrf_tiled(t, N, NB, info)
+    assert int(info.item()) == 0
+
+
+@pytest.mark.parametrize("N,NB,nrhs", [(160, 32, 1), (1000, 128, 3), (2048, 512, 16), (1536, 256, 40)])
+def test_dpotrs_and_residual_vs_oracle(dev, oracle, N, NB, nrhs):
+    import ttg_amd.chol as C
+
+    A0 = oracle.spd_tiles(N, NB, 0.5, 21)
+    L = A0.copy()
+    assert oracle.potrf_tiled(N, NB, L) == 0
+    rng = np.random.default_rng(N + nrhs)
+    B = np.asfortranarray(rng.uniform(-1, 1, (N, nrhs)))
+    Xref = oracle.potrs_tiled(N, NB, L, B)
+    dL = to_dev(L, dev)
+    dB = torch.from_numpy(np.ascontiguousarray(B.T)).to(dev).t()  # column-major N x nrhs
+    X = dB.clone().t().contiguous().t()
+    C.dpotrs_tiled(dL, N, NB, X)
+    Xg = X.t().cpu().numpy().T
+    assert np.abs(Xg - Xref).max() / np.abs(Xref).max() <= TOL
+    dA = to_dev(A0, dev)
+    R = torch.empty_like(X)
+    C.dresidual_tiled(dA, N, NB, X, dB, R)
+    Rg = R.t().cpu().numpy().T
+    Rref = oracle.residual_tiled(N, NB, A0, Xg, B)
+    assert np.abs(Rg - Rref).max() <= 1e-13
+    assert oracle.solve_backward_error(N, NB, A0, Xg, B) <= 1e-14
+
+
+def test_large_factor_solve_backward_error(dev):
+    """Size-independent property at a realistic size: factor N=16384, solve,
+    and the FP64 residual gives a backward error at the level of eps."""
+    import ttg_amd.chol as C
+
+    N, NB, nrhs = 16384, 512, 4
+    t = C.alloc_tiles(N, NB)
+    work = torch.empty(N * N, dtype=torch.float64, device=dev)
+    C.spd_tiled(t, N, NB, 1.0, 77, work)
+    del work
+    A0 = t.clone()
+    info = torch.zeros(1, dtype=torch.int32, device=dev)
+    C.dpotrf_tiled(t, N, NB, info)
+    assert int(info.item()) == 0
+    g = torch.Generator(device="cpu").manual_seed(1)
+    B = (torch.rand(nrhs, N, generator=g, dtype=torch.float64) * 2 - 1).to(dev).t()
+    X = B.clone().t().contiguous().t()
+    C.dpotrs_tiled(t, N, NB, X)
+    R = torch.empty_like(X)
+    C.dresidual_tiled(A0, N, NB, X, B, R)
+    torch.cuda.synchronize()
+    # ||A||_inf <= N * max|a_ij| ~ N/3 + 1 for this construction; use the exact bound via row sums later
+    rn = R.abs().max(dim=0).values
+    xn = X.abs().max(dim=0).values
+    bn = B.abs().max(dim=0).values
+    anorm = 1.0 + N * (1.0 / 3.0 + 0.05)  # upper bound of ||A||_inf for B B^T/N + I with |b| <= 1
+    berr = (rn / (anorm * xn + bn)).max().item()
+    assert berr <= 1e-15

@@ -15,6 +15,8 @@ SOURCES = [
     "chol_kernels.hip",
     "chol_gen.hip",
     "chol_tiled.hip",
+    "chol_solve.hip",
+    "chol_profile.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

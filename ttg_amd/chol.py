@@ -12,7 +12,8 @@ LIB_PATH = os.path.join(HERE, "libttg_chol.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "ttg_chol.h")
 
 OK, EINVAL, EHIP, ENOTSUP, ENOMEM, ESTATE = 0, -1, -2, -3, -4, -5
-F_LOOKAHEAD, F_GRAPH = 0x1, 0x2
+F_LOOKAHEAD, F_GRAPH, F_PROFILE = 0x1, 0x2, 0x4
+PROF_UPDATE, PROF_TRSM, PROF_POTRF = 0, 1, 2
 F_DEFAULT = F_LOOKAHEAD | F_GRAPH
 MAX_TILE = 512
 
@@ -36,6 +37,8 @@ SIGNATURES = {
     "ttg_chol_dresidual_tiled": (_i, [_p, _i, _i, _p, _i, _p, _i, _p, _i, _p, _i]),
     "ttg_chol_plgsy_tiled": (_i, [_p, _i, _i, _d, _ull, _p]),
     "ttg_chol_spd_tiled": (_i, [_p, _i, _i, _d, _ull, _p, _p]),
+    "ttg_chol_selftest_mfma_layout": (_i, [_p, _i, _p]),
+    "ttg_chol_profile_read": (_i, [_i, _p, _p, _p]),
 }
 
 
@@ -91,8 +94,10 @@ def _ld(t):
     """Leading dimension of a column-major 2-D tensor (a transposed row-major one)."""
     if t.dim() == 1:
         return max(1, t.shape[0])
-    if t.stride(0) != 1:
+    if t.shape[0] > 1 and t.stride(0) != 1:
         raise CholError("tensor must be column-major (stride(0) == 1)")
+    if t.shape[1] == 1:
+        return max(1, t.shape[0])
     return max(1, t.stride(1))
 
 
@@ -195,6 +200,20 @@ def dresidual_tiled(tiles, N, NB, X, B, R, stream=None):
     nrhs = 1 if B.dim() == 1 else B.shape[1]
     _check(lib().ttg_chol_dresidual_tiled(_stream(stream), N, NB, _ptr(tiles), nrhs, _ptr(X), _ld(X), _ptr(B),
                                           _ld(B), _ptr(R), _ld(R)), "dresidual_tiled")
+
+
+def selftest_mfma_layout(kind, out, stream=None):
+    _require_cuda(out)
+    _check(lib().ttg_chol_selftest_mfma_layout(_stream(stream), kind, _ptr(out)), "selftest_mfma_layout")
+
+
+def profile_read(kind):
+    """(total ms, launches, algorithmic flops) of the profiled launches of `kind`."""
+    ms = ctypes.c_double()
+    n = ctypes.c_longlong()
+    fl = ctypes.c_double()
+    _check(lib().ttg_chol_profile_read(kind, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(fl)), "profile_read")
+    return ms.value, n.value, fl.value
 
 
 def plgsy_tiled(tiles, N, NB, bump, seed, stream=None):

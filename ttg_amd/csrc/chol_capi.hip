@@ -94,14 +94,6 @@ static thread_local Group g_group;
 
 static bool grouping(hipStream_t s) { return g_group.active && g_group.stream == s; }
 
-static bool upd_fast(const UpdOp<double> &op) {
-  constexpr int bk = update_bk<double>();
-  return op.m % 128 == 0 && op.n % 128 == 0 && op.k % bk == 0 && op.lda % 2 == 0 && op.ldb % 2 == 0 &&
-         op.ldc % 2 == 0 && ((reinterpret_cast<uintptr_t>(op.A) | reinterpret_cast<uintptr_t>(op.B) |
-                              reinterpret_cast<uintptr_t>(op.C)) &
-                             15) == 0;
-}
-
 }  // namespace ttgc
 
 using namespace ttgc;
@@ -134,7 +126,7 @@ int ttg_chol_dpotrf_tile(void *stream, int n, double *A, int lda, int *d_info, i
     g_group.potrf.push_back(op);
     return TTG_CHOL_OK;
   }
-  TTGC_HIP(launch_potrf(s, nullptr, 1, op, nullptr));
+  TTGC_HIP(launch_potrf<double>(s, nullptr, 1, op, nullptr));
   return TTG_CHOL_OK;
 }
 
@@ -158,7 +150,7 @@ int ttg_chol_dtrsm_rlt(void *stream, int m, int n, const double *L, int ldl, dou
   TTGC_HIP(w.dinv.reserve(sizeof(double) * dinv_doubles(n)));
   dj.dinv = static_cast<double *>(w.dinv.ptr);
   op.dinv = dj.dinv;
-  TTGC_HIP(launch_dinv(s, nullptr, 1, dj));
+  TTGC_HIP(launch_dinv<double>(s, nullptr, 1, dj));
   TTGC_HIP(launch_trsm<double>(s, nullptr, 1, op, m, nullptr));
   return TTG_CHOL_OK;
 }
@@ -180,7 +172,7 @@ static int update_entry(const char *name, hipStream_t s, int m, int n, int k, do
     g_group.upd[UpdKey{alpha, beta}].push_back(op);
     return TTG_CHOL_OK;
   }
-  TTGC_HIP(launch_update<double>(s, nullptr, 1, op, m, n, alpha, beta, upd_fast(op), nullptr));
+  TTGC_HIP(launch_update<double>(s, nullptr, 1, op, m, n, alpha, beta, update_fast(op, m), nullptr));
   return TTG_CHOL_OK;
 }
 
@@ -297,8 +289,8 @@ int ttg_chol_group_end(void *stream) {
       for (auto &u : kv.second) {
         max_m = std::max(max_m, u.m);
         max_n = std::max(max_n, u.n);
-        fast = fast && upd_fast(u);
       }
+      for (auto &u : kv.second) fast = fast && update_fast(u, max_m);
       UpdOp<double> unone{};
       TTGC_HIP(launch_update<double>(s, reinterpret_cast<UpdOp<double> *>(base + o_upd[i]), (int)kv.second.size(),
                                      unone, max_m, max_n, kv.first.alpha, kv.first.beta, fast, nullptr));

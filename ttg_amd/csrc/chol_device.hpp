@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace ttgc {
 
 constexpr int kWave = 64;
@@ -20,7 +22,9 @@ typedef float f4_t __attribute__((ext_vector_type(4)));
 // MFMA 16x16x4 (FP64 and FP32).  Operand layout (64 lanes, l = lane):
 //   A fragment 16 x 4:  lane l holds A[i = l % 16][k = l / 16]
 //   B fragment 4 x 16:  lane l holds B[k = l / 16][j = l % 16]
-//   D 16 x 16:          lane l, element r holds D[i = 4*(l/16) + r][j = l % 16]
+//   D 16 x 16:          lane l, element r holds D[i = drow(l / 16, r)][j = l % 16]
+// with drow(g, r) = g + 4 r for the FP64 instruction and 4 g + r for FP32
+// (measured on gfx950 by the lane-layout test).
 // ----------------------------------------------------------------------------
 template <typename T>
 struct Mfma;
@@ -28,6 +32,7 @@ struct Mfma;
 template <>
 struct Mfma<double> {
   using acc_t = d4_t;
+  static __host__ __device__ constexpr int drow(int g, int r) { return g + 4 * r; }
   static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
@@ -36,6 +41,7 @@ struct Mfma<double> {
 template <>
 struct Mfma<float> {
   using acc_t = f4_t;
+  static __host__ __device__ constexpr int drow(int g, int r) { return 4 * g + r; }
   static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
@@ -114,14 +120,58 @@ struct StageLoader {
   }
 };
 
+// Transposed stage loader: the operand element (j, k) is read from
+// base[k + j*ld] (k contiguous), for j in [row0, row0+ROWS), k in
+// [col0, col0+COLS), and stored to LDS as s[k][j].
+template <typename T, int ROWS, int COLS, int NTH, int LD>
+struct StageLoaderT {
+  static constexpr int V = Vec16<T>::n;
+  using vec_t = typename Vec16<T>::type;
+  static constexpr int VPR = COLS / V;  // vectors per row j
+  static_assert(COLS % V == 0, "cols must be a multiple of the vector width");
+  static_assert(NTH % VPR == 0, "threads must cover whole rows");
+  static constexpr int RPP = NTH / VPR;  // rows per pass
+  static_assert(ROWS % RPP == 0, "rows must be a multiple of rows per pass");
+  static constexpr int PASSES = ROWS / RPP;
+
+  vec_t r[PASSES];
+
+  template <bool FAST>
+  __device__ __forceinline__ void load(const T *__restrict__ base, long ld, int row0, int col0, int rlim,
+                                       int clim, int tid) {
+    const int v = tid % VPR, j = tid / VPR;
+    const int k = col0 + v * V;
+#pragma unroll
+    for (int p = 0; p < PASSES; ++p) {
+      const int row = row0 + j + p * RPP;
+      if constexpr (FAST) {
+        r[p] = *reinterpret_cast<const vec_t *>(base + k + (long)row * ld);
+      } else {
+#pragma unroll
+        for (int e = 0; e < V; ++e)
+          r[p][e] = (row < rlim && k + e < clim) ? base[k + e + (long)row * ld] : T(0);
+      }
+    }
+  }
+
+  __device__ __forceinline__ void store(T (*s)[LD], int tid) const {
+    const int v = tid % VPR, j = tid / VPR;
+#pragma unroll
+    for (int p = 0; p < PASSES; ++p)
+#pragma unroll
+      for (int e = 0; e < V; ++e) s[v * V + e][j + p * RPP] = r[p][e];
+  }
+};
+
 // ----------------------------------------------------------------------------
 // GEMM core: acc (BM x BN block, spread over WM x WN waves) += A * B^T where
 // A is rows [i0, i0+BM) and B is rows [j0, j0+BN) of two column-major
-// matrices, summed over k in [kbeg, kend).  Both operands are staged through
+// matrices, summed over k in [kbeg, kend) (TB: B's element (j, k) is read
+// from B[k + j*ldb] instead).  Both operands are staged through
 // LDS in BK-wide slabs, double buffered with a register prefetch of the next
 // slab while MFMAs consume the current one.
 // ----------------------------------------------------------------------------
-template <typename T, int BM, int BN, int BK, int WM, int WN>
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool TB = false>
 struct GemmCore {
   static constexpr int NW = WM * WN;
   static constexpr int NTH = NW * kWave;
@@ -168,14 +218,23 @@ struct GemmCore {
   template <bool FAST>
   __device__ __forceinline__ void run(Smem &sm, const T *__restrict__ A, long lda, const T *__restrict__ B,
                                       long ldb, int i0, int j0, int kbeg, int kend, int mlim, int nlim) {
+    run2<FAST, FAST>(sm, A, lda, B, ldb, i0, j0, kbeg, kend, mlim, nlim);
+  }
+
+  // FA / FB: the fast (unpredicated, vector) path for the A / B operand.
+  template <bool FA, bool FB>
+  __device__ __forceinline__ void run2(Smem &sm, const T *__restrict__ A, long lda, const T *__restrict__ B,
+                                       long ldb, int i0, int j0, int kbeg, int kend, int mlim, int nlim) {
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     const int wm = wave / WN, wn = wave % WN;
     StageLoader<T, BM, BK, NTH, LDA_S> la;
-    StageLoader<T, BN, BK, NTH, LDB_S> lb;
+    using LoaderB = typename std::conditional<TB, StageLoaderT<T, BN, BK, NTH, LDB_S>,
+                                              StageLoader<T, BN, BK, NTH, LDB_S>>::type;
+    LoaderB lb;
     const int nk = (kend - kbeg + BK - 1) / BK;
     if (nk <= 0) return;
-    la.template load<FAST>(A, lda, i0, kbeg, mlim, kend, tid);
-    lb.template load<FAST>(B, ldb, j0, kbeg, nlim, kend, tid);
+    la.template load<FA>(A, lda, i0, kbeg, mlim, kend, tid);
+    lb.template load<FB>(B, ldb, j0, kbeg, nlim, kend, tid);
     la.store(sm.a[0], tid);
     lb.store(sm.b[0], tid);
     __syncthreads();
@@ -184,8 +243,8 @@ struct GemmCore {
       const bool more = kt + 1 < nk;
       if (more) {
         const int k0 = kbeg + (kt + 1) * BK;
-        la.template load<FAST>(A, lda, i0, k0, mlim, kend, tid);
-        lb.template load<FAST>(B, ldb, j0, k0, nlim, kend, tid);
+        la.template load<FA>(A, lda, i0, k0, mlim, kend, tid);
+        lb.template load<FB>(B, ldb, j0, k0, nlim, kend, tid);
       }
       compute(sm, cur, wm, wn, lane);
       if (more) {
@@ -199,7 +258,7 @@ struct GemmCore {
   // Row / column (relative to the block origin) of accumulator element r of
   // MFMA tile (i, j) held by this lane.
   static __device__ __forceinline__ int row_of(int wm, int i, int lane, int r) {
-    return wm * TM + i * 16 + 4 * (lane >> 4) + r;
+    return wm * TM + i * 16 + Mfma<T>::drow(lane >> 4, r);
   }
   static __device__ __forceinline__ int col_of(int wn, int j, int lane) { return wn * TN + j * 16 + (lane & 15); }
 };

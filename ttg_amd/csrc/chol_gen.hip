@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "chol_device.hpp"
 #include "chol_internal.hpp"
 #include "chol_kernels.hpp"
 #include "chol_tiled.hpp"
@@ -148,6 +149,19 @@ __global__ __launch_bounds__(1024) void nrm2_kernel(int n, const double *x, doub
   if (threadIdx.x == 0) *out = sqrt(part[0]);
 }
 
+// Lane-layout self test of the 16x16x4 MFMA builtins: block m feeds A lane l
+// the value l + 1 and B a one-hot 1 in lane m, and stores all 4 outputs of
+// every lane to out[(m * 64 + l) * 4 + r].
+template <typename T>
+__global__ __launch_bounds__(64) void mfma_probe_kernel(double *out) {
+  const int m = blockIdx.x, l = threadIdx.x;
+  const T a = T(l + 1), b = (l == m) ? T(1) : T(0);
+  typename Mfma<T>::acc_t d = {0, 0, 0, 0};
+  d = Mfma<T>::mma(a, b, d);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) out[(m * 64 + l) * 4 + r] = (double)d[r];
+}
+
 hipError_t launch_prepare(hipStream_t s, int N, int NB, double *tiles, int *d_info) {
   const int nt = (N + NB - 1) / NB;
   const long work = (long)NB * NB * nt;
@@ -197,7 +211,7 @@ int ttg_chol_spd_tiled(void *stream, int N, int NB, double alpha, unsigned long 
       const int mb = std::min(NB, N - m * NB), nb = std::min(NB, N - n * NB);
       UpdOp<double> op{work + (long)m * NB, work + (long)n * NB, tiles + tile_index(m, n, nt) * (long)NB * NB,
                        mb, nb, N, N, N, NB, m == n ? 1 : 0};
-      const bool f = mb % 128 == 0 && nb % 128 == 0 && N % update_bk<double>() == 0 && N % 2 == 0 && NB % 2 == 0;
+      const bool f = update_fast(op, NB);
       (f ? fast : slow).push_back(op);
     }
   const double c = 1.0 / (127.0 * 127.0 * (double)N);
@@ -217,6 +231,20 @@ int ttg_chol_spd_tiled(void *stream, int N, int NB, double alpha, unsigned long 
   // the descriptor buffer is freed when this call returns: wait for its users
   TTGC_HIP(hipStreamSynchronize(s));
   (void)hipFree(ops.ptr);
+  return TTG_CHOL_OK;
+}
+
+int ttg_chol_selftest_mfma_layout(void *stream, int kind, double *out) {
+  clear_error();
+  if (!out || (kind != 0 && kind != 1)) {
+    set_error("selftest_mfma_layout: bad argument (kind=%d)", kind);
+    return TTG_CHOL_EINVAL;
+  }
+  if (kind == 0)
+    hipLaunchKernelGGL(mfma_probe_kernel<double>, dim3(64), dim3(64), 0, as_stream(stream), out);
+  else
+    hipLaunchKernelGGL(mfma_probe_kernel<float>, dim3(64), dim3(64), 0, as_stream(stream), out);
+  TTGC_HIP(hipGetLastError());
   return TTG_CHOL_OK;
 }
 

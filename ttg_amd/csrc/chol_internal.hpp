@@ -20,10 +20,10 @@ inline int hip_fail(hipError_t e, const char *what) {
   return TTG_CHOL_EHIP;
 }
 
-#define TTGC_HIP(call)                                  \
-  do {                                                  \
-    hipError_t _e = (call);                             \
-    if (_e != hipSuccess) return ttgc::hip_fail(_e, #call); \
+#define TTGC_HIP(...)                                           \
+  do {                                                          \
+    hipError_t _e = (__VA_ARGS__);                              \
+    if (_e != hipSuccess) return ttgc::hip_fail(_e, #__VA_ARGS__); \
   } while (0)
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }

@@ -24,10 +24,11 @@ constexpr int kPanelLd = kMaxTile - kPB + 4;  // LDS panel row stride
 // ---------------------------------------------------------------------------
 // POTRF
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void potrf_kernel(const PotrfOp *__restrict__ ops, PotrfOp one,
+template <typename TS>
+__global__ __launch_bounds__(256) void potrf_kernel(const PotrfOpT<TS> *__restrict__ ops, PotrfOpT<TS> one,
                                                     const int *__restrict__ d_abort) {
   if (d_abort && *d_abort) return;
-  const PotrfOp op = ops ? ops[blockIdx.x] : one;
+  const PotrfOpT<TS> op = ops ? ops[blockIdx.x] : one;
   __shared__ double Ds[kPB][kPB + 1];
   __shared__ double Di[kPB][kPB + 1];
   __shared__ __align__(16) double Ps[kPB][kPanelLd];
@@ -35,7 +36,7 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOp *__restrict__ 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int n = op.n;
   const long lda = op.lda;
-  double *__restrict__ A = op.A;
+  TS *__restrict__ A = op.A;
   int fail = 0;
   const int nblk = (n + kPB - 1) / kPB;
 
@@ -46,7 +47,7 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOp *__restrict__ 
       const int r = e & 31, c = e >> 5;
       double v;
       if (r < bs && c < bs)
-        v = (r >= c) ? A[c0 + r + (long)(c0 + c) * lda] : 0.0;
+        v = (r >= c) ? (double)A[c0 + r + (long)(c0 + c) * lda] : 0.0;
       else
         v = (r == c) ? 1.0 : 0.0;
       Ds[r][c] = v;
@@ -111,11 +112,11 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOp *__restrict__ 
     }
     for (int e = tid; e < kPB * kPB; e += 256) {
       const int r = e & 31, c = e >> 5;
-      if (r >= c && r < bs && c < bs) A[c0 + r + (long)(c0 + c) * lda] = Ds[r][c];
+      if (r >= c && r < bs && c < bs) A[c0 + r + (long)(c0 + c) * lda] = (TS)Ds[r][c];
     }
     __syncthreads();
     if (op.dinv)
-      for (int e = tid; e < kPB * kPB; e += 256) op.dinv[jb * kPB * kPB + e] = Di[e & 31][e >> 5];
+      for (int e = tid; e < kPB * kPB; e += 256) op.dinv[jb * kPB * kPB + e] = (TS)Di[e & 31][e >> 5];
 
     // 4. panel rows below the block: P = A(:, c0:c0+32) * Di^T
     const int R = n - c0 - kPB;
@@ -123,13 +124,13 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOp *__restrict__ 
       const long i = c0 + kPB + rr;
       double a[kPB];
 #pragma unroll
-      for (int k = 0; k < kPB; ++k) a[k] = A[i + (long)(c0 + k) * lda];
+      for (int k = 0; k < kPB; ++k) a[k] = (double)A[i + (long)(c0 + k) * lda];
 #pragma unroll
       for (int c = 0; c < kPB; ++c) {
         double s = 0.0;
 #pragma unroll
         for (int k = 0; k <= c; ++k) s += a[k] * Di[c][k];
-        A[i + (long)(c0 + c) * lda] = s;
+        A[i + (long)(c0 + c) * lda] = (TS)s;
         Ps[c][rr] = s;
       }
     }
@@ -145,12 +146,12 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOp *__restrict__ 
         while (bi * (bi + 1) / 2 > q) --bi;
         const int bj = q - bi * (bi + 1) / 2;
         const int col = c0 + kPB + bj * 16 + lr;
-        const int rowb = c0 + kPB + bi * 16 + 4 * lk;
+        const int rowb = c0 + kPB + bi * 16;
         double old[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int row = rowb + r;
-          old[r] = (row < n && col < n) ? A[row + (long)col * lda] : 0.0;
+          const int row = rowb + Mfma<double>::drow(lk, r);
+          old[r] = (row < n && col < n) ? (double)A[row + (long)col * lda] : 0.0;
         }
         d4_t acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -161,8 +162,8 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOp *__restrict__ 
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int row = rowb + r;
-          if (row < n && col < n && (bi != bj || row >= col)) A[row + (long)col * lda] = old[r] - acc[r];
+          const int row = rowb + Mfma<double>::drow(lk, r);
+          if (row < n && col < n && (bi != bj || row >= col)) A[row + (long)col * lda] = (TS)(old[r] - acc[r]);
         }
       }
     }
@@ -177,14 +178,15 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOp *__restrict__ 
 }
 
 // inv(L_jj) of every 32 x 32 diagonal block of an already factored tile.
-__global__ __launch_bounds__(256) void dinv_kernel(const PotrfOp *__restrict__ ops, PotrfOp one) {
-  const PotrfOp op = ops ? ops[blockIdx.x] : one;
+template <typename TS>
+__global__ __launch_bounds__(256) void dinv_kernel(const PotrfOpT<TS> *__restrict__ ops, PotrfOpT<TS> one) {
+  const PotrfOpT<TS> op = ops ? ops[blockIdx.x] : one;
   const int n = op.n;
   const long lda = op.lda;
   const int nblk = (n + kPB - 1) / kPB;
   for (int t = threadIdx.x; t < nblk * kPB; t += blockDim.x) {
     const int jb = t / kPB, c = t % kPB, c0 = jb * kPB, bs = min(kPB, n - c0);
-    const double *L = op.A + c0 + (long)c0 * lda;
+    const TS *L = op.A + c0 + (long)c0 * lda;
     double x[kPB];
 #pragma unroll
     for (int i = 0; i < kPB; ++i) {
@@ -194,25 +196,27 @@ __global__ __launch_bounds__(256) void dinv_kernel(const PotrfOp *__restrict__ o
       } else if (i < c || i >= bs) {
         v = 0.0;
       } else if (i == c) {
-        v = 1.0 / L[c + (long)c * lda];
+        v = 1.0 / (double)L[c + (long)c * lda];
       } else {
         double s = 0.0;
 #pragma unroll
         for (int k = 0; k < i; ++k)
-          if (k >= c) s += L[i + (long)k * lda] * x[k];
-        v = -s / L[i + (long)i * lda];
+          if (k >= c) s += (double)L[i + (long)k * lda] * x[k];
+        v = -s / (double)L[i + (long)i * lda];
       }
       x[i] = v;
     }
 #pragma unroll
-    for (int i = 0; i < kPB; ++i) op.dinv[jb * kPB * kPB + i + c * kPB] = x[i];
+    for (int i = 0; i < kPB; ++i) op.dinv[jb * kPB * kPB + i + c * kPB] = (TS)x[i];
   }
 }
 
 // ---------------------------------------------------------------------------
-// TRSM  B := B L^{-T}
+// TRSM  B := B L^{-T}  (TRANS) or  B := B L^{-1}  (!TRANS)
+//   TRANS : column blocks left to right,  X_j = (B_j - X_{<j} L_{j,<j}^T) inv(L_jj)^T
+//   !TRANS: column blocks right to left,  X_j = (B_j - X_{>j} L_{>j,j}) inv(L_jj)
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, bool TRANS>
 __global__ __launch_bounds__(256) void trsm_kernel(const TrsmOp<T> *__restrict__ ops, TrsmOp<T> one, int bpo,
                                                    const int *__restrict__ d_abort) {
   if (d_abort && *d_abort) return;
@@ -220,7 +224,7 @@ __global__ __launch_bounds__(256) void trsm_kernel(const TrsmOp<T> *__restrict__
   const TrsmOp<T> op = ops ? ops[o] : one;
   const int r0 = s * 64;
   if (r0 >= op.m) return;
-  using Core = GemmCore<T, 64, 32, 32, 4, 1>;
+  using Core = GemmCore<T, 64, 32, 32, 4, 1, !TRANS>;
   __shared__ __align__(16) typename Core::Smem sm;
   __shared__ T Ts[kPB][64 + 4];
   __shared__ T Dis[kPB][kPB + 1];
@@ -232,22 +236,36 @@ __global__ __launch_bounds__(256) void trsm_kernel(const TrsmOp<T> *__restrict__
                     ((reinterpret_cast<uintptr_t>(op.B) & 15) == 0) &&
                     ((reinterpret_cast<uintptr_t>(op.L) & 15) == 0);
   const int nblk = (op.n + kPB - 1) / kPB;
-  for (int j = 0; j < nblk; ++j) {
+  for (int jj0 = 0; jj0 < nblk; ++jj0) {
+    const int j = TRANS ? jj0 : nblk - 1 - jj0;
     const int c0 = j * kPB;
     Core core;
     core.zero();
-    if (c0 > 0) {
-      if (fast)
-        core.template run<true>(sm, op.B, op.ldb, op.L, op.ldl, r0, c0, 0, c0, op.m, op.n);
-      else
-        core.template run<false>(sm, op.B, op.ldb, op.L, op.ldl, r0, c0, 0, c0, op.m, op.n);
+    if (TRANS) {
+      // X_{<j} (cols [0, c0) of this stripe) times rows [c0, c0+32) of L
+      if (c0 > 0) {
+        if (fast)
+          core.template run<true>(sm, op.B, op.ldb, op.L, op.ldl, r0, c0, 0, c0, op.m, op.n);
+        else
+          core.template run<false>(sm, op.B, op.ldb, op.L, op.ldl, r0, c0, 0, c0, op.m, op.n);
+      }
+    } else {
+      // X_{>j} (cols [c0+32, n)) times L[k, c0+jj] for k > c0+32  (B(jj, k) = L[k + (c0+jj)*ldl])
+      const int kb = c0 + kPB;
+      if (kb < op.n) {
+        if (fast)
+          core.template run<true>(sm, op.B, op.ldb, op.L + (long)c0 * op.ldl, op.ldl, r0, 0, kb, op.n, op.m, kPB);
+        else
+          core.template run<false>(sm, op.B, op.ldb, op.L + (long)c0 * op.ldl, op.ldl, r0, 0, kb, op.n, op.m,
+                                   min(kPB, op.n - c0));
+      }
     }
     for (int e = tid; e < kPB * kPB; e += 256) Dis[e & 31][e >> 5] = op.dinv[j * kPB * kPB + e];
 #pragma unroll
     for (int jj = 0; jj < 2; ++jj)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int rl = wave * 16 + 4 * lk + r, cl = jj * 16 + lr;
+        const int rl = wave * 16 + Mfma<T>::drow(lk, r), cl = jj * 16 + lr;
         const int row = r0 + rl, col = c0 + cl;
         const T bv = (row < op.m && col < op.n) ? op.B[row + (long)col * op.ldb] : T(0);
         Ts[cl][rl] = bv - core.acc[0][jj][r];
@@ -258,13 +276,17 @@ __global__ __launch_bounds__(256) void trsm_kernel(const TrsmOp<T> *__restrict__
     for (int ks = 0; ks < kPB / 4; ++ks) {
       const T av = Ts[ks * 4 + lk][wave * 16 + lr];
 #pragma unroll
-      for (int jj = 0; jj < 2; ++jj) x[jj] = Mfma<T>::mma(av, Dis[jj * 16 + lr][ks * 4 + lk], x[jj]);
+      for (int jj = 0; jj < 2; ++jj) {
+        // TRANS: out(r,c) = sum_k T(r,k) Dinv(c,k);  !TRANS: sum_k T(r,k) Dinv(k,c)
+        const T bv = TRANS ? Dis[jj * 16 + lr][ks * 4 + lk] : Dis[ks * 4 + lk][jj * 16 + lr];
+        x[jj] = Mfma<T>::mma(av, bv, x[jj]);
+      }
     }
 #pragma unroll
     for (int jj = 0; jj < 2; ++jj)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = r0 + wave * 16 + 4 * lk + r, col = c0 + jj * 16 + lr;
+        const int row = r0 + wave * 16 + Mfma<T>::drow(lk, r), col = c0 + jj * 16 + lr;
         if (row < op.m && col < op.n) op.B[row + (long)col * op.ldb] = x[jj][r];
       }
     wg_sync_global();
@@ -274,61 +296,76 @@ __global__ __launch_bounds__(256) void trsm_kernel(const TrsmOp<T> *__restrict__
 // ---------------------------------------------------------------------------
 // batched update  C := alpha A B^T + beta C
 // ---------------------------------------------------------------------------
-template <typename T>
+// Two block shapes: 128 x 128 for tile updates, 32 x 128 ("skinny") when
+// the output has at most 32 rows (the right-hand-side panels of the solves).
+template <typename T, bool TB, bool SKINNY>
 struct UpdCfg;
-template <>
-struct UpdCfg<double> {
-  using Core = GemmCore<double, 128, 128, 16, 2, 2>;
+template <bool TB>
+struct UpdCfg<double, TB, false> {
+  using Core = GemmCore<double, 128, 128, 16, 2, 2, TB>;
 };
-template <>
-struct UpdCfg<float> {
-  using Core = GemmCore<float, 128, 128, 32, 2, 2>;
+template <bool TB>
+struct UpdCfg<float, TB, false> {
+  using Core = GemmCore<float, 128, 128, 32, 2, 2, TB>;
+};
+template <bool TB>
+struct UpdCfg<double, TB, true> {
+  using Core = GemmCore<double, 32, 128, 32, 1, 4, TB>;
+};
+template <bool TB>
+struct UpdCfg<float, TB, true> {
+  using Core = GemmCore<float, 32, 128, 32, 1, 4, TB>;
 };
 
-template <typename T, bool FAST>
+// FAST applies to both operands for the 128 x 128 shape and to the B operand
+// only for the skinny shape (whose A rows are predicated).
+template <typename T, bool FAST, bool TB, bool SKINNY>
 __global__ __launch_bounds__(256) void update_kernel(const UpdOp<T> *__restrict__ ops, UpdOp<T> one, int bpo, int nbn,
                                                      T alpha, T beta, const int *__restrict__ d_abort) {
   if (d_abort && *d_abort) return;
+  using Core = typename UpdCfg<T, TB, SKINNY>::Core;
+  constexpr int BM = SKINNY ? 32 : 128;
+  constexpr int BN = 128;
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const int o = b / bpo, rr = b % bpo, bi = rr / nbn, bj = rr % nbn;
   const UpdOp<T> op = ops ? ops[o] : one;
   if (op.lower && bi < bj) return;
-  const int i0 = bi * 128, j0 = bj * 128;
+  const int i0 = bi * BM, j0 = bj * BN;
   if (i0 >= op.m || j0 >= op.n) return;
-  using Core = typename UpdCfg<T>::Core;
   __shared__ __align__(16) typename Core::Smem sm;
   Core core;
   core.zero();
-  core.template run<FAST>(sm, op.A, op.lda, op.B, op.ldb, i0, j0, 0, op.k, op.m, op.n);
+  core.template run2<FAST && !SKINNY, FAST>(sm, op.A, op.lda, op.B, op.ldb, i0, j0, 0, op.k, op.m, op.n);
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave / 2, wn = wave % 2;
+  constexpr int WN = SKINNY ? 4 : 2;
+  const int wm = wave / WN, wn = wave % WN;
   const bool diag = op.lower && bi == bj;
+  const bool full = FAST && !SKINNY && !diag;
 #pragma unroll
   for (int i = 0; i < Core::MI; ++i)
 #pragma unroll
     for (int j = 0; j < Core::NI; ++j) {
       const int col = j0 + Core::col_of(wn, j, lane);
-      const int rowb = i0 + Core::row_of(wm, i, lane, 0);
-      T *p = op.C + rowb + (long)col * op.ldc;
-      if (FAST && !diag) {
+      T *pc = op.C + (long)col * op.ldc;
+      if (full) {
         if (beta == T(0)) {
 #pragma unroll
-          for (int r = 0; r < 4; ++r) p[r] = alpha * core.acc[i][j][r];
+          for (int r = 0; r < 4; ++r) pc[i0 + Core::row_of(wm, i, lane, r)] = alpha * core.acc[i][j][r];
         } else {
           T old[4];
 #pragma unroll
-          for (int r = 0; r < 4; ++r) old[r] = p[r];
+          for (int r = 0; r < 4; ++r) old[r] = pc[i0 + Core::row_of(wm, i, lane, r)];
 #pragma unroll
-          for (int r = 0; r < 4; ++r) p[r] = alpha * core.acc[i][j][r] + beta * old[r];
+          for (int r = 0; r < 4; ++r) pc[i0 + Core::row_of(wm, i, lane, r)] = alpha * core.acc[i][j][r] + beta * old[r];
         }
       } else {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int row = rowb + r;
+          const int row = i0 + Core::row_of(wm, i, lane, r);
           if (row < op.m && col < op.n && (!diag || row >= col)) {
             const T v = alpha * core.acc[i][j][r];
-            p[r] = (beta == T(0)) ? v : v + beta * p[r];
+            pc[row] = (beta == T(0)) ? v : v + beta * pc[row];
           }
         }
       }
@@ -338,51 +375,81 @@ __global__ __launch_bounds__(256) void update_kernel(const UpdOp<T> *__restrict_
 // ---------------------------------------------------------------------------
 // launchers (ops == nullptr: a single op passed by value in `one`)
 // ---------------------------------------------------------------------------
-hipError_t launch_potrf(hipStream_t s, const PotrfOp *ops, int nops, const PotrfOp &one, const int *d_abort) {
+template <typename TS>
+hipError_t launch_potrf(hipStream_t s, const PotrfOpT<TS> *ops, int nops, const PotrfOpT<TS> &one,
+                        const int *d_abort) {
   if (nops <= 0) return hipSuccess;
-  hipLaunchKernelGGL(potrf_kernel, dim3(nops), dim3(256), 0, s, ops, one, d_abort);
+  hipLaunchKernelGGL(potrf_kernel<TS>, dim3(nops), dim3(256), 0, s, ops, one, d_abort);
   return hipGetLastError();
 }
 
-hipError_t launch_dinv(hipStream_t s, const PotrfOp *ops, int nops, const PotrfOp &one) {
+template <typename TS>
+hipError_t launch_dinv(hipStream_t s, const PotrfOpT<TS> *ops, int nops, const PotrfOpT<TS> &one) {
   if (nops <= 0) return hipSuccess;
-  hipLaunchKernelGGL(dinv_kernel, dim3(nops), dim3(256), 0, s, ops, one);
+  hipLaunchKernelGGL(dinv_kernel<TS>, dim3(nops), dim3(256), 0, s, ops, one);
   return hipGetLastError();
 }
+
+template hipError_t launch_potrf<double>(hipStream_t, const PotrfOpT<double> *, int, const PotrfOpT<double> &,
+                                         const int *);
+template hipError_t launch_potrf<float>(hipStream_t, const PotrfOpT<float> *, int, const PotrfOpT<float> &,
+                                        const int *);
+template hipError_t launch_dinv<double>(hipStream_t, const PotrfOpT<double> *, int, const PotrfOpT<double> &);
+template hipError_t launch_dinv<float>(hipStream_t, const PotrfOpT<float> *, int, const PotrfOpT<float> &);
 
 template <typename T>
 hipError_t launch_trsm(hipStream_t s, const TrsmOp<T> *ops, int nops, const TrsmOp<T> &one, int max_m,
-                       const int *d_abort) {
+                       const int *d_abort, bool trans) {
   if (nops <= 0 || max_m <= 0) return hipSuccess;
   const int bpo = (max_m + 63) / 64;
-  hipLaunchKernelGGL(trsm_kernel<T>, dim3(nops * bpo), dim3(256), 0, s, ops, one, bpo, d_abort);
+  if (trans)
+    hipLaunchKernelGGL((trsm_kernel<T, true>), dim3(nops * bpo), dim3(256), 0, s, ops, one, bpo, d_abort);
+  else
+    hipLaunchKernelGGL((trsm_kernel<T, false>), dim3(nops * bpo), dim3(256), 0, s, ops, one, bpo, d_abort);
+  return hipGetLastError();
+}
+
+template <typename T, bool SKINNY>
+static hipError_t launch_update_shape(hipStream_t s, const UpdOp<T> *ops, int nops, const UpdOp<T> &one, int max_m,
+                                      int max_n, T alpha, T beta, bool fast, const int *d_abort, bool tb) {
+  constexpr int BM = SKINNY ? 32 : 128;
+  const int nbm = (max_m + BM - 1) / BM, nbn = (max_n + 127) / 128;
+  const int bpo = nbm * nbn;
+  const long grid = (long)nops * bpo;
+  if (grid > 0x7fffffffL) return hipErrorInvalidConfiguration;
+  const dim3 g((unsigned)grid), b(256);
+  if (tb) {
+    if (fast)
+      hipLaunchKernelGGL((update_kernel<T, true, true, SKINNY>), g, b, 0, s, ops, one, bpo, nbn, alpha, beta, d_abort);
+    else
+      hipLaunchKernelGGL((update_kernel<T, false, true, SKINNY>), g, b, 0, s, ops, one, bpo, nbn, alpha, beta,
+                         d_abort);
+  } else {
+    if (fast)
+      hipLaunchKernelGGL((update_kernel<T, true, false, SKINNY>), g, b, 0, s, ops, one, bpo, nbn, alpha, beta,
+                         d_abort);
+    else
+      hipLaunchKernelGGL((update_kernel<T, false, false, SKINNY>), g, b, 0, s, ops, one, bpo, nbn, alpha, beta,
+                         d_abort);
+  }
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_update(hipStream_t s, const UpdOp<T> *ops, int nops, const UpdOp<T> &one, int max_m, int max_n,
-                         T alpha, T beta, bool fast, const int *d_abort) {
+                         T alpha, T beta, bool fast, const int *d_abort, bool tb) {
   if (nops <= 0 || max_m <= 0 || max_n <= 0) return hipSuccess;
-  const int nbm = (max_m + 127) / 128, nbn = (max_n + 127) / 128;
-  const int bpo = nbm * nbn;
-  const long grid = (long)nops * bpo;
-  if (grid > 0x7fffffffL) return hipErrorInvalidConfiguration;
-  if (fast)
-    hipLaunchKernelGGL((update_kernel<T, true>), dim3((unsigned)grid), dim3(256), 0, s, ops, one, bpo, nbn, alpha,
-                       beta, d_abort);
-  else
-    hipLaunchKernelGGL((update_kernel<T, false>), dim3((unsigned)grid), dim3(256), 0, s, ops, one, bpo, nbn, alpha,
-                       beta, d_abort);
-  return hipGetLastError();
+  if (max_m <= 32) return launch_update_shape<T, true>(s, ops, nops, one, max_m, max_n, alpha, beta, fast, d_abort, tb);
+  return launch_update_shape<T, false>(s, ops, nops, one, max_m, max_n, alpha, beta, fast, d_abort, tb);
 }
 
 template hipError_t launch_trsm<double>(hipStream_t, const TrsmOp<double> *, int, const TrsmOp<double> &, int,
-                                        const int *);
+                                        const int *, bool);
 template hipError_t launch_trsm<float>(hipStream_t, const TrsmOp<float> *, int, const TrsmOp<float> &, int,
-                                       const int *);
+                                       const int *, bool);
 template hipError_t launch_update<double>(hipStream_t, const UpdOp<double> *, int, const UpdOp<double> &, int, int,
-                                          double, double, bool, const int *);
+                                          double, double, bool, const int *, bool);
 template hipError_t launch_update<float>(hipStream_t, const UpdOp<float> *, int, const UpdOp<float> &, int, int,
-                                         float, float, bool, const int *);
+                                         float, float, bool, const int *, bool);
 
 }  // namespace ttgc

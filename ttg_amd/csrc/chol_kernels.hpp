@@ -7,15 +7,18 @@
 
 namespace ttgc {
 
-// One POTRF of an n x n tile (n <= 512).
-struct PotrfOp {
-  double *A;
-  double *dinv;  // optional: receives inv(L_jj) of every 32 x 32 diagonal block, 32x32 col-major each
-  int *info;     // optional device int
+// One POTRF of an n x n tile (n <= 512) stored as TS (double, or float for
+// the mixed-precision factor); the arithmetic is FP64 either way.
+template <typename TS>
+struct PotrfOpT {
+  TS *A;
+  TS *dinv;   // optional: receives inv(L_jj) of every 32 x 32 diagonal block, 32x32 col-major each
+  int *info;  // optional device int
   int n, lda;
   int info_offset;
   int write_zero;  // 1: store 0 into *info on success (per-tile API semantics)
 };
+using PotrfOp = PotrfOpT<double>;
 
 // One in-place TRSM  B := B * L^{-T}  of an m x n tile (n <= 512).
 template <typename T>
@@ -42,19 +45,33 @@ struct UpdOp {
 // argument) is used and nops must be 1.  `d_abort` (optional device int):
 // when nonzero at kernel start, the kernel does nothing (a failed POTRF
 // stops the rest of the factorization without a host round trip).
-hipError_t launch_potrf(hipStream_t s, const PotrfOp *ops, int nops, const PotrfOp &one, const int *d_abort);
+template <typename TS>
+hipError_t launch_potrf(hipStream_t s, const PotrfOpT<TS> *ops, int nops, const PotrfOpT<TS> &one,
+                        const int *d_abort);
 // inverses of the 32 x 32 diagonal blocks of an already factored tile
-hipError_t launch_dinv(hipStream_t s, const PotrfOp *ops, int nops, const PotrfOp &one);
+template <typename TS>
+hipError_t launch_dinv(hipStream_t s, const PotrfOpT<TS> *ops, int nops, const PotrfOpT<TS> &one);
+// trans: B := B L^{-T}; otherwise B := B L^{-1}
 template <typename T>
 hipError_t launch_trsm(hipStream_t s, const TrsmOp<T> *ops, int nops, const TrsmOp<T> &one, int max_m,
-                       const int *d_abort);
-// fast: every op has m % 128 == 0, n % 128 == 0, k % BK == 0, leading
-// dimensions a multiple of the 16-byte vector and 16-byte aligned pointers.
+                       const int *d_abort, bool trans = true);
+// fast: every op satisfies update_fast() (see below).  Outputs with at most
+// 32 rows (max_m <= 32) use a 32 x 128 block shape.
+// tb: the B operand's element (j, k) is read from B[k + j*ldb], i.e. the
+// update is C := alpha A B + beta C with B stored k x n column-major.
 template <typename T>
 hipError_t launch_update(hipStream_t s, const UpdOp<T> *ops, int nops, const UpdOp<T> &one, int max_m, int max_n,
-                         T alpha, T beta, bool fast, const int *d_abort);
-// K granularity of the fast update path
+                         T alpha, T beta, bool fast, const int *d_abort, bool tb = false);
+// Whether an update op may take the unpredicated vector path: full 128-wide
+// column blocks, k a multiple of 32, 16-byte aligned operands; plus full
+// 128-row blocks unless the op is skinny (m <= 32, rows always predicated).
 template <typename T>
-constexpr int update_bk() { return sizeof(T) == 8 ? 16 : 32; }
+inline bool update_fast(const UpdOp<T> &op, int max_m) {
+  constexpr int V = 16 / sizeof(T);
+  auto al = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  bool ok = op.n % 128 == 0 && op.k % 32 == 0 && op.ldb % V == 0 && al(op.B);
+  if (max_m > 32) ok = ok && op.m % 128 == 0 && op.lda % V == 0 && op.ldc % V == 0 && al(op.A) && al(op.C);
+  return ok;
+}
 
 }  // namespace ttgc

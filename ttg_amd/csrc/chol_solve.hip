@@ -1,0 +1,405 @@
+// Solves and residuals on the tiled factor (no reference counterpart; the
+// building blocks of iterative refinement).
+//
+//   potrs   : B := (L L^T)^{-1} B.  The right-hand sides are held transposed
+//             (Bt = B^T, nrhs x N) so both sweeps are tile ops the kernel
+//             library already has:  forward  Bt_i := Bt_i L_ii^{-T},
+//             Bt_m -= Bt_i L_mi^T (m > i);  backward  Bt_i := Bt_i L_ii^{-1},
+//             Bt_m -= Bt_i L_im (m < i).  Every step's updates are one launch;
+//             the sequence is captured in a hipGraph.
+//   residual: R := B - A X with A symmetric from its lower tiles.  Each tile
+//             (m, n) is read once; a workgroup forms A_mn X_n and A_mn^T X_m
+//             from 64 x 64 LDS blocks and stores them to per-tile slots, and a
+//             second kernel sums the slots in a fixed order (deterministic).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "chol_internal.hpp"
+#include "chol_kernels.hpp"
+#include "chol_solve.hpp"
+#include "chol_tiled.hpp"
+
+namespace ttgc {
+
+constexpr int kRhsChunk = 16;
+
+// ---------------------------------------------------------------------------
+// transposition of the right-hand sides (zero padding beyond N)
+// ---------------------------------------------------------------------------
+template <typename TS, typename TD>
+__global__ void to_rows_kernel(int N, int Np, int nrhs, const TS *__restrict__ B, long ldb, TD *__restrict__ Bt) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long)Np * nrhs) return;
+  const int j = (int)(g % nrhs), i = (int)(g / nrhs);
+  Bt[g] = (i < N) ? (TD)B[i + (long)j * ldb] : TD(0);
+}
+
+template <typename TS, typename TD>
+__global__ void from_rows_kernel(int N, int nrhs, const TS *__restrict__ Bt, TD *__restrict__ B, long ldb) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long)N * nrhs) return;
+  const int i = (int)(g % N), j = (int)(g / N);
+  B[i + (long)j * ldb] = (TD)Bt[j + (long)i * nrhs];
+}
+
+template <typename TS, typename TD>
+hipError_t launch_to_rows(hipStream_t s, int N, int Np, int nrhs, const TS *B, long ldb, TD *Bt) {
+  const long n = (long)Np * nrhs;
+  hipLaunchKernelGGL((to_rows_kernel<TS, TD>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, N, Np, nrhs, B,
+                     ldb, Bt);
+  return hipGetLastError();
+}
+template <typename TS, typename TD>
+hipError_t launch_from_rows(hipStream_t s, int N, int nrhs, const TS *Bt, TD *B, long ldb) {
+  const long n = (long)N * nrhs;
+  hipLaunchKernelGGL((from_rows_kernel<TS, TD>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, N, nrhs, Bt, B,
+                     ldb);
+  return hipGetLastError();
+}
+
+template hipError_t launch_to_rows<double, double>(hipStream_t, int, int, int, const double *, long, double *);
+template hipError_t launch_from_rows<double, double>(hipStream_t, int, int, const double *, double *, long);
+template hipError_t launch_to_rows<double, float>(hipStream_t, int, int, int, const double *, long, float *);
+template hipError_t launch_from_rows<float, double>(hipStream_t, int, int, const float *, double *, long);
+
+// ---------------------------------------------------------------------------
+// triangular-solve plan
+// ---------------------------------------------------------------------------
+template <typename T>
+int SolvePlan<T>::build() {
+  NT = (N + NB - 1) / NB;
+  Np = NT * NB;
+  dstride = ((NB + 31) / 32) * 32 * 32;
+  TTGC_HIP(d_bt.reserve(sizeof(T) * (size_t)Np * nrhs));
+  TTGC_HIP(d_dinv.reserve(sizeof(T) * (size_t)dstride * NT));
+  T *bt = static_cast<T *>(d_bt.ptr);
+  T *dinv = static_cast<T *>(d_dinv.ptr);
+  std::vector<PotrfOpT<T>> dj;
+  std::vector<TrsmOp<T>> trsm;
+  std::vector<UpdOp<T>> upd;
+  auto tile = [&](int m, int n) { return tiles + tile_index(m, n, NT) * (long)NB * NB; };
+  auto rows = [&](int i) { return bt + (long)i * NB * nrhs; };
+  // actual (edge) sizes everywhere: the solve never reads tile padding
+  auto dim = [&](int i) { return std::min(NB, N - i * NB); };
+  for (int k = 0; k < NT; ++k)
+    dj.push_back(PotrfOpT<T>{const_cast<T *>(tile(k, k)), dinv + (long)k * dstride, nullptr, dim(k), NB, 0, 0});
+  auto dinv_of = [&](int i) { return (const T *)(dinv + (long)i * dstride); };
+  // forward
+  for (int i = 0; i < NT; ++i) {
+    fw_trsm.push_back((int)trsm.size());
+    trsm.push_back(TrsmOp<T>{tile(i, i), rows(i), dinv_of(i), nrhs, dim(i), NB, nrhs});
+    fw_upd_off.push_back((int)upd.size());
+    for (int m = i + 1; m < NT; ++m)
+      upd.push_back(UpdOp<T>{rows(i), tile(m, i), rows(m), nrhs, dim(m), dim(i), nrhs, NB, nrhs, 0});
+    fw_upd_cnt.push_back((int)upd.size() - fw_upd_off.back());
+  }
+  // backward
+  for (int i = NT - 1; i >= 0; --i) {
+    bw_trsm.push_back((int)trsm.size());
+    trsm.push_back(TrsmOp<T>{tile(i, i), rows(i), dinv_of(i), nrhs, dim(i), NB, nrhs});
+    bw_upd_off.push_back((int)upd.size());
+    for (int m = 0; m < i; ++m)
+      upd.push_back(UpdOp<T>{rows(i), tile(i, m), rows(m), nrhs, dim(m), dim(i), nrhs, NB, nrhs, 0});
+    bw_upd_cnt.push_back((int)upd.size() - bw_upd_off.back());
+  }
+  auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+  o_dj = 0;
+  o_trsm = align(dj.size() * sizeof(PotrfOpT<T>));
+  o_upd = align(o_trsm + trsm.size() * sizeof(TrsmOp<T>));
+  const size_t total = align(o_upd + upd.size() * sizeof(UpdOp<T>));
+  TTGC_HIP(d_ops.reserve(total));
+  std::vector<char> host(total);
+  memcpy(host.data() + o_dj, dj.data(), dj.size() * sizeof(PotrfOpT<T>));
+  memcpy(host.data() + o_trsm, trsm.data(), trsm.size() * sizeof(TrsmOp<T>));
+  if (!upd.empty()) memcpy(host.data() + o_upd, upd.data(), upd.size() * sizeof(UpdOp<T>));
+  TTGC_HIP(hipMemcpy(d_ops.ptr, host.data(), total, hipMemcpyHostToDevice));
+  TTGC_HIP(hipStreamCreateWithFlags(&s_main, hipStreamNonBlocking));
+  TTGC_HIP(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
+  TTGC_HIP(hipEventCreateWithFlags(&ev_out, hipEventDisableTiming));
+  return TTG_CHOL_OK;
+}
+
+template <typename T>
+int SolvePlan<T>::enqueue_sweeps(hipStream_t s) {
+  PotrfOpT<T> pnone{};
+  TrsmOp<T> tnone{};
+  UpdOp<T> unone{};
+  TTGC_HIP(launch_dinv(s, dj_ops(), NT, pnone));
+  const T m1 = T(-1), p1 = T(1);
+  const bool fast = NB % 128 == 0 && N % NB == 0;
+  for (int q = 0; q < NT; ++q) {
+    TTGC_HIP(launch_trsm<T>(s, trsm_ops() + fw_trsm[q], 1, tnone, nrhs, nullptr, true));
+    if (fw_upd_cnt[q])
+      TTGC_HIP(launch_update<T>(s, upd_ops() + fw_upd_off[q], fw_upd_cnt[q], unone, nrhs, NB, m1, p1, fast, nullptr,
+                                false));
+  }
+  for (int q = 0; q < NT; ++q) {
+    TTGC_HIP(launch_trsm<T>(s, trsm_ops() + bw_trsm[q], 1, tnone, nrhs, nullptr, false));
+    if (bw_upd_cnt[q])
+      TTGC_HIP(launch_update<T>(s, upd_ops() + bw_upd_off[q], bw_upd_cnt[q], unone, nrhs, NB, m1, p1, fast, nullptr,
+                                true));
+  }
+  return TTG_CHOL_OK;
+}
+
+template <typename T>
+SolvePlan<T>::~SolvePlan() {
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (ev_in) (void)hipEventDestroy(ev_in);
+  if (ev_out) (void)hipEventDestroy(ev_out);
+  if (s_main) (void)hipStreamDestroy(s_main);
+  for (DevBuf *b : {&d_ops, &d_bt, &d_dinv})
+    if (b->ptr) (void)hipFree(b->ptr);
+}
+
+// Run the two sweeps on Bt (already filled) as a replayed graph, ordered
+// after `user` and before whatever `user` does next.
+template <typename T>
+int SolvePlan<T>::run_sweeps(hipStream_t user) {
+  if (!gexec) {
+    TTGC_HIP(hipStreamBeginCapture(s_main, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_sweeps(s_main);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s_main, &g);
+    if (rc != TTG_CHOL_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    TTGC_HIP(e);
+    graph = g;
+    TTGC_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+  }
+  TTGC_HIP(hipGraphLaunch(gexec, user));
+  return TTG_CHOL_OK;
+}
+
+template struct SolvePlan<double>;
+template struct SolvePlan<float>;
+
+static std::mutex g_solve_mu;
+static std::map<std::tuple<int, int, const void *, int, int>, std::unique_ptr<SolvePlan<double>>> g_dplans;
+static std::map<std::tuple<int, int, const void *, int, int>, std::unique_ptr<SolvePlan<float>>> g_splans;
+
+template <typename T>
+SolvePlan<T> *get_solve_plan(int N, int NB, const T *tiles, int nrhs, int *rc) {
+  std::lock_guard<std::mutex> lk(g_solve_mu);
+  auto key = std::make_tuple(N, NB, (const void *)tiles, nrhs, (int)sizeof(T));
+  auto &plans = [&]() -> auto & {
+    if constexpr (sizeof(T) == 8)
+      return g_dplans;
+    else
+      return g_splans;
+  }();
+  auto &p = plans[key];
+  if (!p) {
+    p.reset(new SolvePlan<T>);
+    p->N = N;
+    p->NB = NB;
+    p->tiles = tiles;
+    p->nrhs = nrhs;
+    *rc = p->build();
+    if (*rc != TTG_CHOL_OK) {
+      plans.erase(key);
+      return nullptr;
+    }
+  }
+  *rc = TTG_CHOL_OK;
+  return p.get();
+}
+
+template SolvePlan<double> *get_solve_plan<double>(int, int, const double *, int, int *);
+template SolvePlan<float> *get_solve_plan<float>(int, int, const float *, int, int *);
+
+// ---------------------------------------------------------------------------
+// residual
+// ---------------------------------------------------------------------------
+constexpr int kRB = 64;  // LDS block of A
+
+// W layout: [tile][slot 0/1][kRhsChunk][NB]
+__global__ __launch_bounds__(256) void residual_tile_kernel(int N, int NB, int nt, const double *__restrict__ tiles,
+                                                            const double *__restrict__ X, long ldx, int nr,
+                                                            double *__restrict__ W) {
+  const long t = blockIdx.x;
+  int m, n;
+  {
+    int lo = 0, hi = nt - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) / 2;
+      if ((long)mid * nt - (long)mid * (mid - 1) / 2 <= t)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    n = lo;
+    m = n + (int)(t - ((long)n * nt - (long)n * (n - 1) / 2));
+  }
+  const double *A = tiles + t * (long)NB * NB;
+  double *W0 = W + t * 2L * kRhsChunk * NB;
+  double *W1 = W0 + (long)kRhsChunk * NB;
+  __shared__ double blk[kRB][kRB + 1];  // blk[c][r]
+  __shared__ double xn[kRB][kRhsChunk];
+  __shared__ double xm[kRB][kRhsChunk];
+  const int tid = threadIdx.x, rl = tid & 63, jg = tid >> 6;
+  const bool diag = (m == n);
+  const int nsb = (NB + kRB - 1) / kRB;
+  const int mrows = min(NB, N - m * NB), ncols = min(NB, N - n * NB);
+  double pacc[8][4];  // nsb <= 8 (NB <= 512)
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) pacc[a][q] = 0.0;
+  for (int cs = 0; cs < nsb; ++cs) {
+    double qacc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int rs = 0; rs < 8; ++rs) {
+      if (rs >= nsb) break;
+      if (diag && rs < cs) continue;  // strictly upper blocks of a diagonal tile are not stored
+      __syncthreads();
+      for (int e = tid; e < kRB * kRB; e += 256) {
+        const int r = e & 63, c = e >> 6;
+        const int gr = rs * kRB + r, gc = cs * kRB + c;
+        double v = 0.0;
+        if (gr < mrows && gc < ncols && (!diag || gr >= gc)) v = A[gr + (long)gc * NB];
+        blk[c][r] = v;
+      }
+      for (int e = tid; e < kRB * kRhsChunk; e += 256) {
+        const int r = e % kRB, j = e / kRB;
+        const int gcol = n * NB + cs * kRB + r, grow = m * NB + rs * kRB + r;
+        xn[r][j] = (j < nr && gcol < N && cs * kRB + r < ncols) ? X[gcol + (long)j * ldx] : 0.0;
+        xm[r][j] = (j < nr && grow < N && rs * kRB + r < mrows) ? X[grow + (long)j * ldx] : 0.0;
+      }
+      __syncthreads();
+      // P: rows of this block times X_n
+#pragma unroll 8
+      for (int c = 0; c < kRB; ++c) {
+        const double a = blk[c][rl];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) pacc[rs][q] += a * xn[c][jg + 4 * q];
+      }
+      // Q: columns of this block times X_m (strictly lower part only on a diagonal tile)
+      const bool dblk = diag && rs == cs;
+#pragma unroll 8
+      for (int r = 0; r < kRB; ++r) {
+        double a = blk[rl][r];
+        if (dblk && r <= rl) a = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) qacc[q] += a * xm[r][jg + 4 * q];
+      }
+    }
+    if (cs * kRB + rl < NB)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) W1[(long)(jg + 4 * q) * NB + cs * kRB + rl] = qacc[q];
+  }
+#pragma unroll
+  for (int rs = 0; rs < 8; ++rs) {
+    if (rs >= nsb) break;
+    if (rs * kRB + rl < NB)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) W0[(long)(jg + 4 * q) * NB + rs * kRB + rl] = pacc[rs][q];
+  }
+}
+
+// R(i, j) = B(i, j) - sum_{n <= m} W0(m, n) - sum_{n' >= m} W1(n', m),  m = i / NB
+__global__ void residual_reduce_kernel(int N, int NB, int nt, const double *__restrict__ W, int nr,
+                                       const double *__restrict__ B, long ldb, double *__restrict__ R, long ldr,
+                                       int j0) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long)N * nr) return;
+  const int i = (int)(g % N), j = (int)(g / N);
+  const int m = i / NB, r = i % NB;
+  double s = 0.0;
+  for (int n = 0; n <= m; ++n) {
+    const long t = tile_index(m, n, nt);
+    s += W[t * 2L * kRhsChunk * NB + (long)j * NB + r];
+  }
+  for (int n2 = m; n2 < nt; ++n2) {
+    const long t = tile_index(n2, m, nt);
+    s += W[t * 2L * kRhsChunk * NB + (long)kRhsChunk * NB + (long)j * NB + r];
+  }
+  R[i + (long)(j0 + j) * ldr] = B[i + (long)(j0 + j) * ldb] - s;
+}
+
+static std::mutex g_res_mu;
+static std::map<hipStream_t, std::unique_ptr<DevBuf>> g_res_ws;
+
+int residual_fp64(hipStream_t s, int N, int NB, const double *tiles, int nrhs, const double *X, long ldx,
+                  const double *B, long ldb, double *R, long ldr) {
+  const int nt = (N + NB - 1) / NB;
+  const long ntiles = (long)nt * (nt + 1) / 2;
+  DevBuf *ws;
+  {
+    std::lock_guard<std::mutex> lk(g_res_mu);
+    auto &p = g_res_ws[s];
+    if (!p) p.reset(new DevBuf);
+    ws = p.get();
+  }
+  TTGC_HIP(ws->reserve(sizeof(double) * (size_t)ntiles * 2 * kRhsChunk * NB));
+  double *W = static_cast<double *>(ws->ptr);
+  for (int j0 = 0; j0 < nrhs; j0 += kRhsChunk) {
+    const int nr = std::min(kRhsChunk, nrhs - j0);
+    hipLaunchKernelGGL(residual_tile_kernel, dim3((unsigned)ntiles), dim3(256), 0, s, N, NB, nt, tiles,
+                       X + (long)j0 * ldx, ldx, nr, W);
+    TTGC_HIP(hipGetLastError());
+    const long n = (long)N * nr;
+    hipLaunchKernelGGL(residual_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, N, NB, nt, W, nr,
+                       B, ldb, R, ldr, j0);
+    TTGC_HIP(hipGetLastError());
+  }
+  return TTG_CHOL_OK;
+}
+
+}  // namespace ttgc
+
+using namespace ttgc;
+
+extern "C" {
+
+int ttg_chol_dpotrs_tiled(void *stream, int N, int NB, const double *tiles, int nrhs, double *B, int ldb) {
+  clear_error();
+  if (N < 0 || NB <= 0 || NB > TTG_CHOL_MAX_TILE || NB % 32 != 0 || nrhs < 0 || ldb < std::max(1, N) ||
+      (N > 0 && nrhs > 0 && (!tiles || !B))) {
+    set_error("dpotrs_tiled: bad argument (N=%d NB=%d nrhs=%d ldb=%d)", N, NB, nrhs, ldb);
+    return TTG_CHOL_EINVAL;
+  }
+  if (N == 0 || nrhs == 0) return TTG_CHOL_OK;
+  if (nrhs > 32) {
+    // the sweeps use the skinny (<= 32 rows) update shape: split wide solves
+    for (int j0 = 0; j0 < nrhs; j0 += 32) {
+      int rc = ttg_chol_dpotrs_tiled(stream, N, NB, tiles, std::min(32, nrhs - j0), B + (long)j0 * ldb, ldb);
+      if (rc != TTG_CHOL_OK) return rc;
+    }
+    return TTG_CHOL_OK;
+  }
+  hipStream_t s = as_stream(stream);
+  int rc = 0;
+  SolvePlan<double> *p = get_solve_plan<double>(N, NB, tiles, nrhs, &rc);
+  if (!p) return rc;
+  double *bt = static_cast<double *>(p->d_bt.ptr);
+  TTGC_HIP(launch_to_rows<double, double>(s, N, p->Np, nrhs, B, ldb, bt));
+  rc = p->run_sweeps(s);
+  if (rc != TTG_CHOL_OK) return rc;
+  TTGC_HIP(launch_from_rows<double, double>(s, N, nrhs, bt, B, ldb));
+  return TTG_CHOL_OK;
+}
+
+int ttg_chol_dresidual_tiled(void *stream, int N, int NB, const double *tiles, int nrhs, const double *X, int ldx,
+                             const double *B, int ldb, double *R, int ldr) {
+  clear_error();
+  if (N < 0 || NB <= 0 || NB > TTG_CHOL_MAX_TILE || nrhs < 0 || ldx < std::max(1, N) || ldb < std::max(1, N) ||
+      ldr < std::max(1, N) || (N > 0 && nrhs > 0 && (!tiles || !X || !B || !R))) {
+    set_error("dresidual_tiled: bad argument (N=%d NB=%d nrhs=%d)", N, NB, nrhs);
+    return TTG_CHOL_EINVAL;
+  }
+  if (N == 0 || nrhs == 0) return TTG_CHOL_OK;
+  return residual_fp64(as_stream(stream), N, NB, tiles, nrhs, X, ldx, B, ldb, R, ldr);
+}
+
+}  // extern "C"

@@ -24,6 +24,7 @@
 
 #include "chol_internal.hpp"
 #include "chol_kernels.hpp"
+#include "chol_profile.hpp"
 #include "chol_tiled.hpp"
 
 namespace ttgc {
@@ -108,9 +109,11 @@ struct TiledPlan {
   }
 
   // Enqueue the whole factorization on s_main (and s_look), writing *info.
-  int enqueue(int *info) {
+  // With `prof`, every launch is bracketed by timing events.
+  int enqueue(int *info, bool prof) {
     const bool fast = NB % 128 == 0;
     const double m1 = -1.0, p1 = 1.0;
+    const double b3 = (double)NB * NB * NB;
     UpdOp<double> unone{};
     TrsmOp<double> tnone{};
     PotrfOp pnone{};
@@ -118,19 +121,35 @@ struct TiledPlan {
     const bool la = (flags & TTG_CHOL_F_LOOKAHEAD) != 0;
     hipStream_t sp = la ? s_look : s_main;
     if (la) TTGC_HIP(hipEventRecord(ev_col, s_main));
+    auto upd = [&](int off, int cnt, int nsyrk) -> int {
+      const int slot = prof ? prof_begin(s_main) : -1;
+      TTGC_HIP(launch_update<double>(s_main, upd_ops() + off, cnt, unone, NB, NB, m1, p1, fast, info));
+      if (prof) prof_end(slot, s_main, PROF_UPDATE, (cnt - nsyrk) * 2.0 * b3 + nsyrk * (double)NB * NB * (NB + 1));
+      return TTG_CHOL_OK;
+    };
     for (int k = 0; k < NT; ++k) {
       if (la) TTGC_HIP(hipStreamWaitEvent(s_look, ev_col, 0));
+      int slot = prof ? prof_begin(sp) : -1;
       TTGC_HIP(launch_potrf(sp, potrf_ops() + k, 1, pnone, info));
-      if (trsm_cnt[k]) TTGC_HIP(launch_trsm<double>(sp, trsm_ops() + trsm_off[k], trsm_cnt[k], tnone, NB, info));
+      if (prof) prof_end(slot, sp, PROF_POTRF, b3 / 3.0);
+      if (trsm_cnt[k]) {
+        slot = prof ? prof_begin(sp) : -1;
+        TTGC_HIP(launch_trsm<double>(sp, trsm_ops() + trsm_off[k], trsm_cnt[k], tnone, NB, info));
+        if (prof) prof_end(slot, sp, PROF_TRSM, trsm_cnt[k] * b3);
+      }
       if (la) {
         TTGC_HIP(hipEventRecord(ev_panel, s_look));
         TTGC_HIP(hipStreamWaitEvent(s_main, ev_panel, 0));
       }
-      if (updA_cnt[k])
-        TTGC_HIP(launch_update<double>(s_main, upd_ops() + updA_off[k], updA_cnt[k], unone, NB, NB, m1, p1, fast, info));
+      if (updA_cnt[k]) {
+        int rc = upd(updA_off[k], updA_cnt[k], 1);
+        if (rc) return rc;
+      }
       if (la) TTGC_HIP(hipEventRecord(ev_col, s_main));
-      if (updB_cnt[k])
-        TTGC_HIP(launch_update<double>(s_main, upd_ops() + updB_off[k], updB_cnt[k], unone, NB, NB, m1, p1, fast, info));
+      if (updB_cnt[k]) {
+        int rc = upd(updB_off[k], updB_cnt[k], NT - k - 2);
+        if (rc) return rc;
+      }
     }
     if (la) {
       TTGC_HIP(hipEventRecord(ev_panel, s_look));
@@ -140,7 +159,8 @@ struct TiledPlan {
   }
 
   int run(hipStream_t user, int *info) {
-    if (flags & TTG_CHOL_F_GRAPH) {
+    const bool prof = (flags & TTG_CHOL_F_PROFILE) != 0;
+    if ((flags & TTG_CHOL_F_GRAPH) && !prof) {
       if (!gexec || bound_info != info) {
         if (gexec) {
           TTGC_HIP(hipGraphExecDestroy(gexec));
@@ -151,7 +171,7 @@ struct TiledPlan {
           graph = nullptr;
         }
         TTGC_HIP(hipStreamBeginCapture(s_main, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue(info);
+        int rc = enqueue(info, false);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(s_main, &g);
         if (rc != TTG_CHOL_OK) {
@@ -168,7 +188,7 @@ struct TiledPlan {
     }
     TTGC_HIP(hipEventRecord(ev_in, user));
     TTGC_HIP(hipStreamWaitEvent(s_main, ev_in, 0));
-    int rc = enqueue(info);
+    int rc = enqueue(info, prof);
     if (rc != TTG_CHOL_OK) return rc;
     TTGC_HIP(hipEventRecord(ev_out, s_main));
     TTGC_HIP(hipStreamWaitEvent(user, ev_out, 0));
