@@ -47,10 +47,12 @@ def parse():
     p.add_argument("--nb", type=int, default=512)
     p.add_argument("--nrhs", type=int, default=16)
     p.add_argument("--kappa", type=float, default=1e4)
-    p.add_argument("--mode", choices=["fp64"], default="fp64")
+    p.add_argument("--mode", choices=["mixed", "fp64"], default="mixed",
+                   help="mixed: FP32 working factor + FP64 refinement (the metric); fp64: FP64 factor")
+    p.add_argument("--max-iter", type=int, default=30)
     p.add_argument("--tol", type=float, default=1e-12)
     p.add_argument("--seed", type=int, default=2024)
-    p.add_argument("--cpu-n", type=int, default=8192, help="order of the bounded CPU-baseline sample")
+    p.add_argument("--cpu-n", type=int, default=16384, help="order of the bounded CPU-baseline sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-profile", action="store_true")
     return p.parse_args()
@@ -106,7 +108,6 @@ def main():
     torch.cuda.empty_cache()
     g = torch.Generator(device="cpu").manual_seed(a.seed)
     B = (torch.rand(nrhs, N, generator=g, dtype=torch.float64) * 2 - 1).to(dev).t()  # N x nrhs column-major
-    Aw = torch.empty_like(A0)
     X = torch.empty_like(B.t()).t()
     info = torch.zeros(1, dtype=torch.int32, device=dev)
     R = torch.empty_like(X)
@@ -122,11 +123,14 @@ def main():
     anorm = float((-rs).max().item())
     del Aabs
 
-    def solve_fp64():
-        Aw.copy_(A0)
-        C.dpotrf_tiled(Aw, N, NB, info)
-        X.copy_(B)
-        C.dpotrs_tiled(Aw, N, NB, X)
+    prec = {"mixed": C.PREC_FP32, "fp64": C.PREC_FP64}
+    iters = {}
+
+    def make_step(mode):
+        def step():
+            it, be = C.dposv_mixed(A0, N, NB, B, X, prec=prec[mode], tol=a.tol, max_iter=a.max_iter, info=info)
+            iters[mode] = (it, be)
+        return step
 
     def backward_error():
         C.dresidual_tiled(A0, N, NB, X, B, R)
@@ -135,45 +139,58 @@ def main():
         bn = B.abs().max(dim=0).values
         return float((rn / (anorm * xn + bn)).max().item())
 
-    step = solve_fp64
-    for _ in range(a.warmup):
-        step()
-    torch.cuda.synchronize()
-    if int(info.item()) != 0:
-        raise RuntimeError(f"factorization failed: info={int(info.item())}")
-    t0 = time.perf_counter()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    ev0.record(stream)
-    for _ in range(a.steps):
-        step()
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_step = ev0.elapsed_time(ev1) / a.steps
-    berr = backward_error()
-    if int(info.item()) != 0 or not (berr <= a.tol):
-        raise RuntimeError(f"solve failed: info={int(info.item())} backward error={berr}")
+    def timed(mode, steps, warmup):
+        step = make_step(mode)
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        if int(info.item()) != 0:
+            raise RuntimeError(f"{mode}: factorization failed: info={int(info.item())}")
+        t0 = time.perf_counter()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(steps):
+            step()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        berr = backward_error()
+        if int(info.item()) != 0 or not (berr <= a.tol):
+            raise RuntimeError(f"{mode}: solve failed: info={int(info.item())} backward error={berr}")
+        return ev0.elapsed_time(ev1) / steps, wall, berr
+
+    ms_step, wall, berr = timed(a.mode, a.steps, a.warmup)
+    it_main = iters[a.mode][0]
+    other = "fp64" if a.mode == "mixed" else None
+    ms_fp64 = berr_fp64 = None
+    if other:
+        ms_fp64, _, berr_fp64 = timed(other, 1, 1)
     F = flops_potrf(N) + 2.0 * N * N * nrhs
     value = F / (ms_step * 1e-3) / 1e12
 
-    # ---- roofline of the dominant kernel (batched SYRK+GEMM update), from
-    # one extra event-profiled factorization after the timed region
+    # ---- roofline of the dominant kernel (the batched SYRK+GEMM update of the
+    # factorization run by this mode), from one extra solve after the timed
+    # region with every launch timed by HIP events on its own stream
     roof = None
     if not a.no_profile:
-        C.profile_read(C.PROF_UPDATE)
-        C.profile_read(C.PROF_TRSM)
-        C.profile_read(C.PROF_POTRF)
-        Aw.copy_(A0)
-        C.dpotrf_tiled(Aw, N, NB, info, flags=C.F_LOOKAHEAD | C.F_PROFILE)
-        torch.cuda.synchronize()
+        for k in (C.PROF_UPDATE, C.PROF_TRSM, C.PROF_POTRF):
+            C.profile_read(k)
+        C.profile_enable(True)
+        try:
+            make_step(a.mode)()
+            torch.cuda.synchronize()
+        finally:
+            C.profile_enable(False)
         ms_u, n_u, fl_u = C.profile_read(C.PROF_UPDATE)
         ms_t, n_t, fl_t = C.profile_read(C.PROF_TRSM)
         ms_p, n_p, fl_p = C.profile_read(C.PROF_POTRF)
+        peak = FP32_PEAK if a.mode == "mixed" else FP64_PEAK
         ach = fl_u / (ms_u * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": round(ach, 3), "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
-                "frac": round(ach / (FP64_PEAK / 1e12), 4), "traffic": None,
-                "kernel": "update_kernel<double> (batched SYRK+GEMM, 128x128 blocks)",
+        roof = {"bound": "mfma", "achieved": round(ach, 3), "peak": round(peak / 1e12, 1), "unit": "TFLOP/s",
+                "frac": round(ach / (peak / 1e12), 4), "traffic": None,
+                "kernel": "update_kernel<%s> (batched SYRK+GEMM trailing update)" % (
+                    "float" if a.mode == "mixed" else "double"),
                 "launches": n_u, "avg_launch_ms": round(ms_u / max(n_u, 1), 4),
                 "flops_per_launch": fl_u / max(n_u, 1),
                 "others": {"trsm_ms": round(ms_t, 2), "trsm_tflops": round(fl_t / max(ms_t, 1e-9) / 1e9, 2),
@@ -194,12 +211,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if a.mode == "mixed" else "f64",
         "data": "synthetic: A = B B^T/N + alpha I, B(i,j) = k/127 with k uniform integer in [-127,127] "
                 "(seeded LCG), alpha for kappa ~ 1e4; 16 RHS uniform[-1,1]",
         "config": {"workload": f"SPD solve N={N} NB={NB} nrhs={nrhs} kappa~{a.kappa:g} (tiled POTRF graph)",
                    "N": N, "NB": NB, "nrhs": nrhs, "mode": a.mode, "parallelism": "single GPU"},
         "backward_error": berr,
+        "refinement_steps": it_main,
+        "fp64_ms_per_solve": None if ms_fp64 is None else round(ms_fp64, 3),
+        "fp64_backward_error": berr_fp64,
+        "speedup_vs_fp64": None if ms_fp64 is None else round(ms_fp64 / ms_step, 3),
         "wall_s": round(wall, 3),
         "roofline": roof,
         "cpu_baseline": cpu,

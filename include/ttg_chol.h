@@ -43,6 +43,7 @@ extern "C" {
 #define TTG_CHOL_ENOTSUP (-3)  /* size or option outside what the kernels support */
 #define TTG_CHOL_ENOMEM (-4)   /* device allocation of a plan or workspace failed */
 #define TTG_CHOL_ESTATE (-5)   /* call out of order (e.g. group_end without begin) */
+#define TTG_CHOL_ENOCONV (-6)  /* iterative refinement did not reach the tolerance */
 
 /* Largest tile order handled by the single-tile POTRF/TRSM kernels. */
 #define TTG_CHOL_MAX_TILE 512
@@ -137,6 +138,26 @@ int ttg_chol_dpotrs_tiled(void *stream, int N, int NB, const double *tiles, int 
 int ttg_chol_dresidual_tiled(void *stream, int N, int NB, const double *tiles, int nrhs, const double *X,
                              int ldx, const double *B, int ldb, double *R, int ldr);
 
+/* Working precisions of ttg_chol_dposv_mixed. */
+#define TTG_CHOL_PREC_FP64 0 /* FP64 factor (time-to-solution reference) */
+#define TTG_CHOL_PREC_FP32 1 /* FP32 TRSM + trailing updates, FP64 diagonal POTRF */
+
+/* Solve A X = B (A SPD, FP64 lower tiles, left unchanged; B, X N x nrhs,
+ * nrhs <= 32) with a factorization in working precision `prec` followed by
+ * FP64 iterative refinement until the normwise backward error
+ * max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf) <= tol or
+ * max_iter refinement steps.  The residual r = b - A x and the update
+ * x += d run in FP64 on the device; the diagonal POTRF computes in FP64.
+ * Synchronizes the host once per refinement step (to test convergence).
+ * *iters = refinement steps taken, *berr = final backward error;
+ * *d_info = factorization info (nonzero: not positive definite in the
+ * working precision; X is then not set).  Returns TTG_CHOL_ENOCONV if the
+ * tolerance was not reached.  Workspaces are owned by the library and reused
+ * across calls with the same (N, NB, nrhs, prec).
+ * (No counterpart in the reference: the mixed-precision solve.) */
+int ttg_chol_dposv_mixed(void *stream, int N, int NB, const double *A, int nrhs, const double *B, int ldb, double *X,
+                         int ldx, int prec, double tol, int max_iter, int *d_info, int *iters, double *berr);
+
 /* ---------------------------------------------------------------- */
 /* generators                                                        */
 /* ---------------------------------------------------------------- */
@@ -169,6 +190,9 @@ int ttg_chol_spd_tiled(void *stream, int N, int NB, double alpha, unsigned long 
  * driver calls since the last read.  Waits for those launches to finish.
  * (No counterpart in the reference.) */
 int ttg_chol_profile_read(int kind, double *ms_total, long long *launches, double *flops);
+/* Profile every tiled factorization (also inside ttg_chol_dposv_mixed) as
+ * with TTG_CHOL_F_PROFILE while `on` is nonzero. */
+int ttg_chol_profile_enable(int on);
 
 /* Lane-layout self test of the 16x16x4 MFMA instructions the kernels use
  * (kind 0: FP64, kind 1: FP32).  `out` (device, 64*64*4 doubles) receives,

@@ -1,0 +1,59 @@
+"""Mixed-precision solve with FP64 iterative refinement (no reference
+counterpart): pinned by the FP64 backward error recomputed by the oracle on
+the CPU, and by agreement with the oracle's FP64 solution within the
+forward-error bound kappa * tol."""
+import numpy as np
+import pytest
+import torch
+
+from gpu_util import to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def sigma2():
+    return sum(k * k for k in range(-127, 128)) / 255.0 / 127.0 ** 2
+
+
+@pytest.mark.parametrize("N,NB,nrhs,kappa,prec", [(1024, 256, 4, 1e4, 1), (2048, 512, 16, 1e4, 1),
+                                                  (1000, 128, 3, 1e3, 1), (1536, 512, 2, 1e4, 0)])
+def test_dposv_mixed_reaches_fp64_backward_error(dev, oracle, N, NB, nrhs, kappa, prec):
+    import ttg_amd.chol as C
+
+    alpha = 4 * sigma2() / (kappa - 1)
+    A0 = oracle.spd_tiles(N, NB, alpha, 13)
+    rng = np.random.default_rng(N)
+    Bh = rng.uniform(-1, 1, (N, nrhs))
+    dA = to_dev(A0, dev)
+    dB = torch.from_numpy(np.ascontiguousarray(Bh.T)).to(dev).t()
+    X = torch.zeros(nrhs, N, dtype=torch.float64, device=dev).t()
+    info = torch.zeros(1, dtype=torch.int32, device=dev)
+    its, berr = C.dposv_mixed(dA, N, NB, dB, X, prec=prec, tol=1e-12, max_iter=30, info=info)
+    assert int(info.item()) == 0
+    assert berr <= 1e-12
+    Xh = X.t().cpu().numpy().T
+    # the oracle recomputes the backward error from scratch
+    assert oracle.solve_backward_error(N, NB, A0, Xh, Bh) <= 1e-12
+    # and the solution agrees with the FP64 oracle solve within kappa * 1e-12
+    L = A0.copy()
+    assert oracle.potrf_tiled(N, NB, L) == 0
+    Xref = oracle.potrs_tiled(N, NB, L, Bh)
+    assert np.abs(Xh - Xref).max() / np.abs(Xref).max() <= kappa * 1e-11
+    # A is left unchanged
+    assert np.array_equal(dA.cpu().numpy(), A0)
+    if prec == 1:
+        assert its >= 1  # FP32 factor needs refinement to reach 1e-12
+
+
+def test_dposv_mixed_nonspd_reports_info(dev, oracle):
+    import ttg_amd.chol as C
+
+    N, NB = 1024, 256
+    A0 = oracle.spd_tiles(N, NB, 1.0, 3)
+    oracle.tile_view(A0, N, NB, 1, 1)[10, 10] = -5.0
+    dA = to_dev(A0, dev)
+    dB = torch.ones(N, 1, dtype=torch.float64, device=dev)
+    X = torch.zeros(N, 1, dtype=torch.float64, device=dev)
+    info = torch.zeros(1, dtype=torch.int32, device=dev)
+    its, berr = C.dposv_mixed(dA, N, NB, dB, X, prec=1, info=info)
+    assert int(info.item()) == NB + 11

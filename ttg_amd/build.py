@@ -17,6 +17,7 @@ SOURCES = [
     "chol_tiled.hip",
     "chol_solve.hip",
     "chol_profile.hip",
+    "chol_mixed.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

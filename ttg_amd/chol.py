@@ -11,7 +11,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libttg_chol.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "ttg_chol.h")
 
-OK, EINVAL, EHIP, ENOTSUP, ENOMEM, ESTATE = 0, -1, -2, -3, -4, -5
+OK, EINVAL, EHIP, ENOTSUP, ENOMEM, ESTATE, ENOCONV = 0, -1, -2, -3, -4, -5, -6
+PREC_FP64, PREC_FP32 = 0, 1
 F_LOOKAHEAD, F_GRAPH, F_PROFILE = 0x1, 0x2, 0x4
 PROF_UPDATE, PROF_TRSM, PROF_POTRF = 0, 1, 2
 F_DEFAULT = F_LOOKAHEAD | F_GRAPH
@@ -39,6 +40,8 @@ SIGNATURES = {
     "ttg_chol_spd_tiled": (_i, [_p, _i, _i, _d, _ull, _p, _p]),
     "ttg_chol_selftest_mfma_layout": (_i, [_p, _i, _p]),
     "ttg_chol_profile_read": (_i, [_i, _p, _p, _p]),
+    "ttg_chol_profile_enable": (_i, [_i]),
+    "ttg_chol_dposv_mixed": (_i, [_p, _i, _i, _p, _i, _p, _i, _p, _i, _i, _d, _i, _p, _p, _p]),
 }
 
 
@@ -214,6 +217,28 @@ def profile_read(kind):
     fl = ctypes.c_double()
     _check(lib().ttg_chol_profile_read(kind, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(fl)), "profile_read")
     return ms.value, n.value, fl.value
+
+
+def dposv_mixed(A, N, NB, B, X, prec=PREC_FP32, tol=1e-12, max_iter=30, info=None, stream=None, check=True):
+    """Mixed-precision solve with FP64 refinement; returns (iterations, backward error)."""
+    import torch
+
+    _require_cuda(A, B, X)
+    if info is None:
+        info = torch.zeros(1, dtype=torch.int32, device=A.device)
+    nrhs = 1 if B.dim() == 1 else B.shape[1]
+    it = ctypes.c_int()
+    be = ctypes.c_double()
+    rc = lib().ttg_chol_dposv_mixed(_stream(stream), N, NB, _ptr(A), nrhs, _ptr(B), _ld(B), _ptr(X), _ld(X), prec,
+                                    tol, max_iter, _ptr(info), ctypes.byref(it), ctypes.byref(be))
+    if rc == ENOCONV and not check:
+        return it.value, be.value
+    _check(rc, "dposv_mixed")
+    return it.value, be.value
+
+
+def profile_enable(on):
+    _check(lib().ttg_chol_profile_enable(1 if on else 0), "profile_enable")
 
 
 def plgsy_tiled(tiles, N, NB, bump, seed, stream=None):

@@ -12,6 +12,7 @@
 #include "chol_internal.hpp"
 #include "chol_kernels.hpp"
 #include "chol_tiled.hpp"
+#include "chol_tiled_plan.hpp"
 
 namespace ttgc {
 
@@ -117,7 +118,8 @@ __global__ void add_diag_kernel(int N, int NB, int nt, double alpha, double *til
 // Reset *d_info and set the padding of the edge tiles (rows/cols >= N of the
 // last tile row) to zero, with ones on the padded diagonal, so every tile op
 // of the factorization can run on full NB x NB tiles.
-__global__ void prepare_kernel(int N, int NB, int nt, double *tiles, int *d_info) {
+template <typename T>
+__global__ void prepare_kernel(int N, int NB, int nt, T *tiles, int *d_info) {
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g == 0 && d_info) *d_info = 0;
   const int rem = N - (nt - 1) * NB;  // valid rows of the last tile row
@@ -128,11 +130,11 @@ __global__ void prepare_kernel(int N, int NB, int nt, double *tiles, int *d_info
   const int n = (int)(g / per);
   const long e = g % per;
   const int r = (int)(e % NB), c = (int)(e / NB);
-  double *t = tiles + tile_index(nt - 1, n, nt) * per;
+  T *t = tiles + tile_index(nt - 1, n, nt) * per;
   if (n < nt - 1) {
-    if (r >= rem) t[e] = 0.0;
+    if (r >= rem) t[e] = T(0);
   } else {
-    if (r >= rem || c >= rem) t[e] = (r == c) ? 1.0 : 0.0;
+    if (r >= rem || c >= rem) t[e] = (r == c) ? T(1) : T(0);
   }
 }
 
@@ -162,12 +164,19 @@ __global__ __launch_bounds__(64) void mfma_probe_kernel(double *out) {
   for (int r = 0; r < 4; ++r) out[(m * 64 + l) * 4 + r] = (double)d[r];
 }
 
-hipError_t launch_prepare(hipStream_t s, int N, int NB, double *tiles, int *d_info) {
+template <typename T>
+hipError_t launch_prepare_t(hipStream_t s, int N, int NB, T *tiles, int *d_info) {
   const int nt = (N + NB - 1) / NB;
   const long work = (long)NB * NB * nt;
   const long blocks = std::max(1L, (work + 255) / 256);
-  hipLaunchKernelGGL(prepare_kernel, dim3((unsigned)blocks), dim3(256), 0, s, N, NB, nt, tiles, d_info);
+  hipLaunchKernelGGL(prepare_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, s, N, NB, nt, tiles, d_info);
   return hipGetLastError();
+}
+template hipError_t launch_prepare_t<double>(hipStream_t, int, int, double *, int *);
+template hipError_t launch_prepare_t<float>(hipStream_t, int, int, float *, int *);
+
+hipError_t launch_prepare(hipStream_t s, int N, int NB, double *tiles, int *d_info) {
+  return launch_prepare_t<double>(s, N, NB, tiles, d_info);
 }
 
 }  // namespace ttgc
@@ -222,9 +231,8 @@ int ttg_chol_spd_tiled(void *stream, int N, int NB, double alpha, unsigned long 
   if (!fast.empty()) TTGC_HIP(hipMemcpy(d, fast.data(), sizeof(UpdOp<double>) * fast.size(), hipMemcpyHostToDevice));
   if (!slow.empty())
     TTGC_HIP(hipMemcpy(d + fast.size(), slow.data(), sizeof(UpdOp<double>) * slow.size(), hipMemcpyHostToDevice));
-  UpdOp<double> none{};
-  TTGC_HIP(launch_update<double>(s, d, (int)fast.size(), none, NB, NB, c, 0.0, true, nullptr));
-  TTGC_HIP(launch_update<double>(s, d + fast.size(), (int)slow.size(), none, NB, NB, c, 0.0, false, nullptr));
+  TTGC_HIP(launch_gen_gemm(s, d, (int)fast.size(), NB, NB, c, 0.0, true));
+  TTGC_HIP(launch_gen_gemm(s, d + fast.size(), (int)slow.size(), NB, NB, c, 0.0, false));
   hipLaunchKernelGGL(add_diag_kernel, dim3((unsigned)(((long)N + 255) / 256)), dim3(256), 0, s, N, NB, nt, alpha,
                      tiles);
   TTGC_HIP(hipGetLastError());

@@ -320,9 +320,8 @@ struct UpdCfg<float, TB, true> {
 // FAST applies to both operands for the 128 x 128 shape and to the B operand
 // only for the skinny shape (whose A rows are predicated).
 template <typename T, bool FAST, bool TB, bool SKINNY>
-__global__ __launch_bounds__(256) void update_kernel(const UpdOp<T> *__restrict__ ops, UpdOp<T> one, int bpo, int nbn,
+__device__ __forceinline__ void update_body(const UpdOp<T> *__restrict__ ops, UpdOp<T> one, int bpo, int nbn,
                                                      T alpha, T beta, const int *__restrict__ d_abort) {
-  if (d_abort && *d_abort) return;
   using Core = typename UpdCfg<T, TB, SKINNY>::Core;
   constexpr int BM = SKINNY ? 32 : 128;
   constexpr int BN = 128;
@@ -370,6 +369,21 @@ __global__ __launch_bounds__(256) void update_kernel(const UpdOp<T> *__restrict_
         }
       }
     }
+}
+
+template <typename T, bool FAST, bool TB, bool SKINNY>
+__global__ __launch_bounds__(256) void update_kernel(const UpdOp<T> *__restrict__ ops, UpdOp<T> one, int bpo, int nbn,
+                                                     T alpha, T beta, const int *__restrict__ d_abort) {
+  if (d_abort && *d_abort) return;
+  update_body<T, FAST, TB, SKINNY>(ops, one, bpo, nbn, alpha, beta, d_abort);
+}
+
+// The same update, as its own kernel symbol for the matrix generator (so
+// profiles keep generation apart from the factorization).
+template <typename T, bool FAST>
+__global__ __launch_bounds__(256) void gen_gemm_kernel(const UpdOp<T> *__restrict__ ops, UpdOp<T> one, int bpo,
+                                                       int nbn, T alpha, T beta) {
+  update_body<T, FAST, false, false>(ops, one, bpo, nbn, alpha, beta, nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -441,6 +455,23 @@ hipError_t launch_update(hipStream_t s, const UpdOp<T> *ops, int nops, const Upd
   if (nops <= 0 || max_m <= 0 || max_n <= 0) return hipSuccess;
   if (max_m <= 32) return launch_update_shape<T, true>(s, ops, nops, one, max_m, max_n, alpha, beta, fast, d_abort, tb);
   return launch_update_shape<T, false>(s, ops, nops, one, max_m, max_n, alpha, beta, fast, d_abort, tb);
+}
+
+hipError_t launch_gen_gemm(hipStream_t s, const UpdOp<double> *ops, int nops, int max_m, int max_n, double alpha,
+                           double beta, bool fast) {
+  if (nops <= 0 || max_m <= 0 || max_n <= 0) return hipSuccess;
+  const int nbm = (max_m + 127) / 128, nbn = (max_n + 127) / 128;
+  const int bpo = nbm * nbn;
+  const long grid = (long)nops * bpo;
+  if (grid > 0x7fffffffL) return hipErrorInvalidConfiguration;
+  UpdOp<double> none{};
+  if (fast)
+    hipLaunchKernelGGL((gen_gemm_kernel<double, true>), dim3((unsigned)grid), dim3(256), 0, s, ops, none, bpo, nbn,
+                       alpha, beta);
+  else
+    hipLaunchKernelGGL((gen_gemm_kernel<double, false>), dim3((unsigned)grid), dim3(256), 0, s, ops, none, bpo, nbn,
+                       alpha, beta);
+  return hipGetLastError();
 }
 
 template hipError_t launch_trsm<double>(hipStream_t, const TrsmOp<double> *, int, const TrsmOp<double> &, int,

@@ -62,6 +62,11 @@ hipError_t launch_trsm(hipStream_t s, const TrsmOp<T> *ops, int nops, const Trsm
 template <typename T>
 hipError_t launch_update(hipStream_t s, const UpdOp<T> *ops, int nops, const UpdOp<T> &one, int max_m, int max_n,
                          T alpha, T beta, bool fast, const int *d_abort, bool tb = false);
+// C := alpha A B^T + beta C for the SPD generator (same code as the update,
+// separate kernel symbol).
+hipError_t launch_gen_gemm(hipStream_t s, const UpdOp<double> *ops, int nops, int max_m, int max_n, double alpha,
+                           double beta, bool fast);
+
 // Whether an update op may take the unpredicated vector path: full 128-wide
 // column blocks, k a multiple of 32, 16-byte aligned operands; plus full
 // 128-row blocks unless the op is skinny (m <= 32, rows always predicated).

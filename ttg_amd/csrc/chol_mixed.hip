@@ -1,0 +1,267 @@
+// Mixed-precision solve with FP64 iterative refinement (no reference
+// counterpart; the target of the tiled factorization on MI355X).
+//
+//   1. A (FP64 lower tiles, kept intact) is copied into a working factor in
+//      the working precision (FP32, or FP64 for the plain path).
+//   2. The tiled factorization of the examples/potrf graph runs on it: the
+//      diagonal POTRF always computes in FP64 (tile converted on the fly),
+//      TRSM and the SYRK/GEMM trailing updates in the working precision.
+//   3. X0 = (L L^T)^{-1} B in the working precision.
+//   4. Refinement in FP64:  R = B - A X (FP64, on the original A), stop when
+//      max_j ||r_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf) <= tol,
+//      else D = (L L^T)^{-1} R (working precision) and X += D (FP64).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "chol_internal.hpp"
+#include "chol_kernels.hpp"
+#include "chol_solve.hpp"
+#include "chol_tiled.hpp"
+#include "chol_tiled_plan.hpp"
+
+namespace ttgc {
+
+// Working copy of the lower tiles in precision T (padding is set later by
+// the factorization's prepare step).
+template <typename T>
+__global__ void convert_tiles_kernel(long n, const double *__restrict__ A, T *__restrict__ W) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    W[i] = (T)A[i];
+}
+
+// Per right-hand side j: berr_j = max|R_j| / (anorm * max|X_j| + max|B_j|).
+__global__ __launch_bounds__(256) void berr_kernel(int N, const double *__restrict__ R, long ldr,
+                                                   const double *__restrict__ X, long ldx,
+                                                   const double *__restrict__ B, long ldb,
+                                                   const double *__restrict__ anorm, double *__restrict__ out) {
+  const int j = blockIdx.x;
+  __shared__ double sr[256], sx[256], sb[256];
+  double r = 0.0, x = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < N; i += 256) {
+    const double rv = fabs(R[i + (long)j * ldr]);
+    r = (rv > r || rv != rv) ? rv : r;  // NaN propagates
+    x = fmax(x, fabs(X[i + (long)j * ldx]));
+    b = fmax(b, fabs(B[i + (long)j * ldb]));
+  }
+  sr[threadIdx.x] = r;
+  sx[threadIdx.x] = x;
+  sb[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const double o = sr[threadIdx.x + w];
+      sr[threadIdx.x] = (o > sr[threadIdx.x] || o != o) ? o : sr[threadIdx.x];
+      sx[threadIdx.x] = fmax(sx[threadIdx.x], sx[threadIdx.x + w]);
+      sb[threadIdx.x] = fmax(sb[threadIdx.x], sb[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = sr[0] / (anorm[0] * sx[0] + sb[0]);
+}
+
+// anorm = max_i (-rowsum_i) where rowsum = 0 - |A| 1
+__global__ __launch_bounds__(1024) void max_neg_kernel(int N, const double *__restrict__ v, double *__restrict__ out) {
+  __shared__ double s[1024];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < N; i += 1024) m = fmax(m, -v[i]);
+  s[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = s[0];
+}
+
+// X(i, j) (+)= Bt(j, i) for the first N rows (Bt holds nrhs x Np, transposed).
+template <typename TS, bool ADD>
+__global__ void from_rows_acc_kernel(int N, int nrhs, const TS *__restrict__ Bt, double *__restrict__ X, long ldx) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long)N * nrhs) return;
+  const int i = (int)(g % N), j = (int)(g / N);
+  const double v = (double)Bt[j + (long)i * nrhs];
+  if (ADD)
+    X[i + (long)j * ldx] += v;
+  else
+    X[i + (long)j * ldx] = v;
+}
+
+struct MixedPlanBase {
+  virtual ~MixedPlanBase() = default;
+};
+
+template <typename T>
+struct MixedPlan : MixedPlanBase {
+  int N = 0, NB = 0, nrhs = 0;
+  DevBuf W;      // working factor tiles (T)
+  DevBuf R;      // residual, N x nrhs (FP64)
+  DevBuf stats;  // anorm, berr[nrhs]
+  DevBuf ones, zeros, rows;
+  std::unique_ptr<TiledPlanT<T>> fplan;
+  SolvePlan<T> *splan = nullptr;
+  std::vector<double> hstats;
+
+  T *w() { return static_cast<T *>(W.ptr); }
+  double *r() { return static_cast<double *>(R.ptr); }
+  double *st() { return static_cast<double *>(stats.ptr); }
+
+  int build() {
+    const int nt = (N + NB - 1) / NB;
+    const size_t ntile = (size_t)nt * (nt + 1) / 2 * NB * NB;
+    TTGC_HIP(W.reserve(sizeof(T) * ntile));
+    TTGC_HIP(R.reserve(sizeof(double) * (size_t)N * nrhs));
+    TTGC_HIP(stats.reserve(sizeof(double) * (1 + nrhs)));
+    TTGC_HIP(ones.reserve(sizeof(double) * (size_t)N));
+    TTGC_HIP(zeros.reserve(sizeof(double) * (size_t)N));
+    TTGC_HIP(rows.reserve(sizeof(double) * (size_t)N));
+    {
+      std::vector<double> h(N, 1.0);
+      TTGC_HIP(hipMemcpy(ones.ptr, h.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+      std::fill(h.begin(), h.end(), 0.0);
+      TTGC_HIP(hipMemcpy(zeros.ptr, h.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    }
+    fplan.reset(new TiledPlanT<T>);
+    fplan->N = N;
+    fplan->NB = NB;
+    fplan->tiles = w();
+    fplan->flags = TTG_CHOL_F_DEFAULT;
+    int rc = fplan->build();
+    if (rc) return rc;
+    splan = get_solve_plan<T>(N, NB, w(), nrhs, &rc);
+    if (!splan) return rc;
+    hstats.resize(1 + nrhs);
+    return TTG_CHOL_OK;
+  }
+
+  // Solve into X (overwrite when !add, accumulate otherwise) the rhs Y.
+  int solve(hipStream_t s, const double *Y, long ldy, double *X, long ldx, bool add) {
+    T *bt = splan->bt();
+    TTGC_HIP(launch_to_rows<double, T>(s, N, splan->Np, nrhs, Y, ldy, bt));
+    int rc = splan->run_sweeps(s);
+    if (rc) return rc;
+    const long n = (long)N * nrhs;
+    if (add)
+      hipLaunchKernelGGL((from_rows_acc_kernel<T, true>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, N, nrhs,
+                         bt, X, ldx);
+    else
+      hipLaunchKernelGGL((from_rows_acc_kernel<T, false>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, N,
+                         nrhs, bt, X, ldx);
+    TTGC_HIP(hipGetLastError());
+    return TTG_CHOL_OK;
+  }
+
+  int run(hipStream_t s, const double *A, const double *B, long ldb, double *X, long ldx, double tol, int max_iter,
+          int *d_info, int *iters, double *berr) {
+    const int nt = (N + NB - 1) / NB;
+    const long ntile = (long)nt * (nt + 1) / 2 * NB * NB;
+    // 1. working copy
+    {
+      const long blocks = std::min<long>((ntile + 255) / 256, 65536L);
+      hipLaunchKernelGGL(convert_tiles_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, s, ntile, A, w());
+      TTGC_HIP(hipGetLastError());
+    }
+    // ||A||_inf: rows = 0 - |A| 1
+    {
+      int rc = residual_fp64(s, N, NB, A, 1, static_cast<double *>(ones.ptr), N, static_cast<double *>(zeros.ptr), N,
+                             static_cast<double *>(rows.ptr), N, true);
+      if (rc) return rc;
+      hipLaunchKernelGGL(max_neg_kernel, dim3(1), dim3(1024), 0, s, N, static_cast<double *>(rows.ptr), st());
+      TTGC_HIP(hipGetLastError());
+    }
+    // 2. factor
+    int rc = fplan->run(s, d_info);
+    if (rc) return rc;
+    int hinfo = 0;
+    TTGC_HIP(hipMemcpyAsync(&hinfo, d_info, sizeof(int), hipMemcpyDeviceToHost, s));
+    TTGC_HIP(hipStreamSynchronize(s));
+    if (hinfo != 0) {
+      *iters = 0;
+      *berr = INFINITY;
+      return TTG_CHOL_OK;  // caller reads *d_info
+    }
+    // 3. initial solve
+    rc = solve(s, B, ldb, X, ldx, false);
+    if (rc) return rc;
+    // 4. refinement
+    double e = INFINITY;
+    int it = 0;
+    for (;; ++it) {
+      rc = residual_fp64(s, N, NB, A, nrhs, X, ldx, B, ldb, r(), N, false);
+      if (rc) return rc;
+      hipLaunchKernelGGL(berr_kernel, dim3(nrhs), dim3(256), 0, s, N, r(), (long)N, X, ldx, B, ldb, st(), st() + 1);
+      TTGC_HIP(hipGetLastError());
+      TTGC_HIP(hipMemcpyAsync(hstats.data(), st(), sizeof(double) * (1 + nrhs), hipMemcpyDeviceToHost, s));
+      TTGC_HIP(hipStreamSynchronize(s));
+      e = 0.0;
+      for (int j = 0; j < nrhs; ++j) e = (hstats[1 + j] > e || std::isnan(hstats[1 + j])) ? hstats[1 + j] : e;
+      if (e <= tol || it >= max_iter) break;
+      rc = solve(s, r(), N, X, ldx, true);
+      if (rc) return rc;
+    }
+    *iters = it;
+    *berr = e;
+    return (e <= tol) ? TTG_CHOL_OK : TTG_CHOL_ENOCONV;
+  }
+};
+
+static std::mutex g_mixed_mu;
+static std::map<std::tuple<int, int, int, int>, std::unique_ptr<MixedPlanBase>> g_mixed;
+
+}  // namespace ttgc
+
+using namespace ttgc;
+
+extern "C" int ttg_chol_dposv_mixed(void *stream, int N, int NB, const double *A, int nrhs, const double *B, int ldb,
+                                    double *X, int ldx, int prec, double tol, int max_iter, int *d_info, int *iters,
+                                    double *berr) {
+  clear_error();
+  if (N <= 0 || NB <= 0 || NB > TTG_CHOL_MAX_TILE || NB % 32 != 0 || nrhs <= 0 || nrhs > 32 ||
+      ldb < std::max(1, N) || ldx < std::max(1, N) || !A || !B || !X || !d_info || !iters || !berr ||
+      max_iter < 0 || !(tol > 0.0)) {
+    set_error("dposv_mixed: bad argument (N=%d NB=%d nrhs=%d ldb=%d ldx=%d)", N, NB, nrhs, ldb, ldx);
+    return TTG_CHOL_EINVAL;
+  }
+  if (prec != TTG_CHOL_PREC_FP64 && prec != TTG_CHOL_PREC_FP32) {
+    set_error("dposv_mixed: precision %d not supported", prec);
+    return TTG_CHOL_ENOTSUP;
+  }
+  hipStream_t s = as_stream(stream);
+  std::lock_guard<std::mutex> lk(g_mixed_mu);
+  auto key = std::make_tuple(N, NB, nrhs, prec);
+  auto &p = g_mixed[key];
+  if (prec == TTG_CHOL_PREC_FP64) {
+    if (!p) {
+      auto *q = new MixedPlan<double>;
+      p.reset(q);
+      q->N = N;
+      q->NB = NB;
+      q->nrhs = nrhs;
+      int rc = q->build();
+      if (rc) {
+        g_mixed.erase(key);
+        return rc;
+      }
+    }
+    return static_cast<MixedPlan<double> *>(p.get())->run(s, A, B, ldb, X, ldx, tol, max_iter, d_info, iters, berr);
+  }
+  if (!p) {
+    auto *q = new MixedPlan<float>;
+    p.reset(q);
+    q->N = N;
+    q->NB = NB;
+    q->nrhs = nrhs;
+    int rc = q->build();
+    if (rc) {
+      g_mixed.erase(key);
+      return rc;
+    }
+  }
+  return static_cast<MixedPlan<float> *>(p.get())->run(s, A, B, ldb, X, ldx, tol, max_iter, d_info, iters, berr);
+}

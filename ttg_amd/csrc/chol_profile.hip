@@ -16,6 +16,9 @@ struct ProfRec {
 static std::mutex g_prof_mu;
 static std::vector<ProfRec> g_prof;
 static size_t g_prof_n = 0;
+static bool g_prof_on = false;
+
+bool prof_enabled() { return g_prof_on; }
 
 int prof_begin(hipStream_t s) {
   std::lock_guard<std::mutex> lk(g_prof_mu);
@@ -42,6 +45,11 @@ void prof_end(int slot, hipStream_t s, int kind, double flops) {
 }  // namespace ttgc
 
 using namespace ttgc;
+
+extern "C" int ttg_chol_profile_enable(int on) {
+  g_prof_on = on != 0;
+  return TTG_CHOL_OK;
+}
 
 extern "C" int ttg_chol_profile_read(int kind, double *ms_total, long long *launches, double *flops) {
   clear_error();

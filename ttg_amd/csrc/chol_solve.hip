@@ -224,6 +224,7 @@ template SolvePlan<float> *get_solve_plan<float>(int, int, const float *, int, i
 constexpr int kRB = 64;  // LDS block of A
 
 // W layout: [tile][slot 0/1][kRhsChunk][NB]
+template <bool ABS>
 __global__ __launch_bounds__(256) void residual_tile_kernel(int N, int NB, int nt, const double *__restrict__ tiles,
                                                             const double *__restrict__ X, long ldx, int nr,
                                                             double *__restrict__ W) {
@@ -267,7 +268,7 @@ __global__ __launch_bounds__(256) void residual_tile_kernel(int N, int NB, int n
         const int r = e & 63, c = e >> 6;
         const int gr = rs * kRB + r, gc = cs * kRB + c;
         double v = 0.0;
-        if (gr < mrows && gc < ncols && (!diag || gr >= gc)) v = A[gr + (long)gc * NB];
+        if (gr < mrows && gc < ncols && (!diag || gr >= gc)) v = ABS ? fabs(A[gr + (long)gc * NB]) : A[gr + (long)gc * NB];
         blk[c][r] = v;
       }
       for (int e = tid; e < kRB * kRhsChunk; e += 256) {
@@ -331,7 +332,7 @@ static std::mutex g_res_mu;
 static std::map<hipStream_t, std::unique_ptr<DevBuf>> g_res_ws;
 
 int residual_fp64(hipStream_t s, int N, int NB, const double *tiles, int nrhs, const double *X, long ldx,
-                  const double *B, long ldb, double *R, long ldr) {
+                  const double *B, long ldb, double *R, long ldr, bool abs_a) {
   const int nt = (N + NB - 1) / NB;
   const long ntiles = (long)nt * (nt + 1) / 2;
   DevBuf *ws;
@@ -345,8 +346,12 @@ int residual_fp64(hipStream_t s, int N, int NB, const double *tiles, int nrhs, c
   double *W = static_cast<double *>(ws->ptr);
   for (int j0 = 0; j0 < nrhs; j0 += kRhsChunk) {
     const int nr = std::min(kRhsChunk, nrhs - j0);
-    hipLaunchKernelGGL(residual_tile_kernel, dim3((unsigned)ntiles), dim3(256), 0, s, N, NB, nt, tiles,
-                       X + (long)j0 * ldx, ldx, nr, W);
+    if (abs_a)
+      hipLaunchKernelGGL(residual_tile_kernel<true>, dim3((unsigned)ntiles), dim3(256), 0, s, N, NB, nt, tiles,
+                         X + (long)j0 * ldx, ldx, nr, W);
+    else
+      hipLaunchKernelGGL(residual_tile_kernel<false>, dim3((unsigned)ntiles), dim3(256), 0, s, N, NB, nt, tiles,
+                         X + (long)j0 * ldx, ldx, nr, W);
     TTGC_HIP(hipGetLastError());
     const long n = (long)N * nr;
     hipLaunchKernelGGL(residual_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, N, NB, nt, W, nr,
@@ -399,7 +404,7 @@ int ttg_chol_dresidual_tiled(void *stream, int N, int NB, const double *tiles, i
     return TTG_CHOL_EINVAL;
   }
   if (N == 0 || nrhs == 0) return TTG_CHOL_OK;
-  return residual_fp64(as_stream(stream), N, NB, tiles, nrhs, X, ldx, B, ldb, R, ldr);
+  return residual_fp64(as_stream(stream), N, NB, tiles, nrhs, X, ldx, B, ldb, R, ldr, false);
 }
 
 }  // extern "C"

@@ -1,0 +1,208 @@
+// Plan of the tiled factorization (the examples/potrf task graph with the
+// launches grouped per panel step), templated on the storage type T of the
+// factor tiles: double (plain FP64) or float (mixed precision: FP64 POTRF
+// arithmetic on the diagonal tiles, FP32 TRSM and trailing updates).
+//
+// Step k of the graph is POTRF(k), TRSM(m,k) for m > k, and the updates
+// SYRK(k,m) / GEMM(m,n,k) for k < n <= m.  Every TRSM of a step is one
+// launch; the updates of a step are two launches:
+//   updA(k): the tiles of column k+1, which feed POTRF/TRSM(k+1);
+//   updB(k): the rest of the trailing matrix.
+// With lookahead, POTRF(k+1) and TRSM(k+1) run on a second stream as soon as
+// updA(k) is done, overlapping updB(k).  Per tile the update order over k is
+// the one the reference graph fixes (GEMM(m,n,0), GEMM(m,n,1), ...).  The
+// launch sequence is captured once into a hipGraph and replayed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "chol_internal.hpp"
+#include "chol_kernels.hpp"
+#include "chol_profile.hpp"
+#include "chol_tiled.hpp"
+
+namespace ttgc {
+
+template <typename T>
+hipError_t launch_prepare_t(hipStream_t s, int N, int NB, T *tiles, int *d_info);
+
+template <typename T>
+struct TiledPlanT {
+  int N = 0, NB = 0, NT = 0;
+  T *tiles = nullptr;
+  unsigned flags = 0;
+  DevBuf d_ops, d_dinv, d_info;
+  size_t o_potrf = 0, o_trsm = 0, o_upd = 0;
+  std::vector<int> trsm_off, trsm_cnt, updA_off, updA_cnt, updB_off, updB_cnt;
+  hipStream_t s_main = nullptr, s_look = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr, ev_panel = nullptr, ev_col = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int *bound_info = nullptr;  // the d_info the captured graph writes
+  int *desc_info = nullptr;   // the d_info stored in the POTRF descriptors
+
+  ~TiledPlanT() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (hipEvent_t e : {ev_in, ev_out, ev_panel, ev_col})
+      if (e) (void)hipEventDestroy(e);
+    if (s_main) (void)hipStreamDestroy(s_main);
+    if (s_look) (void)hipStreamDestroy(s_look);
+    for (DevBuf *b : {&d_ops, &d_dinv, &d_info})
+      if (b->ptr) (void)hipFree(b->ptr);
+  }
+
+  PotrfOpT<T> *potrf_ops() { return reinterpret_cast<PotrfOpT<T> *>(static_cast<char *>(d_ops.ptr) + o_potrf); }
+  TrsmOp<T> *trsm_ops() { return reinterpret_cast<TrsmOp<T> *>(static_cast<char *>(d_ops.ptr) + o_trsm); }
+  UpdOp<T> *upd_ops() { return reinterpret_cast<UpdOp<T> *>(static_cast<char *>(d_ops.ptr) + o_upd); }
+  T *tile(int m, int n) { return tiles + tile_index(m, n, NT) * (long)NB * NB; }
+
+  int build() {
+    NT = (N + NB - 1) / NB;
+    const int dstride = ((NB + 31) / 32) * 32 * 32;
+    TTGC_HIP(d_dinv.reserve(sizeof(T) * dstride));
+    TTGC_HIP(d_info.reserve(sizeof(int)));
+    T *dinv = static_cast<T *>(d_dinv.ptr);
+    std::vector<PotrfOpT<T>> potrf;
+    std::vector<TrsmOp<T>> trsm;
+    std::vector<UpdOp<T>> upd;
+    // every tile is a full NB x NB block once the edge padding is set
+    for (int k = 0; k < NT; ++k) {
+      potrf.push_back(PotrfOpT<T>{tile(k, k), dinv, nullptr, NB, NB, k * NB, 0});
+      trsm_off.push_back((int)trsm.size());
+      for (int m = k + 1; m < NT; ++m) trsm.push_back(TrsmOp<T>{tile(k, k), tile(m, k), dinv, NB, NB, NB, NB});
+      trsm_cnt.push_back((int)trsm.size() - trsm_off.back());
+      updA_off.push_back((int)upd.size());
+      if (k + 1 < NT)
+        for (int m = k + 1; m < NT; ++m)
+          upd.push_back(UpdOp<T>{tile(m, k), tile(k + 1, k), tile(m, k + 1), NB, NB, NB, NB, NB, NB, m == k + 1 ? 1 : 0});
+      updA_cnt.push_back((int)upd.size() - updA_off.back());
+      updB_off.push_back((int)upd.size());
+      for (int n = k + 2; n < NT; ++n)
+        for (int m = n; m < NT; ++m)
+          upd.push_back(UpdOp<T>{tile(m, k), tile(n, k), tile(m, n), NB, NB, NB, NB, NB, NB, m == n ? 1 : 0});
+      updB_cnt.push_back((int)upd.size() - updB_off.back());
+    }
+    auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+    o_potrf = 0;
+    o_trsm = align(potrf.size() * sizeof(PotrfOpT<T>));
+    o_upd = align(o_trsm + trsm.size() * sizeof(TrsmOp<T>));
+    const size_t total = align(o_upd + upd.size() * sizeof(UpdOp<T>));
+    TTGC_HIP(d_ops.reserve(total));
+    std::vector<char> host(total);
+    memcpy(host.data() + o_potrf, potrf.data(), potrf.size() * sizeof(PotrfOpT<T>));
+    if (!trsm.empty()) memcpy(host.data() + o_trsm, trsm.data(), trsm.size() * sizeof(TrsmOp<T>));
+    if (!upd.empty()) memcpy(host.data() + o_upd, upd.data(), upd.size() * sizeof(UpdOp<T>));
+    TTGC_HIP(hipMemcpy(d_ops.ptr, host.data(), total, hipMemcpyHostToDevice));
+    TTGC_HIP(hipStreamCreateWithFlags(&s_main, hipStreamNonBlocking));
+    TTGC_HIP(hipStreamCreateWithFlags(&s_look, hipStreamNonBlocking));
+    for (hipEvent_t *e : {&ev_in, &ev_out, &ev_panel, &ev_col})
+      TTGC_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    return TTG_CHOL_OK;
+  }
+
+  // Point the POTRF descriptors at `info` (host round trip only when it changes).
+  int bind_info(int *info) {
+    if (desc_info == info) return TTG_CHOL_OK;
+    std::vector<PotrfOpT<T>> ops(NT);
+    TTGC_HIP(hipMemcpy(ops.data(), potrf_ops(), sizeof(PotrfOpT<T>) * NT, hipMemcpyDeviceToHost));
+    for (auto &o : ops) o.info = info;
+    TTGC_HIP(hipMemcpy(potrf_ops(), ops.data(), sizeof(PotrfOpT<T>) * NT, hipMemcpyHostToDevice));
+    desc_info = info;
+    return TTG_CHOL_OK;
+  }
+
+  // Enqueue the whole factorization on s_main (and s_look), writing *info.
+  // With `prof`, every launch is bracketed by timing events.
+  int enqueue(int *info, bool prof) {
+    const bool fast = NB % 128 == 0;
+    const T m1 = T(-1), p1 = T(1);
+    const double b3 = (double)NB * NB * NB;
+    UpdOp<T> unone{};
+    TrsmOp<T> tnone{};
+    PotrfOpT<T> pnone{};
+    TTGC_HIP(launch_prepare_t<T>(s_main, N, NB, tiles, info));
+    const bool la = (flags & TTG_CHOL_F_LOOKAHEAD) != 0;
+    hipStream_t sp = la ? s_look : s_main;
+    if (la) TTGC_HIP(hipEventRecord(ev_col, s_main));
+    auto upd = [&](int off, int cnt, int nsyrk) -> int {
+      const int slot = prof ? prof_begin(s_main) : -1;
+      TTGC_HIP(launch_update<T>(s_main, upd_ops() + off, cnt, unone, NB, NB, m1, p1, fast, info));
+      if (prof) prof_end(slot, s_main, PROF_UPDATE, (cnt - nsyrk) * 2.0 * b3 + nsyrk * (double)NB * NB * (NB + 1));
+      return TTG_CHOL_OK;
+    };
+    for (int k = 0; k < NT; ++k) {
+      if (la) TTGC_HIP(hipStreamWaitEvent(s_look, ev_col, 0));
+      int slot = prof ? prof_begin(sp) : -1;
+      TTGC_HIP(launch_potrf<T>(sp, potrf_ops() + k, 1, pnone, info));
+      if (prof) prof_end(slot, sp, PROF_POTRF, b3 / 3.0);
+      if (trsm_cnt[k]) {
+        slot = prof ? prof_begin(sp) : -1;
+        TTGC_HIP(launch_trsm<T>(sp, trsm_ops() + trsm_off[k], trsm_cnt[k], tnone, NB, info));
+        if (prof) prof_end(slot, sp, PROF_TRSM, trsm_cnt[k] * b3);
+      }
+      if (la) {
+        TTGC_HIP(hipEventRecord(ev_panel, s_look));
+        TTGC_HIP(hipStreamWaitEvent(s_main, ev_panel, 0));
+      }
+      if (updA_cnt[k]) {
+        int rc = upd(updA_off[k], updA_cnt[k], 1);
+        if (rc) return rc;
+      }
+      if (la) TTGC_HIP(hipEventRecord(ev_col, s_main));
+      if (updB_cnt[k]) {
+        int rc = upd(updB_off[k], updB_cnt[k], NT - k - 2);
+        if (rc) return rc;
+      }
+    }
+    if (la) {
+      TTGC_HIP(hipEventRecord(ev_panel, s_look));
+      TTGC_HIP(hipStreamWaitEvent(s_main, ev_panel, 0));
+    }
+    return TTG_CHOL_OK;
+  }
+
+  int run(hipStream_t user, int *info) {
+    int rc = bind_info(info);
+    if (rc) return rc;
+    const bool prof = (flags & TTG_CHOL_F_PROFILE) != 0 || prof_enabled();
+    if ((flags & TTG_CHOL_F_GRAPH) && !prof) {
+      if (!gexec || bound_info != info) {
+        if (gexec) {
+          TTGC_HIP(hipGraphExecDestroy(gexec));
+          gexec = nullptr;
+        }
+        if (graph) {
+          TTGC_HIP(hipGraphDestroy(graph));
+          graph = nullptr;
+        }
+        TTGC_HIP(hipStreamBeginCapture(s_main, hipStreamCaptureModeThreadLocal));
+        rc = enqueue(info, false);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s_main, &g);
+        if (rc != TTG_CHOL_OK) {
+          if (g) (void)hipGraphDestroy(g);
+          return rc;
+        }
+        TTGC_HIP(e);
+        graph = g;
+        TTGC_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        bound_info = info;
+      }
+      TTGC_HIP(hipGraphLaunch(gexec, user));
+      return TTG_CHOL_OK;
+    }
+    TTGC_HIP(hipEventRecord(ev_in, user));
+    TTGC_HIP(hipStreamWaitEvent(s_main, ev_in, 0));
+    rc = enqueue(info, prof);
+    if (rc != TTG_CHOL_OK) return rc;
+    TTGC_HIP(hipEventRecord(ev_out, s_main));
+    TTGC_HIP(hipStreamWaitEvent(user, ev_out, 0));
+    return TTG_CHOL_OK;
+  }
+};
+
+}  // namespace ttgc
