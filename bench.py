@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK = 8.0e12          # B/s
 FP64_PEAK = 78.6e12        # dense FP64 MFMA, flop/s
 FP32_PEAK = 157.3e12
+FP16_PEAK = 2.5e15         # dense FP16 MFMA
 
 
 def flops_potrf(n):
@@ -47,8 +48,9 @@ def parse():
     p.add_argument("--nb", type=int, default=512)
     p.add_argument("--nrhs", type=int, default=16)
     p.add_argument("--kappa", type=float, default=1e4)
-    p.add_argument("--mode", choices=["mixed", "fp64"], default="mixed",
-                   help="mixed: FP32 working factor + FP64 refinement (the metric); fp64: FP64 factor")
+    p.add_argument("--mode", choices=["f16x2", "fp32", "fp64"], default="f16x2",
+                   help="f16x2: FP32 factor with split-FP16 MFMA trailing updates + FP64 refinement (the metric); "
+                        "fp32: FP32 MFMA updates; fp64: plain FP64 factor")
     p.add_argument("--max-iter", type=int, default=30)
     p.add_argument("--tol", type=float, default=1e-12)
     p.add_argument("--seed", type=int, default=2024)
@@ -123,7 +125,7 @@ def main():
     anorm = float((-rs).max().item())
     del Aabs
 
-    prec = {"mixed": C.PREC_FP32, "fp64": C.PREC_FP64}
+    prec = {"f16x2": C.PREC_F16X2, "fp32": C.PREC_FP32, "fp64": C.PREC_FP64}
     iters = {}
 
     def make_step(mode):
@@ -162,7 +164,7 @@ def main():
 
     ms_step, wall, berr = timed(a.mode, a.steps, a.warmup)
     it_main = iters[a.mode][0]
-    other = "fp64" if a.mode == "mixed" else None
+    other = "fp64" if a.mode != "fp64" else None
     ms_fp64 = berr_fp64 = None
     if other:
         ms_fp64, _, berr_fp64 = timed(other, 1, 1)
@@ -185,12 +187,14 @@ def main():
         ms_u, n_u, fl_u = C.profile_read(C.PROF_UPDATE)
         ms_t, n_t, fl_t = C.profile_read(C.PROF_TRSM)
         ms_p, n_p, fl_p = C.profile_read(C.PROF_POTRF)
-        peak = FP32_PEAK if a.mode == "mixed" else FP64_PEAK
+        peak = {"f16x2": FP16_PEAK / 3.0, "fp32": FP32_PEAK, "fp64": FP64_PEAK}[a.mode]
         ach = fl_u / (ms_u * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": round(ach, 3), "peak": round(peak / 1e12, 1), "unit": "TFLOP/s",
                 "frac": round(ach / (peak / 1e12), 4), "traffic": None,
-                "kernel": "update_kernel<%s> (batched SYRK+GEMM trailing update)" % (
-                    "float" if a.mode == "mixed" else "double"),
+                "kernel": {"f16x2": "update_f16x2_kernel (batched SYRK+GEMM, 3 FP16 MFMA products per output; "
+                                    "peak = FP16 dense / 3, flops counted once)",
+                           "fp32": "update_kernel<float> (batched SYRK+GEMM trailing update)",
+                           "fp64": "update_kernel<double> (batched SYRK+GEMM trailing update)"}[a.mode],
                 "launches": n_u, "avg_launch_ms": round(ms_u / max(n_u, 1), 4),
                 "flops_per_launch": fl_u / max(n_u, 1),
                 "others": {"trsm_ms": round(ms_t, 2), "trsm_tflops": round(fl_t / max(ms_t, 1e-9) / 1e9, 2),
@@ -211,7 +215,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if a.mode == "mixed" else "f64",
+        "dtype": {"f16x2": "f16x2+f32+f64", "fp32": "f32+f64", "fp64": "f64"}[a.mode],
         "data": "synthetic: A = B B^T/N + alpha I, B(i,j) = k/127 with k uniform integer in [-127,127] "
                 "(seeded LCG), alpha for kappa ~ 1e4; 16 RHS uniform[-1,1]",
         "config": {"workload": f"SPD solve N={N} NB={NB} nrhs={nrhs} kappa~{a.kappa:g} (tiled POTRF graph)",

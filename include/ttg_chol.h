@@ -141,6 +141,9 @@ int ttg_chol_dresidual_tiled(void *stream, int N, int NB, const double *tiles, i
 /* Working precisions of ttg_chol_dposv_mixed. */
 #define TTG_CHOL_PREC_FP64 0 /* FP64 factor (time-to-solution reference) */
 #define TTG_CHOL_PREC_FP32 1 /* FP32 TRSM + trailing updates, FP64 diagonal POTRF */
+#define TTG_CHOL_PREC_F16X2 2 /* FP32 TRSM; trailing updates on FP16 MFMA with each panel tile split into
+                                 hi + lo FP16 parts under a per-tile power-of-two scale (3 products,
+                                 FP32 accumulation); FP64 diagonal POTRF; NB % 128 == 0 */
 
 /* Solve A X = B (A SPD, FP64 lower tiles, left unchanged; B, X N x nrhs,
  * nrhs <= 32) with a factorization in working precision `prec` followed by
@@ -166,6 +169,13 @@ int ttg_chol_dposv_mixed(void *stream, int N, int NB, const double *A, int nrhs,
  * every lower tile of the N x N symmetric random matrix with `bump` on the
  * diagonal, bit-identical to the reference generator. */
 int ttg_chol_plgsy_tiled(void *stream, int N, int NB, double bump, unsigned long long seed, double *tiles);
+
+/* CORE_dplgsy (examples/potrf/core_plgsy.h:63-150) for one m x n tile at
+ * global offset (m0, n0) of a gM-row matrix, on the device; bit-identical to
+ * the reference's host generator.  The device-task form of the PLGSY task
+ * (examples/potrf/plgsy.h:12-33). */
+int ttg_chol_dplgsy_tile(void *stream, double bump, int m, int n, double *A, int lda, int gM, int m0, int n0,
+                         unsigned long long seed);
 
 /* A = B B^T / N + alpha I into the lower tiles, with B(i,j) = k/127, k an
  * integer in [-127, 127] drawn from the PLASMA LCG stream position i + j*N.
@@ -194,9 +204,10 @@ int ttg_chol_profile_read(int kind, double *ms_total, long long *launches, doubl
  * with TTG_CHOL_F_PROFILE while `on` is nonzero. */
 int ttg_chol_profile_enable(int on);
 
-/* Lane-layout self test of the 16x16x4 MFMA instructions the kernels use
- * (kind 0: FP64, kind 1: FP32).  `out` (device, 64*64*4 doubles) receives,
- * for each one-hot B lane m, the four outputs of every lane.  The kernels'
+/* Lane-layout self test of the MFMA instructions the kernels use
+ * (kind 0: FP64 16x16x4, kind 1: FP32 16x16x4, kind 2: FP16 16x16x32).
+ * `out` (device; 64*64*4 doubles for kinds 0-1, 512*64*4 for kind 2)
+ * receives, for each one-hot B position, the four outputs of every lane.  The kernels'
  * operand/result index maps are checked against it by the test suite.
  * (No counterpart in the reference.) */
 int ttg_chol_selftest_mfma_layout(void *stream, int kind, double *out);

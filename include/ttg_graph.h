@@ -1,0 +1,34 @@
+/* ttg_graph.h -- C entry points of libttg_graph.so: the examples/potrf task
+ * graph (POTRF/TRSM/SYRK/GEMM template tasks with the reference's keys,
+ * successor rules and DPLASMA priorities) executed by the single-GPU TTG
+ * runtime of ttg_amd/include/ttg.h over libttg_chol's gfx950 tile kernels.
+ *
+ * The matrix is host memory in the lower tile-packed layout: tile (m, n),
+ * n <= m < NT = ceil(N / NB), starts at element (n*NT - n(n-1)/2 + m - n) NB^2
+ * and is NB x NB column-major with leading dimension NB; edge tiles hold
+ * N - m*NB valid rows / columns.
+ */
+#ifndef TTG_GRAPH_H
+#define TTG_GRAPH_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Factor the tiles in place with the task graph: the body of
+ * examples/potrf/testing_dpotrf.cc:227-262 (load -> make_potrf_ttg -> result
+ * writer, then fence).  stats (optional, 4 longs): tasks run, scheduling
+ * rounds, batched kernel launches, largest round.  Returns 0, -1 on a bad
+ * argument; a tile that is not positive definite aborts the process, as the
+ * reference's POTRF task does (potrf.h:160-164). */
+int ttg_graph_dpotrf(int N, int NB, double *tiles, long *stats);
+
+/* PLGSY(N, bump, seed) into the tiles with one device task per tile (the
+ * PLGSY template task of examples/potrf/plgsy.h:7-50; values bit-identical
+ * to CORE_dplgsy).  Returns 0, -1 on a bad argument. */
+int ttg_graph_dplgsy(int N, int NB, double bump, unsigned long seed, double *tiles);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

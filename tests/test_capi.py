@@ -58,3 +58,24 @@ def test_no_cpu_fallback():
 
     with pytest.raises(C.CholError):
         C.dpotrf_tile(torch.eye(4, dtype=torch.float64))
+
+
+def test_graph_library_exports_every_declared_symbol():
+    import ttg_amd.graph as G
+
+    text = open(G.HEADER).read()
+    names = sorted(set(re.findall(r"\b(ttg_graph_\w+)\s*\(", text)))
+    assert names == sorted(G.SIGNATURES)
+    L = G.lib()
+    assert all(hasattr(L, n) for n in names)
+
+
+def test_graph_rejects_bad_arguments_without_gpu():
+    import numpy as np
+
+    import ttg_amd.graph as G
+
+    t = np.zeros(4, dtype=np.float64)
+    assert G.lib().ttg_graph_dpotrf(0, 32, t.ctypes.data, None) == -1
+    assert G.lib().ttg_graph_dpotrf(100, 1024, t.ctypes.data, None) == -1
+    assert G.lib().ttg_graph_dplgsy(100, 0, 1.0, 1, t.ctypes.data) == -1

@@ -18,6 +18,7 @@ SOURCES = [
     "chol_solve.hip",
     "chol_profile.hip",
     "chol_mixed.hip",
+    "chol_f16.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -33,6 +34,39 @@ FLAGS = [
 ]
 
 
+EXAMPLES = os.path.join(HERE, "examples", "potrf")
+INCLUDES = ["-I" + os.path.join(HERE, "include"), "-I" + os.path.join(os.path.dirname(HERE), "include"),
+            "-I" + EXAMPLES]
+GRAPH_LIB = os.path.join(HERE, "libttg_graph.so")
+DRIVER = os.path.join(HERE, "bin", "testing_dpotrf")
+# host-only C++ (no device code): clang++ with the HIP runtime headers
+CXX = os.environ.get("TTG_CXX", "/opt/rocm/llvm/bin/clang++")
+HOST_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
+HOST_LIBS = ["-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"]
+
+
+def _run(cmd, what, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("compiler failed building " + what)
+
+
+def build_graph(verbose=False):
+    """The TTG-API runtime examples: libttg_graph.so and bin/testing_dpotrf
+    (host C++ over libttg_chol.so)."""
+    os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
+    link = ["-L" + HERE, "-lttg_chol", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/.."]
+    _run([CXX] + HOST_FLAGS + ["-shared"] + INCLUDES + ["-o", GRAPH_LIB + ".tmp",
+         os.path.join(EXAMPLES, "graph_capi.cc")] + link + HOST_LIBS, "libttg_graph.so", verbose)
+    os.replace(GRAPH_LIB + ".tmp", GRAPH_LIB)
+    _run([CXX] + HOST_FLAGS + INCLUDES + ["-o", DRIVER + ".tmp", os.path.join(EXAMPLES, "testing_dpotrf.cc")] + link +
+         HOST_LIBS, "testing_dpotrf", verbose)
+    os.replace(DRIVER + ".tmp", DRIVER)
+
+
 def build(verbose=False):
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     tmp = LIB + ".tmp"
@@ -44,6 +78,7 @@ def build(verbose=False):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building libttg_chol.so")
     os.replace(tmp, LIB)
+    build_graph(verbose)
     return LIB
 
 

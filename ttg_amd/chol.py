@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "libttg_chol.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "ttg_chol.h")
 
 OK, EINVAL, EHIP, ENOTSUP, ENOMEM, ESTATE, ENOCONV = 0, -1, -2, -3, -4, -5, -6
-PREC_FP64, PREC_FP32 = 0, 1
+PREC_FP64, PREC_FP32, PREC_F16X2 = 0, 1, 2
 F_LOOKAHEAD, F_GRAPH, F_PROFILE = 0x1, 0x2, 0x4
 PROF_UPDATE, PROF_TRSM, PROF_POTRF = 0, 1, 2
 F_DEFAULT = F_LOOKAHEAD | F_GRAPH
@@ -37,6 +37,7 @@ SIGNATURES = {
     "ttg_chol_dpotrs_tiled": (_i, [_p, _i, _i, _p, _i, _p, _i]),
     "ttg_chol_dresidual_tiled": (_i, [_p, _i, _i, _p, _i, _p, _i, _p, _i, _p, _i]),
     "ttg_chol_plgsy_tiled": (_i, [_p, _i, _i, _d, _ull, _p]),
+    "ttg_chol_dplgsy_tile": (_i, [_p, _d, _i, _i, _p, _i, _i, _i, _i, _ull]),
     "ttg_chol_spd_tiled": (_i, [_p, _i, _i, _d, _ull, _p, _p]),
     "ttg_chol_selftest_mfma_layout": (_i, [_p, _i, _p]),
     "ttg_chol_profile_read": (_i, [_i, _p, _p, _p]),
@@ -239,6 +240,14 @@ def dposv_mixed(A, N, NB, B, X, prec=PREC_FP32, tol=1e-12, max_iter=30, info=Non
 
 def profile_enable(on):
     _check(lib().ttg_chol_profile_enable(1 if on else 0), "profile_enable")
+
+
+def dplgsy_tile(A, bump, gM, m0, n0, seed, stream=None):
+    """One CORE_dplgsy tile (column-major device tensor A) at global offset (m0, n0)."""
+    _require_cuda(A)
+    m, n = A.shape
+    _check(lib().ttg_chol_dplgsy_tile(_stream(stream), float(bump), m, n, _ptr(A), _ld(A), gM, m0, n0, seed),
+           "dplgsy_tile")
 
 
 def plgsy_tiled(tiles, N, NB, bump, seed, stream=None):

@@ -89,9 +89,10 @@ struct StageLoader {
   static constexpr int VPC = ROWS / V;  // vectors per column
   static_assert(ROWS % V == 0, "rows must be a multiple of the vector width");
   static_assert(NTH % VPC == 0, "threads must cover whole columns");
-  static constexpr int CPP = NTH / VPC;  // columns per pass
-  static_assert(COLS % CPP == 0, "columns must be a multiple of columns per pass");
-  static constexpr int PASSES = COLS / CPP;
+  static constexpr int CPP = NTH / VPC;  // columns per pass (threads beyond COLS idle)
+  static_assert(COLS % CPP == 0 || CPP % COLS == 0, "columns and columns per pass must divide");
+  static constexpr int PASSES = COLS >= CPP ? COLS / CPP : 1;
+  static constexpr bool PARTIAL = CPP > COLS;
 
   vec_t r[PASSES];
 
@@ -100,6 +101,7 @@ struct StageLoader {
                                        int clim, int tid) {
     const int v = tid % VPC, c = tid / VPC;
     const int row = row0 + v * V;
+    if (PARTIAL && c >= COLS) return;
 #pragma unroll
     for (int p = 0; p < PASSES; ++p) {
       const int col = col0 + c + p * CPP;
@@ -115,6 +117,7 @@ struct StageLoader {
 
   __device__ __forceinline__ void store(T (*s)[LD], int tid) const {
     const int v = tid % VPC, c = tid / VPC;
+    if (PARTIAL && c >= COLS) return;
 #pragma unroll
     for (int p = 0; p < PASSES; ++p) *reinterpret_cast<vec_t *>(&s[c + p * CPP][v * V]) = r[p];
   }
@@ -130,9 +133,10 @@ struct StageLoaderT {
   static constexpr int VPR = COLS / V;  // vectors per row j
   static_assert(COLS % V == 0, "cols must be a multiple of the vector width");
   static_assert(NTH % VPR == 0, "threads must cover whole rows");
-  static constexpr int RPP = NTH / VPR;  // rows per pass
-  static_assert(ROWS % RPP == 0, "rows must be a multiple of rows per pass");
-  static constexpr int PASSES = ROWS / RPP;
+  static constexpr int RPP = NTH / VPR;  // rows per pass (threads beyond ROWS idle)
+  static_assert(ROWS % RPP == 0 || RPP % ROWS == 0, "rows and rows per pass must divide");
+  static constexpr int PASSES = ROWS >= RPP ? ROWS / RPP : 1;
+  static constexpr bool PARTIAL = RPP > ROWS;
 
   vec_t r[PASSES];
 
@@ -141,6 +145,7 @@ struct StageLoaderT {
                                        int clim, int tid) {
     const int v = tid % VPR, j = tid / VPR;
     const int k = col0 + v * V;
+    if (PARTIAL && j >= ROWS) return;
 #pragma unroll
     for (int p = 0; p < PASSES; ++p) {
       const int row = row0 + j + p * RPP;
@@ -156,6 +161,7 @@ struct StageLoaderT {
 
   __device__ __forceinline__ void store(T (*s)[LD], int tid) const {
     const int v = tid % VPR, j = tid / VPR;
+    if (PARTIAL && j >= ROWS) return;
 #pragma unroll
     for (int p = 0; p < PASSES; ++p)
 #pragma unroll

@@ -1,0 +1,31 @@
+// Split-FP16 panel and trailing-update kernels (internal API).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ttgc {
+
+// Split one FP32 panel tile (NB x NB, column-major, ld NB) into FP16 hi / lo
+// parts (row-major, NB x NB each) with a power-of-two scale.
+struct SplitOp {
+  const float *src;
+  uint16_t *hi, *lo;  // FP16 bit patterns
+  float *scale;
+};
+
+// C (FP32 tile, column-major) -= (A_hi B_hi^T + A_hi B_lo^T + A_lo B_hi^T) / (sa sb)
+// for n x n tiles with inner dimension k; A/B parts are row-major with row
+// stride ld (halves).
+struct F16UpdOp {
+  const uint16_t *ahi, *alo, *bhi, *blo;
+  const float *sa, *sb;
+  float *C;
+  int n, k, ld, ldc;
+  int lower;
+};
+
+hipError_t launch_split(hipStream_t s, const SplitOp *ops, int nops, int NB);
+hipError_t launch_update_f16x2(hipStream_t s, const F16UpdOp *ops, int nops, int NB, const int *d_abort);
+
+}  // namespace ttgc

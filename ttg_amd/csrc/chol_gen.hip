@@ -83,6 +83,24 @@ __global__ __launch_bounds__(256) void plgsy_kernel(int N, int NB, int nt, doubl
   }
 }
 
+// One tile of CORE_dplgsy (core_plgsy.h:63-150) for any tile position:
+// entry (gi, gj) takes the LCG value of stream position max + min * gM of its
+// global indices (the lower part column-wise, the upper part row-wise, as the
+// three branches of the reference generate them) and diagonal tiles add bump.
+__global__ __launch_bounds__(256) void plgsy_tile_kernel(double bump, int m, int n, double *A, long lda, long gM,
+                                                         long m0, long n0, unsigned long long seed) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64, lane = threadIdx.x & 63;
+  if (wave >= n) return;
+  const long gj = n0 + wave;
+  for (int ii = lane; ii < m; ii += 64) {
+    const long gi = m0 + ii;
+    const long hi = gi > gj ? gi : gj, lo = gi > gj ? gj : gi;
+    double v = plgsy_value(rnd64_jump((unsigned long long)hi + (unsigned long long)lo * (unsigned long long)gM, seed));
+    if (m0 == n0 && gi == gj) v += bump;
+    A[ii + (long)wave * lda] = v;
+  }
+}
+
 // Integer-valued B (N x N, column-major, stored as doubles).
 __global__ __launch_bounds__(256) void spd_b_kernel(long N, unsigned long long seed, double *B) {
   const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
@@ -164,6 +182,24 @@ __global__ __launch_bounds__(64) void mfma_probe_kernel(double *out) {
   for (int r = 0; r < 4; ++r) out[(m * 64 + l) * 4 + r] = (double)d[r];
 }
 
+// FP16 16x16x32 probe: A lane l element e holds l*8 + e + 1; B is one-hot
+// at (lane m / 8 ... ) -- block q sets B lane q / 8, element q % 8 to 1.
+// out[(q * 64 + l) * 4 + r] = D element r of lane l.
+__global__ __launch_bounds__(64) void mfma_probe_f16_kernel(double *out) {
+  typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+  const int q = blockIdx.x, l = threadIdx.x;
+  h8 a, b;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    a[e] = (_Float16)(float)(l * 8 + e + 1);
+    b[e] = (_Float16)((l == q / 8 && e == q % 8) ? 1.0f : 0.0f);
+  }
+  f4_t d = {0, 0, 0, 0};
+  d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, d, 0, 0, 0);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) out[(q * 64 + l) * 4 + r] = (double)d[r];
+}
+
 template <typename T>
 hipError_t launch_prepare_t(hipStream_t s, int N, int NB, T *tiles, int *d_info) {
   const int nt = (N + NB - 1) / NB;
@@ -197,6 +233,20 @@ int ttg_chol_plgsy_tiled(void *stream, int N, int NB, double bump, unsigned long
   const long blocks = (waves + 3) / 4;
   hipLaunchKernelGGL(plgsy_kernel, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), N, NB, nt, bump, seed,
                      tiles);
+  TTGC_HIP(hipGetLastError());
+  return TTG_CHOL_OK;
+}
+
+int ttg_chol_dplgsy_tile(void *stream, double bump, int m, int n, double *A, int lda, int gM, int m0, int n0,
+                         unsigned long long seed) {
+  clear_error();
+  if (m < 0 || n < 0 || lda < std::max(1, m) || m0 < 0 || n0 < 0 || gM < m0 + m || (m > 0 && n > 0 && !A)) {
+    set_error("dplgsy_tile: bad argument (m=%d n=%d lda=%d gM=%d m0=%d n0=%d)", m, n, lda, gM, m0, n0);
+    return TTG_CHOL_EINVAL;
+  }
+  if (m == 0 || n == 0) return TTG_CHOL_OK;
+  hipLaunchKernelGGL(plgsy_tile_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, as_stream(stream), bump, m, n, A,
+                     (long)lda, (long)gM, (long)m0, (long)n0, seed);
   TTGC_HIP(hipGetLastError());
   return TTG_CHOL_OK;
 }
@@ -244,14 +294,16 @@ int ttg_chol_spd_tiled(void *stream, int N, int NB, double alpha, unsigned long 
 
 int ttg_chol_selftest_mfma_layout(void *stream, int kind, double *out) {
   clear_error();
-  if (!out || (kind != 0 && kind != 1)) {
+  if (!out || kind < 0 || kind > 2) {
     set_error("selftest_mfma_layout: bad argument (kind=%d)", kind);
     return TTG_CHOL_EINVAL;
   }
   if (kind == 0)
     hipLaunchKernelGGL(mfma_probe_kernel<double>, dim3(64), dim3(64), 0, as_stream(stream), out);
-  else
+  else if (kind == 1)
     hipLaunchKernelGGL(mfma_probe_kernel<float>, dim3(64), dim3(64), 0, as_stream(stream), out);
+  else
+    hipLaunchKernelGGL(mfma_probe_f16_kernel, dim3(512), dim3(64), 0, as_stream(stream), out);
   TTGC_HIP(hipGetLastError());
   return TTG_CHOL_OK;
 }

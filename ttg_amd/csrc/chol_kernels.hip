@@ -19,11 +19,19 @@ namespace ttgc {
 
 constexpr int kPB = 32;                       // POTRF/TRSM inner block
 constexpr int kMaxTile = 512;                 // largest single-tile POTRF/TRSM
-constexpr int kPanelLd = kMaxTile - kPB + 4;  // LDS panel row stride
 
 // ---------------------------------------------------------------------------
-// POTRF
+// POTRF (one workgroup per tile, FP64 arithmetic, left-looking by 32-column
+// blocks).  For block column j, the rows c0..n-1 of that column are held in
+// MFMA accumulators spread over the 4 waves (16-row blocks); they are updated
+// by every earlier block column in one streamed GEMM (operands straight from
+// L2), then the 32 x 32 diagonal block is factored and inverted in LDS and the
+// rows below are multiplied by the inverse.  Nothing but the 32 x 32 blocks
+// lives in LDS, so the kernel co-resides with the trailing-update kernels.
+// Float tiles are factored in an FP64 copy (op.pw, n x n) and rounded once.
 // ---------------------------------------------------------------------------
+constexpr int kRowBlocks = kMaxTile / 16 / 4;  // 16-row blocks per wave (<= 8)
+
 template <typename TS>
 __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOpT<TS> *__restrict__ ops, PotrfOpT<TS> one,
                                                     const int *__restrict__ d_abort) {
@@ -31,30 +39,79 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOpT<TS> *__restri
   const PotrfOpT<TS> op = ops ? ops[blockIdx.x] : one;
   __shared__ double Ds[kPB][kPB + 1];
   __shared__ double Di[kPB][kPB + 1];
-  __shared__ __align__(16) double Ps[kPB][kPanelLd];
+  __shared__ double Xs[4][16][kPB + 1];  // per-wave staging of one 16 x 32 block
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
   const int n = op.n;
-  const long lda = op.lda;
-  TS *__restrict__ A = op.A;
+  // working FP64 matrix
+  double *__restrict__ W;
+  long ldw;
+  if constexpr (sizeof(TS) == 8) {
+    W = reinterpret_cast<double *>(op.A);
+    ldw = op.lda;
+  } else {
+    W = op.pw;
+    ldw = n;
+    for (long e = tid; e < (long)n * n; e += 256) {
+      const int r = (int)(e % n), c = (int)(e / n);
+      if (r >= c) W[r + (long)c * ldw] = (double)op.A[r + (long)c * op.lda];
+    }
+    wg_sync_global();
+  }
   int fail = 0;
   const int nblk = (n + kPB - 1) / kPB;
 
   for (int jb = 0; jb < nblk; ++jb) {
     const int c0 = jb * kPB, bs = min(kPB, n - c0);
-    // 1. diagonal block, lower part, identity padding beyond n
-    for (int e = tid; e < kPB * kPB; e += 256) {
-      const int r = e & 31, c = e >> 5;
-      double v;
-      if (r < bs && c < bs)
-        v = (r >= c) ? (double)A[c0 + r + (long)(c0 + c) * lda] : 0.0;
-      else
-        v = (r == c) ? 1.0 : 0.0;
-      Ds[r][c] = v;
+    const int R = n - c0;           // rows of the block column (diagonal block included)
+    const int nrb = (R + 15) / 16;  // 16-row blocks
+    d4_t acc[kRowBlocks][2];
+    // initialise with the current column block
+#pragma unroll
+    for (int t = 0; t < kRowBlocks; ++t) {
+      const int q = wave + 4 * t;
+#pragma unroll
+      for (int jj = 0; jj < 2; ++jj) {
+        const int col = c0 + jj * 16 + lr;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = c0 + q * 16 + Mfma<double>::drow(lk, r);
+          acc[t][jj][r] = (q < nrb && row < n && col < n && row >= col) ? W[row + (long)col * ldw] : 0.0;
+        }
+      }
+    }
+    // left-looking update with the columns [0, c0):  acc -= L(rows, k) L(c0+cols, k)
+    for (int k0 = 0; k0 < c0; k0 += 4) {
+      const int k = k0 + lk;
+      const double b0 = (c0 + lr < n) ? W[c0 + lr + (long)k * ldw] : 0.0;
+      const double b1 = (c0 + 16 + lr < n) ? W[c0 + 16 + lr + (long)k * ldw] : 0.0;
+#pragma unroll
+      for (int t = 0; t < kRowBlocks; ++t) {
+        const int q = wave + 4 * t;
+        if (q < nrb) {
+          const int row = c0 + q * 16 + lr;
+          const double a = (row < n) ? -W[row + (long)k * ldw] : 0.0;
+          acc[t][0] = Mfma<double>::mma(a, b0, acc[t][0]);
+          acc[t][1] = Mfma<double>::mma(a, b1, acc[t][1]);
+        }
+      }
+    }
+    // diagonal block (row blocks 0 and 1, held by waves 0 and 1) to LDS
+    if (wave < 2) {
+#pragma unroll
+      for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int rl = wave * 16 + Mfma<double>::drow(lk, r), cl = jj * 16 + lr;
+          double v = acc[0][jj][r];
+          if (rl >= bs || cl >= bs) v = (rl == cl) ? 1.0 : 0.0;  // identity padding
+          else if (rl < cl) v = 0.0;
+          Ds[rl][cl] = v;
+        }
     }
     __syncthreads();
-
-    // 2. right-looking Cholesky of the 32 x 32 block (dpotf2 order)
+    // right-looking Cholesky of the 32 x 32 block (dpotf2 order)
     {
       const int i = tid & 31;
       for (int j = 0; j < kPB; ++j) {
@@ -67,15 +124,15 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOpT<TS> *__restri
         const double lij = Ds[i][j] * rinv;
         double upd[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int l = (tid >> 5) + 8 * q;
-          upd[q] = (l > j && i >= l) ? lij * (Ds[l][j] * rinv) : 0.0;
+        for (int qq = 0; qq < 4; ++qq) {
+          const int l = (tid >> 5) + 8 * qq;
+          upd[qq] = (l > j && i >= l) ? lij * (Ds[l][j] * rinv) : 0.0;
         }
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int l = (tid >> 5) + 8 * q;
-          if (l > j && i >= l) Ds[i][l] -= upd[q];
+        for (int qq = 0; qq < 4; ++qq) {
+          const int l = (tid >> 5) + 8 * qq;
+          if (l > j && i >= l) Ds[i][l] -= upd[qq];
         }
         if (tid < kPB) {
           if (tid > j)
@@ -87,87 +144,64 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOpT<TS> *__restri
       }
     }
     if (fail) break;
-
-    // 3. Di = inv(L_D) (one column per thread), L_D back to global
+    // Di = inv(L_D), one column per thread, accumulated in LDS (no register arrays)
     if (tid < kPB) {
       const int c = tid;
-      double x[kPB];
-#pragma unroll
-      for (int i = 0; i < kPB; ++i) {
-        double v;
-        if (i < c) {
-          v = 0.0;
-        } else if (i == c) {
-          v = 1.0 / Ds[c][c];
-        } else {
-          double s = 0.0;
-#pragma unroll
-          for (int k = 0; k < i; ++k)
-            if (k >= c) s += Ds[i][k] * x[k];
-          v = -s / Ds[i][i];
-        }
-        x[i] = v;
-        Di[i][c] = v;
+      for (int i = 0; i < c; ++i) Di[i][c] = 0.0;
+      Di[c][c] = 1.0 / Ds[c][c];
+      for (int i = c + 1; i < kPB; ++i) {
+        double s = 0.0;
+        for (int k = c; k < i; ++k) s += Ds[i][k] * Di[k][c];
+        Di[i][c] = -s / Ds[i][i];
       }
     }
+    __syncthreads();
+    // L_D back to the working matrix; inverse blocks out for the TRSM
     for (int e = tid; e < kPB * kPB; e += 256) {
       const int r = e & 31, c = e >> 5;
-      if (r >= c && r < bs && c < bs) A[c0 + r + (long)(c0 + c) * lda] = (TS)Ds[r][c];
+      if (r >= c && r < bs && c < bs) W[c0 + r + (long)(c0 + c) * ldw] = Ds[r][c];
+      if (op.dinv) op.dinv[jb * kPB * kPB + e] = (TS)Di[r][c];
     }
-    __syncthreads();
-    if (op.dinv)
-      for (int e = tid; e < kPB * kPB; e += 256) op.dinv[jb * kPB * kPB + e] = (TS)Di[e & 31][e >> 5];
-
-    // 4. panel rows below the block: P = A(:, c0:c0+32) * Di^T
-    const int R = n - c0 - kPB;
-    for (int rr = tid; rr < R; rr += 256) {
-      const long i = c0 + kPB + rr;
-      double a[kPB];
+    // rows below the diagonal block: X = acc * Di^T (staged through LDS into
+    // MFMA A-operand layout, one 16 x 32 block at a time per wave)
 #pragma unroll
-      for (int k = 0; k < kPB; ++k) a[k] = (double)A[i + (long)(c0 + k) * lda];
+    for (int t = 0; t < kRowBlocks; ++t) {
+      const int q = wave + 4 * t;
+      if (q >= 2 && q < nrb) {
 #pragma unroll
-      for (int c = 0; c < kPB; ++c) {
-        double s = 0.0;
+        for (int jj = 0; jj < 2; ++jj)
 #pragma unroll
-        for (int k = 0; k <= c; ++k) s += a[k] * Di[c][k];
-        A[i + (long)(c0 + c) * lda] = (TS)s;
-        Ps[c][rr] = s;
-      }
-    }
-    __syncthreads();
-
-    // 5. trailing lower update  A22 -= P P^T  on 16 x 16 blocks
-    if (R > 0) {
-      const int nb = (R + 15) / 16, nblocks = nb * (nb + 1) / 2;
-      const int lr = lane & 15, lk = lane >> 4;
-      for (int q = wave; q < nblocks; q += 4) {
-        int bi = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
-        while ((bi + 1) * (bi + 2) / 2 <= q) ++bi;
-        while (bi * (bi + 1) / 2 > q) --bi;
-        const int bj = q - bi * (bi + 1) / 2;
-        const int col = c0 + kPB + bj * 16 + lr;
-        const int rowb = c0 + kPB + bi * 16;
-        double old[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = rowb + Mfma<double>::drow(lk, r);
-          old[r] = (row < n && col < n) ? (double)A[row + (long)col * lda] : 0.0;
-        }
-        d4_t acc = {0.0, 0.0, 0.0, 0.0};
+          for (int r = 0; r < 4; ++r) Xs[wave][Mfma<double>::drow(lk, r)][jj * 16 + lr] = acc[t][jj][r];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        d4_t x0 = {0, 0, 0, 0}, x1 = {0, 0, 0, 0};
 #pragma unroll
         for (int ks = 0; ks < kPB / 4; ++ks) {
-          const double av = Ps[ks * 4 + lk][bi * 16 + lr];
-          const double bv = Ps[ks * 4 + lk][bj * 16 + lr];
-          acc = Mfma<double>::mma(av, bv, acc);
+          const double av = Xs[wave][lr][ks * 4 + lk];
+          x0 = Mfma<double>::mma(av, Di[lr][ks * 4 + lk], x0);
+          x1 = Mfma<double>::mma(av, Di[16 + lr][ks * 4 + lk], x1);
         }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int row = rowb + Mfma<double>::drow(lk, r);
-          if (row < n && col < n && (bi != bj || row >= col)) A[row + (long)col * lda] = (TS)(old[r] - acc[r]);
+          const int row = c0 + q * 16 + Mfma<double>::drow(lk, r);
+          if (row < n) {
+            if (lr < bs) W[row + (long)(c0 + lr) * ldw] = x0[r];
+            if (16 + lr < bs) W[row + (long)(c0 + 16 + lr) * ldw] = x1[r];
+          }
         }
       }
     }
     wg_sync_global();
+  }
+  if constexpr (sizeof(TS) != 8) {
+    // round the FP64 factor once into the float tile (columns done so far)
+    const int done = fail ? (fail - 1) / kPB * kPB : n;
+    for (long e = tid; e < (long)n * n; e += 256) {
+      const int r = (int)(e % n), c = (int)(e / n);
+      if (r >= c && c < done) op.A[r + (long)c * op.lda] = (TS)W[r + (long)c * ldw];
+    }
   }
   if (tid == 0 && op.info) {
     if (fail)
@@ -224,7 +258,7 @@ __global__ __launch_bounds__(256) void trsm_kernel(const TrsmOp<T> *__restrict__
   const TrsmOp<T> op = ops ? ops[o] : one;
   const int r0 = s * 64;
   if (r0 >= op.m) return;
-  using Core = GemmCore<T, 64, 32, 32, 4, 1, !TRANS>;
+  using Core = GemmCore<T, 64, 32, 16, 4, 1, !TRANS>;
   __shared__ __align__(16) typename Core::Smem sm;
   __shared__ T Ts[kPB][64 + 4];
   __shared__ T Dis[kPB][kPB + 1];

@@ -17,6 +17,7 @@ struct PotrfOpT {
   int n, lda;
   int info_offset;
   int write_zero;  // 1: store 0 into *info on success (per-tile API semantics)
+  double *pw;      // FP64 workspace (n x n) for float tiles; unused for double tiles
 };
 using PotrfOp = PotrfOpT<double>;
 

@@ -19,6 +19,7 @@
 #include <cstring>
 #include <vector>
 
+#include "chol_f16.hpp"
 #include "chol_internal.hpp"
 #include "chol_kernels.hpp"
 #include "chol_profile.hpp"
@@ -34,7 +35,11 @@ struct TiledPlanT {
   int N = 0, NB = 0, NT = 0;
   T *tiles = nullptr;
   unsigned flags = 0;
-  DevBuf d_ops, d_dinv, d_info;
+  bool f16x2 = false;  // float tiles only: split-FP16 trailing updates
+  DevBuf d_ops, d_dinv, d_info, d_pw;
+  DevBuf d_panel, d_scale, d_f16ops;  // split-FP16 panels (two slots), scales, descriptors
+  std::vector<int> split_off, split_cnt;
+  size_t o_split = 0, o_f16 = 0;
   size_t o_potrf = 0, o_trsm = 0, o_upd = 0;
   std::vector<int> trsm_off, trsm_cnt, updA_off, updA_cnt, updB_off, updB_cnt;
   hipStream_t s_main = nullptr, s_look = nullptr;
@@ -51,7 +56,7 @@ struct TiledPlanT {
       if (e) (void)hipEventDestroy(e);
     if (s_main) (void)hipStreamDestroy(s_main);
     if (s_look) (void)hipStreamDestroy(s_look);
-    for (DevBuf *b : {&d_ops, &d_dinv, &d_info})
+    for (DevBuf *b : {&d_ops, &d_dinv, &d_info, &d_pw, &d_panel, &d_scale, &d_f16ops})
       if (b->ptr) (void)hipFree(b->ptr);
   }
 
@@ -65,13 +70,15 @@ struct TiledPlanT {
     const int dstride = ((NB + 31) / 32) * 32 * 32;
     TTGC_HIP(d_dinv.reserve(sizeof(T) * dstride));
     TTGC_HIP(d_info.reserve(sizeof(int)));
+    TTGC_HIP(d_pw.reserve(sizeof(double) * (size_t)NB * NB));
     T *dinv = static_cast<T *>(d_dinv.ptr);
+    double *pw = sizeof(T) == 8 ? nullptr : static_cast<double *>(d_pw.ptr);
     std::vector<PotrfOpT<T>> potrf;
     std::vector<TrsmOp<T>> trsm;
     std::vector<UpdOp<T>> upd;
     // every tile is a full NB x NB block once the edge padding is set
     for (int k = 0; k < NT; ++k) {
-      potrf.push_back(PotrfOpT<T>{tile(k, k), dinv, nullptr, NB, NB, k * NB, 0});
+      potrf.push_back(PotrfOpT<T>{tile(k, k), dinv, nullptr, NB, NB, k * NB, 0, pw});
       trsm_off.push_back((int)trsm.size());
       for (int m = k + 1; m < NT; ++m) trsm.push_back(TrsmOp<T>{tile(k, k), tile(m, k), dinv, NB, NB, NB, NB});
       trsm_cnt.push_back((int)trsm.size() - trsm_off.back());
@@ -97,10 +104,63 @@ struct TiledPlanT {
     if (!trsm.empty()) memcpy(host.data() + o_trsm, trsm.data(), trsm.size() * sizeof(TrsmOp<T>));
     if (!upd.empty()) memcpy(host.data() + o_upd, upd.data(), upd.size() * sizeof(UpdOp<T>));
     TTGC_HIP(hipMemcpy(d_ops.ptr, host.data(), total, hipMemcpyHostToDevice));
+    if (f16x2) {
+      int rc = build_f16(upd);
+      if (rc) return rc;
+    }
     TTGC_HIP(hipStreamCreateWithFlags(&s_main, hipStreamNonBlocking));
     TTGC_HIP(hipStreamCreateWithFlags(&s_look, hipStreamNonBlocking));
     for (hipEvent_t *e : {&ev_in, &ev_out, &ev_panel, &ev_col})
       TTGC_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    return TTG_CHOL_OK;
+  }
+
+  // Split-FP16 descriptors: panel k lives in slot k % 2 (tile m at position
+  // m - k - 1); the update descriptors mirror the FP32 ones one to one.
+  SplitOp *split_ops() { return reinterpret_cast<SplitOp *>(static_cast<char *>(d_f16ops.ptr) + o_split); }
+  F16UpdOp *f16_ops() { return reinterpret_cast<F16UpdOp *>(static_cast<char *>(d_f16ops.ptr) + o_f16); }
+
+  int build_f16(const std::vector<UpdOp<T>> &upd) {
+    const size_t per = (size_t)NB * NB;  // halves per part
+    const size_t slot_tiles = NT > 1 ? NT - 1 : 1;
+    TTGC_HIP(d_panel.reserve(2 * slot_tiles * 2 * per * sizeof(uint16_t)));
+    TTGC_HIP(d_scale.reserve(2 * slot_tiles * sizeof(float)));
+    uint16_t *panel = static_cast<uint16_t *>(d_panel.ptr);
+    float *scale = static_cast<float *>(d_scale.ptr);
+    auto hi = [&](int k, int m) { return panel + ((size_t)(k % 2) * slot_tiles + (m - k - 1)) * 2 * per; };
+    auto sc = [&](int k, int m) { return scale + (size_t)(k % 2) * slot_tiles + (m - k - 1); };
+    std::vector<SplitOp> sp;
+    std::vector<F16UpdOp> fu;
+    for (int k = 0; k < NT; ++k) {
+      split_off.push_back((int)sp.size());
+      for (int m = k + 1; m < NT; ++m)
+        sp.push_back(SplitOp{reinterpret_cast<const float *>(tile(m, k)), hi(k, m), hi(k, m) + per, sc(k, m)});
+      split_cnt.push_back((int)sp.size() - split_off.back());
+    }
+    // same order as the FP32 update descriptors (updA(k) then updB(k))
+    for (int k = 0; k < NT; ++k) {
+      auto add = [&](int m, int n) {
+        fu.push_back(F16UpdOp{hi(k, m), hi(k, m) + per, hi(k, n), hi(k, n) + per, sc(k, m), sc(k, n),
+                              reinterpret_cast<float *>(tile(m, n)), NB, NB, NB, NB, m == n ? 1 : 0});
+      };
+      if (k + 1 < NT)
+        for (int m = k + 1; m < NT; ++m) add(m, k + 1);
+      for (int n = k + 2; n < NT; ++n)
+        for (int m = n; m < NT; ++m) add(m, n);
+    }
+    if (fu.size() != upd.size()) {
+      set_error("internal: split-FP16 descriptor count mismatch");
+      return TTG_CHOL_ESTATE;
+    }
+    auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+    o_split = 0;
+    o_f16 = align(sp.size() * sizeof(SplitOp));
+    const size_t total = align(o_f16 + fu.size() * sizeof(F16UpdOp));
+    TTGC_HIP(d_f16ops.reserve(total));
+    std::vector<char> host(total);
+    if (!sp.empty()) memcpy(host.data() + o_split, sp.data(), sp.size() * sizeof(SplitOp));
+    if (!fu.empty()) memcpy(host.data() + o_f16, fu.data(), fu.size() * sizeof(F16UpdOp));
+    TTGC_HIP(hipMemcpy(d_f16ops.ptr, host.data(), total, hipMemcpyHostToDevice));
     return TTG_CHOL_OK;
   }
 
@@ -130,7 +190,10 @@ struct TiledPlanT {
     if (la) TTGC_HIP(hipEventRecord(ev_col, s_main));
     auto upd = [&](int off, int cnt, int nsyrk) -> int {
       const int slot = prof ? prof_begin(s_main) : -1;
-      TTGC_HIP(launch_update<T>(s_main, upd_ops() + off, cnt, unone, NB, NB, m1, p1, fast, info));
+      if (f16x2)
+        TTGC_HIP(launch_update_f16x2(s_main, f16_ops() + off, cnt, NB, info));
+      else
+        TTGC_HIP(launch_update<T>(s_main, upd_ops() + off, cnt, unone, NB, NB, m1, p1, fast, info));
       if (prof) prof_end(slot, s_main, PROF_UPDATE, (cnt - nsyrk) * 2.0 * b3 + nsyrk * (double)NB * NB * (NB + 1));
       return TTG_CHOL_OK;
     };
@@ -143,6 +206,7 @@ struct TiledPlanT {
         slot = prof ? prof_begin(sp) : -1;
         TTGC_HIP(launch_trsm<T>(sp, trsm_ops() + trsm_off[k], trsm_cnt[k], tnone, NB, info));
         if (prof) prof_end(slot, sp, PROF_TRSM, trsm_cnt[k] * b3);
+        if (f16x2) TTGC_HIP(launch_split(sp, split_ops() + split_off[k], split_cnt[k], NB));
       }
       if (la) {
         TTGC_HIP(hipEventRecord(ev_panel, s_look));

@@ -1,0 +1,56 @@
+"""ctypes binding of libttg_graph.so (include/ttg_graph.h): the examples/potrf
+task graph on the single-GPU TTG runtime.  Host numpy tiles in, host tiles
+out; there is no CPU path."""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libttg_graph.so")
+DRIVER = os.path.join(HERE, "bin", "testing_dpotrf")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "ttg_graph.h")
+
+SIGNATURES = {
+    "ttg_graph_dpotrf": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "ttg_graph_dplgsy": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_ulong,
+                                        ctypes.c_void_p]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} is not built: run python -m ttg_amd.build")
+        L = ctypes.CDLL(LIB)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _tiles(t):
+    if not (isinstance(t, np.ndarray) and t.dtype == np.float64 and t.flags.c_contiguous):
+        raise TypeError("tiles must be a contiguous float64 numpy array")
+    return t.ctypes.data
+
+
+def dpotrf(N, NB, tiles):
+    """Factor host tiles in place with the task graph; returns the runtime's
+    statistics (tasks, rounds, kernel launches, largest round)."""
+    stats = (ctypes.c_long * 4)()
+    rc = lib().ttg_graph_dpotrf(N, NB, _tiles(tiles), ctypes.addressof(stats))
+    if rc != 0:
+        raise ValueError(f"ttg_graph_dpotrf(N={N}, NB={NB}) rejected its arguments")
+    return {"tasks": stats[0], "rounds": stats[1], "launches": stats[2], "max_round": stats[3]}
+
+
+def dplgsy(N, NB, bump, seed, tiles):
+    rc = lib().ttg_graph_dplgsy(N, NB, float(bump), seed, _tiles(tiles))
+    if rc != 0:
+        raise ValueError(f"ttg_graph_dplgsy(N={N}, NB={NB}) rejected its arguments")
