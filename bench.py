@@ -57,7 +57,38 @@ def parse():
     p.add_argument("--cpu-n", type=int, default=16384, help="order of the bounded CPU-baseline sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-profile", action="store_true")
+    p.add_argument("--no-fp64", action="store_true", help="skip the plain-FP64 comparison solve")
+    p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                   help="only run a plain device copy of MIB MiB three times (calibrates the HBM byte counters) "
+                        "and exit")
     return p.parse_args()
+
+
+def copy_probe(mib):
+    """A plain copy kernel of known size: what FETCH_SIZE / WRITE_SIZE read for
+    it gives the factor that turns those counters into HBM bytes."""
+    import torch
+
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 3}), flush=True)
+
+
+def read_traffic(mode, N, NB, nrhs):
+    """HBM bytes per launch of the dominant kernel from the committed PMC
+    summary for this workload (profiles/pmc_<kernel>_N<N>.txt), or None."""
+    kern = {"f16x2": "update_f16x2", "fp32": "update_fp32", "fp64": "update_fp64"}[mode]
+    path = os.path.join(ROOT, "profiles", f"pmc_{kern}_N{N}_NB{NB}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def cpu_baseline(n, nb, nrhs):
@@ -90,6 +121,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the kernels and drivers here are "
               f"single-GPU", file=sys.stderr)
         sys.exit(2)
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     import torch
 
     import ttg_amd.chol as C
@@ -164,7 +198,7 @@ def main():
 
     ms_step, wall, berr = timed(a.mode, a.steps, a.warmup)
     it_main = iters[a.mode][0]
-    other = "fp64" if a.mode != "fp64" else None
+    other = "fp64" if a.mode != "fp64" and not a.no_fp64 else None
     ms_fp64 = berr_fp64 = None
     if other:
         ms_fp64, _, berr_fp64 = timed(other, 1, 1)
@@ -190,7 +224,7 @@ def main():
         peak = {"f16x2": FP16_PEAK / 3.0, "fp32": FP32_PEAK, "fp64": FP64_PEAK}[a.mode]
         ach = fl_u / (ms_u * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": round(ach, 3), "peak": round(peak / 1e12, 1), "unit": "TFLOP/s",
-                "frac": round(ach / (peak / 1e12), 4), "traffic": None,
+                "frac": round(ach / (peak / 1e12), 4), "traffic": read_traffic(a.mode, N, NB, nrhs),
                 "kernel": {"f16x2": "update_f16x2_kernel (batched SYRK+GEMM, 3 FP16 MFMA products per output; "
                                     "peak = FP16 dense / 3, flops counted once)",
                            "fp32": "update_kernel<float> (batched SYRK+GEMM trailing update)",
