@@ -1,13 +1,14 @@
 // Split-FP16 trailing update for the mixed-precision factorization.
 //
-// After TRSM(k), every panel tile A_mk (FP32) is split with a power-of-two
-// per-tile scale s_m (max |A_mk| s_m in [2^13, 2^14)) into two FP16 parts
+// After TRSM(k), every row i of every panel tile A_mk (FP32) is scaled by a
+// power of two s_i (max_j |A_mk(i,j)| s_i in [2^12, 2^13)) and split into two
+// FP16 parts
 //   hi = fp16(A s),  lo = fp16(A s - hi),
 // stored row-major (k contiguous).  The update of tile (m, n) is then
-//   C -= (hi_m hi_n^T + hi_m lo_n^T + lo_m hi_n^T) / (s_m s_n)
+//   C -= diag(1/s_m) (hi_m hi_n^T + hi_m lo_n^T + lo_m hi_n^T) diag(1/s_n)
 // with three FP16 MFMAs (v_mfma_f32_16x16x32_f16) accumulating in FP32.  The
 // dropped lo lo^T term and the FP16 rounding of lo bound the representation
-// error of each operand entry by ~2^-22 of its tile's largest entry, close to
+// error of each operand entry by ~2^-22 of its row's largest entry, close to
 // FP32 (2^-24), at a third of the FP16 MFMA rate instead of the FP32 one.
 #include <hip/hip_runtime.h>
 
@@ -20,48 +21,59 @@ namespace ttgc {
 typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
 
 // ---------------------------------------------------------------------------
-// split: one workgroup per panel tile
+// split: one workgroup per 64-row stripe of a panel tile
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void split_kernel(const SplitOp *__restrict__ ops, int NB) {
-  const SplitOp op = ops[blockIdx.x];
-  __shared__ float red[256];
+  __builtin_amdgcn_s_setprio(2);  // panel work: ahead of co-resident update waves
+  const int nst = NB / 64;
+  const SplitOp op = ops[blockIdx.x / nst];
+  const int r0 = (blockIdx.x % nst) * 64;
+  __shared__ float red[16][64];
+  __shared__ float rsc[64];
   __shared__ float blk[64][65];
   const int tid = threadIdx.x;
-  // 1. max |a| over the tile
-  float mx = 0.f;
-  const long n2 = (long)NB * NB;
-  for (long e = tid; e < n2; e += 256) mx = fmaxf(mx, fabsf(op.src[e]));
-  red[tid] = mx;
+  // 1. max |a| of each row of the stripe: thread (v, c) covers rows 4v..4v+3
+  //    of the columns c, c + 16, ...
+  {
+    const int v = tid & 15, c = tid >> 4;
+    float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f;
+    for (int col = c; col < NB; col += 16) {
+      const float4 x = *reinterpret_cast<const float4 *>(op.src + r0 + 4 * v + (long)col * NB);
+      m0 = fmaxf(m0, fabsf(x.x));
+      m1 = fmaxf(m1, fabsf(x.y));
+      m2 = fmaxf(m2, fabsf(x.z));
+      m3 = fmaxf(m3, fabsf(x.w));
+    }
+    red[c][4 * v + 0] = m0;
+    red[c][4 * v + 1] = m1;
+    red[c][4 * v + 2] = m2;
+    red[c][4 * v + 3] = m3;
+  }
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (tid < w) red[tid] = fmaxf(red[tid], red[tid + w]);
-    __syncthreads();
+  if (tid < 64) {
+    float mx = 0.f;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) mx = fmaxf(mx, red[c][tid]);
+    // s = 2^(13 - e) with 2^(e-1) <= mx < 2^e, so max|a| s lies in [2^12, 2^13)
+    int e = 13;
+    if (mx > 0.f && isfinite(mx)) frexpf(mx, &e);
+    rsc[tid] = ldexpf(1.f, 13 - e);
+    op.rinv[r0 + tid] = ldexpf(1.f, e - 13);
   }
-  mx = red[0];
-  // s = 2^(13 - e) with 2^(e-1) <= mx < 2^e, so max|a| s lies in [2^12, 2^13)
-  float s = 1.f;
-  if (mx > 0.f && isfinite(mx)) {
-    int e;
-    frexpf(mx, &e);
-    s = ldexpf(1.f, 13 - e);
-  }
-  if (tid == 0) *op.scale = s;
-  // 2. transpose through LDS in 64 x 64 blocks and write hi / lo row-major
-  const int nb64 = NB / 64;
-  for (int b = 0; b < nb64 * nb64; ++b) {
-    const int br = b % nb64, bc = b / nb64;  // block row (i), block col (k)
+  // 2. 64 x 64 blocks: transpose through LDS, split, write row-major
+  for (int bc = 0; bc < NB / 64; ++bc) {
     __syncthreads();
     for (int e = tid; e < 64 * 64; e += 256) {
       const int r = e & 63, c = e >> 6;  // coalesced column-major read
-      blk[c][r] = op.src[(br * 64 + r) + (long)(bc * 64 + c) * NB];
+      blk[c][r] = op.src[(r0 + r) + (long)(bc * 64 + c) * NB];
     }
     __syncthreads();
     for (int e = tid; e < 64 * 64; e += 256) {
       const int c = e & 63, r = e >> 6;  // row-major write, k (= c) contiguous
-      const float v = blk[c][r] * s;
+      const float v = blk[c][r] * rsc[r];
       const _Float16 h = (_Float16)v;
       const _Float16 l = (_Float16)(v - (float)h);
-      const long o = (long)(br * 64 + r) * NB + bc * 64 + c;
+      const long o = (long)(r0 + r) * NB + bc * 64 + c;
       reinterpret_cast<_Float16 *>(op.hi)[o] = h;
       reinterpret_cast<_Float16 *>(op.lo)[o] = l;
     }
@@ -166,25 +178,30 @@ __global__ __launch_bounds__(256) void update_f16x2_kernel(const F16UpdOp *__res
     if (more) f16_store(sm, cur ^ 1, tid, ra, rb);
     __syncthreads();
   }
-  const float inv = 1.f / (*op.sa * *op.sb);  // exact: powers of two
   const bool diag = op.lower && bi == bj;
+  float rrow[4][4];  // row factors (powers of two: the products below are exact)
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int col = j0 + wn * 64 + j * 16 + lr;
-      float *pc = op.C + (long)col * op.ldc;
+    for (int r = 0; r < 4; ++r) rrow[i][r] = op.ra[i0 + wm * 64 + i * 16 + Mfma<float>::drow(lk, r)];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int col = j0 + wn * 64 + j * 16 + lr;
+    const float rc = op.rb[col];
+    float *pc = op.C + (long)col * op.ldc;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = i0 + wm * 64 + i * 16 + Mfma<float>::drow(lk, r);
-        if (!diag || row >= col) pc[row] -= acc[i][j][r] * inv;
+        if (!diag || row >= col) pc[row] -= acc[i][j][r] * (rrow[i][r] * rc);
       }
-    }
+  }
 }
 
 hipError_t launch_split(hipStream_t s, const SplitOp *ops, int nops, int NB) {
   if (nops <= 0) return hipSuccess;
-  hipLaunchKernelGGL(split_kernel, dim3(nops), dim3(256), 0, s, ops, NB);
+  hipLaunchKernelGGL(split_kernel, dim3(nops * (NB / 64)), dim3(256), 0, s, ops, NB);
   return hipGetLastError();
 }
 

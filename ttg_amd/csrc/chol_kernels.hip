@@ -32,6 +32,81 @@ constexpr int kMaxTile = 512;                 // largest single-tile POTRF/TRSM
 // ---------------------------------------------------------------------------
 constexpr int kRowBlocks = kMaxTile / 16 / 4;  // 16-row blocks per wave (<= 8)
 
+// Lane l's value of v, for a wave-uniform l (two 32-bit readlanes).
+__device__ __forceinline__ double rdlane(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Cholesky factor and inverse of the 32 x 32 block in Ds (lower part read)
+// by ONE wave, without barriers.  Lane i (mod 32) holds row i of the block in
+// registers and factors it left-looking, column by column
+//   L(i,j) = (A(i,j) - sum_{k<j} L(i,k) L(j,k)) / L(j,j),
+// storing each finished column to Ds so that row j's entries are read back
+// as LDS broadcasts; the pivot comes from lane j by readlane.  Then lane c
+// computes column c of inv(L) by forward substitution
+//   X(r,c) = -(sum_{c<=k<r} L(r,k) X(k,c)) / L(r,r),  X(c,c) = 1 / L(c,c),
+// and stores it to Di.  Returns the 1-based column of the first pivot that
+// is not positive (NaN included), 0 if none; on failure Ds and Di are left
+// unspecified.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int diag_chol_inv_wave(double (*Ds)[kPB + 1], double (*Di)[kPB + 1], int lane) {
+  // an opaque copy of the lane index: keeps the compiler from hoisting the
+  // 100+ lane masks below out of the caller's loop (and spilling them)
+  asm volatile("" : "+v"(lane));
+  const int i = lane & 31;
+  const bool writer = lane < kPB;
+  int fail = 0;
+  double a[kPB];
+#pragma unroll
+  for (int k = 0; k < kPB; ++k) a[k] = (k <= i) ? Ds[i][k] : 0.0;
+#pragma unroll
+  for (int j = 0; j < kPB; ++j) {
+    double s0 = a[j], s1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < j; k += 2) {
+      s0 -= a[k] * Ds[j][k];
+      if (k + 1 < j) s1 -= a[k + 1] * Ds[j][k + 1];
+    }
+    const double s = s0 + s1;
+    const double djj = rdlane(s, j);
+    fail = (fail == 0 && !(djj > 0.0)) ? j + 1 : fail;  // keep going: the caller discards the block
+    const double d = sqrt(djj);
+    if (i == j)
+      a[j] = d;
+    else if (i > j)
+      a[j] = s / d;
+    if (writer) Ds[i][j] = a[j];
+    wave_lds_sync();
+  }
+  if (fail) return fail;
+  const int c = lane & 31;
+  double x[kPB];
+#pragma unroll
+  for (int r = 0; r < kPB; ++r) {
+    double t0 = 0.0, t1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < r; k += 2) {
+      t0 += Ds[r][k] * x[k];
+      if (k + 1 < r) t1 += Ds[r][k + 1] * x[k + 1];
+    }
+    const double lrr = Ds[r][r];
+    x[r] = (r < c) ? 0.0 : (r == c ? 1.0 / lrr : -(t0 + t1) / lrr);
+  }
+  if (lane < kPB) {
+#pragma unroll
+    for (int r = 0; r < kPB; ++r) Di[r][c] = x[r];
+  }
+  return 0;
+}
+
 template <typename TS>
 __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOpT<TS> *__restrict__ ops, PotrfOpT<TS> one,
                                                     const int *__restrict__ d_abort) {
@@ -40,16 +115,21 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOpT<TS> *__restri
   __shared__ double Ds[kPB][kPB + 1];
   __shared__ double Di[kPB][kPB + 1];
   __shared__ double Xs[4][16][kPB + 1];  // per-wave staging of one 16 x 32 block
+  __shared__ int fail_s;
+  // the diagonal tiles are the critical path of the factorization: issue
+  // ahead of co-resident trailing-update waves
+  __builtin_amdgcn_s_setprio(3);
 
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: lets the row-block tests branch on scalars
   const int lr = lane & 15, lk = lane >> 4;
-  const int n = op.n;
+  const int n = __builtin_amdgcn_readfirstlane(op.n);
   // working FP64 matrix
   double *__restrict__ W;
   long ldw;
   if constexpr (sizeof(TS) == 8) {
     W = reinterpret_cast<double *>(op.A);
-    ldw = op.lda;
+    ldw = __builtin_amdgcn_readfirstlane(op.lda);
   } else {
     W = op.pw;
     ldw = n;
@@ -67,35 +147,91 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOpT<TS> *__restri
     const int R = n - c0;           // rows of the block column (diagonal block included)
     const int nrb = (R + 15) / 16;  // 16-row blocks
     d4_t acc[kRowBlocks][2];
-    // initialise with the current column block
+    typedef __attribute__((address_space(1))) const double gdouble;
+    const gdouble *Wg = (const gdouble *)W;  // tiles and workspace are global memory
+    const int tact = (nrb - wave + 3) >> 2;  // active row blocks of this wave (uniform)
+    // initialise with MINUS the current column block (the GEMM below adds
+    // L(rows, k) L(c0+cols, k) and the sign is restored after it); loads from
+    // clamped addresses, masked afterwards, so that all of them are in flight
 #pragma unroll
     for (int t = 0; t < kRowBlocks; ++t) {
       const int q = wave + 4 * t;
+      if (t < tact) {
 #pragma unroll
-      for (int jj = 0; jj < 2; ++jj) {
-        const int col = c0 + jj * 16 + lr;
+        for (int jj = 0; jj < 2; ++jj) {
+          const int col = c0 + jj * 16 + lr;
+          const long oc = (long)min(col, n - 1) * ldw;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = c0 + q * 16 + Mfma<double>::drow(lk, r);
-          acc[t][jj][r] = (q < nrb && row < n && col < n && row >= col) ? W[row + (long)col * ldw] : 0.0;
+          for (int r = 0; r < 4; ++r) acc[t][jj][r] = Wg[min(c0 + q * 16 + Mfma<double>::drow(lk, r), n - 1) + oc];
         }
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj) {
+          const int col = c0 + jj * 16 + lr;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = c0 + q * 16 + Mfma<double>::drow(lk, r);
+            acc[t][jj][r] = (row < n && col < n && row >= col) ? -acc[t][jj][r] : 0.0;
+          }
+        }
+      } else {
+        acc[t][0] = d4_t{0.0, 0.0, 0.0, 0.0};
+        acc[t][1] = d4_t{0.0, 0.0, 0.0, 0.0};
       }
     }
-    // left-looking update with the columns [0, c0):  acc -= L(rows, k) L(c0+cols, k)
-    for (int k0 = 0; k0 < c0; k0 += 4) {
-      const int k = k0 + lk;
-      const double b0 = (c0 + lr < n) ? W[c0 + lr + (long)k * ldw] : 0.0;
-      const double b1 = (c0 + 16 + lr < n) ? W[c0 + 16 + lr + (long)k * ldw] : 0.0;
+    // left-looking update with the columns [0, c0), streamed from L2 with the
+    // operands of the next 4 columns in flight while 4 are multiplied.  Rows
+    // and columns past the tile edge read row / column n-1 instead: they only
+    // feed outputs past the edge, which are never stored (the diagonal block
+    // pads them with the identity, whose inverse is zero off its block).
+    if (c0 > 0) {
+      const int ob0 = min(c0 + lr, n - 1), ob1 = min(c0 + 16 + lr, n - 1);
+      // straight-line code per count of active row blocks: no branch inside
+      // the loop, so the loads of one step stay in flight across the MFMAs of
+      // the other
+      auto kloop = [&]<int TA>() {
+        int oa[TA];
 #pragma unroll
-      for (int t = 0; t < kRowBlocks; ++t) {
-        const int q = wave + 4 * t;
-        if (q < nrb) {
-          const int row = c0 + q * 16 + lr;
-          const double a = (row < n) ? -W[row + (long)k * ldw] : 0.0;
-          acc[t][0] = Mfma<double>::mma(a, b0, acc[t][0]);
-          acc[t][1] = Mfma<double>::mma(a, b1, acc[t][1]);
+        for (int t = 0; t < TA; ++t) oa[t] = min(c0 + (wave + 4 * t) * 16 + lr, n - 1);
+        double bv[2][2], av[2][TA];
+        auto load = [&](int sl, int k) {
+          const gdouble *Wk = Wg + (long)k * ldw;
+          bv[sl][0] = Wk[ob0];
+          bv[sl][1] = Wk[ob1];
+#pragma unroll
+          for (int t = 0; t < TA; ++t) av[sl][t] = Wk[oa[t]];
+        };
+        auto mma = [&](int sl) {
+#pragma unroll
+          for (int t = 0; t < TA; ++t) {
+            acc[t][0] = Mfma<double>::mma(av[sl][t], bv[sl][0], acc[t][0]);
+            acc[t][1] = Mfma<double>::mma(av[sl][t], bv[sl][1], acc[t][1]);
+          }
+        };
+        load(0, lk);
+        load(1, 4 + lk);  // c0 is a multiple of 32
+        for (int k0 = 0; k0 < c0; k0 += 8) {
+          mma(0);
+          if (k0 + 8 < c0) load(0, k0 + 8 + lk);
+          mma(1);
+          if (k0 + 12 < c0) load(1, k0 + 12 + lk);
         }
+      };
+      switch (tact) {
+        case 8: kloop.template operator()<8>(); break;
+        case 7: kloop.template operator()<7>(); break;
+        case 6: kloop.template operator()<6>(); break;
+        case 5: kloop.template operator()<5>(); break;
+        case 4: kloop.template operator()<4>(); break;
+        case 3: kloop.template operator()<3>(); break;
+        case 2: kloop.template operator()<2>(); break;
+        case 1: kloop.template operator()<1>(); break;
+        default: break;
       }
+    }
+#pragma unroll
+    for (int t = 0; t < kRowBlocks; ++t) {
+      acc[t][0] = -acc[t][0];
+      acc[t][1] = -acc[t][1];
     }
     // diagonal block (row blocks 0 and 1, held by waves 0 and 1) to LDS
     if (wave < 2) {
@@ -111,51 +247,16 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOpT<TS> *__restri
         }
     }
     __syncthreads();
-    // right-looking Cholesky of the 32 x 32 block (dpotf2 order)
-    {
-      const int i = tid & 31;
-      for (int j = 0; j < kPB; ++j) {
-        const double djj = Ds[j][j];
-        if (!(djj > 0.0)) {  // also catches NaN; the value is the same in every thread
-          fail = c0 + j + 1;
-          break;
-        }
-        const double d = sqrt(djj), rinv = 1.0 / d;
-        const double lij = Ds[i][j] * rinv;
-        double upd[4];
-#pragma unroll
-        for (int qq = 0; qq < 4; ++qq) {
-          const int l = (tid >> 5) + 8 * qq;
-          upd[qq] = (l > j && i >= l) ? lij * (Ds[l][j] * rinv) : 0.0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int qq = 0; qq < 4; ++qq) {
-          const int l = (tid >> 5) + 8 * qq;
-          if (l > j && i >= l) Ds[i][l] -= upd[qq];
-        }
-        if (tid < kPB) {
-          if (tid > j)
-            Ds[tid][j] *= rinv;
-          else if (tid == j)
-            Ds[j][j] = d;
-        }
-        __syncthreads();
-      }
-    }
-    if (fail) break;
-    // Di = inv(L_D), one column per thread, accumulated in LDS (no register arrays)
-    if (tid < kPB) {
-      const int c = tid;
-      for (int i = 0; i < c; ++i) Di[i][c] = 0.0;
-      Di[c][c] = 1.0 / Ds[c][c];
-      for (int i = c + 1; i < kPB; ++i) {
-        double s = 0.0;
-        for (int k = c; k < i; ++k) s += Ds[i][k] * Di[k][c];
-        Di[i][c] = -s / Ds[i][i];
-      }
+    // factor and invert the diagonal block with wave 0; the others wait
+    if (wave == 0) {
+      const int f = diag_chol_inv_wave(Ds, Di, lane);
+      if (lane == 0) fail_s = f;
     }
     __syncthreads();
+    if (fail_s) {
+      fail = c0 + fail_s;
+      break;
+    }
     // L_D back to the working matrix; inverse blocks out for the TRSM
     for (int e = tid; e < kPB * kPB; e += 256) {
       const int r = e & 31, c = e >> 5;
@@ -254,6 +355,7 @@ template <typename T, bool TRANS>
 __global__ __launch_bounds__(256) void trsm_kernel(const TrsmOp<T> *__restrict__ ops, TrsmOp<T> one, int bpo,
                                                    const int *__restrict__ d_abort) {
   if (d_abort && *d_abort) return;
+  __builtin_amdgcn_s_setprio(2);  // panel work: ahead of co-resident update waves
   const int o = blockIdx.x / bpo, s = blockIdx.x % bpo;
   const TrsmOp<T> op = ops ? ops[o] : one;
   const int r0 = s * 64;

@@ -188,7 +188,9 @@ struct MixedPlan : MixedPlanBase {
       *berr = INFINITY;
       return TTG_CHOL_OK;  // caller reads *d_info
     }
-    // 3. initial solve
+    // 3. initial solve (inverses of the diagonal tiles first)
+    rc = splan->prepare(s);
+    if (rc) return rc;
     rc = solve(s, B, ldb, X, ldx, false);
     if (rc) return rc;
     // 4. refinement

@@ -2,11 +2,14 @@
 // building blocks of iterative refinement).
 //
 //   potrs   : B := (L L^T)^{-1} B.  The right-hand sides are held transposed
-//             (Bt = B^T, nrhs x N) so both sweeps are tile ops the kernel
-//             library already has:  forward  Bt_i := Bt_i L_ii^{-T},
-//             Bt_m -= Bt_i L_mi^T (m > i);  backward  Bt_i := Bt_i L_ii^{-1},
-//             Bt_m -= Bt_i L_im (m < i).  Every step's updates are one launch;
-//             the sequence is captured in a hipGraph.
+//             (Bt = B^T, nrhs x N).  The diagonal tiles are inverted once per
+//             factor (batched TRSM of identity tiles, all tiles in one launch),
+//             so both sweeps are skinny GEMMs with a fixed, short latency per
+//             step instead of a sequential triangular solve:
+//             forward  Xt_i = Bt_i Linv_i^T, Bt_m -= Xt_i L_mi^T (m > i);
+//             backward Bt_i = Xt_i Linv_i,   Xt_m -= Bt_i L_im   (m < i).
+//             Every step is two launches; the sequence is captured in a
+//             hipGraph.
 //   residual: R := B - A X with A symmetric from its lower tiles.  Each tile
 //             (m, n) is read once; a workgroup forms A_mn X_n and A_mn^T X_m
 //             from 64 x 64 LDS blocks and stores them to per-tile slots, and a
@@ -72,52 +75,72 @@ template hipError_t launch_from_rows<float, double>(hipStream_t, int, int, const
 // ---------------------------------------------------------------------------
 // triangular-solve plan
 // ---------------------------------------------------------------------------
+// identity into every diagonal-tile slot of the inverse (dim x dim, zero padding)
+template <typename T>
+__global__ void identity_tiles_kernel(int N, int NB, T *__restrict__ linv) {
+  const int k = blockIdx.y;
+  const int dim = min(NB, N - k * NB);
+  T *t = linv + (long)k * NB * NB;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < (long)NB * NB; e += (long)gridDim.x * blockDim.x) {
+    const int r = (int)(e % NB), c = (int)(e / NB);
+    t[e] = (r == c && r < dim) ? T(1) : T(0);
+  }
+}
+
 template <typename T>
 int SolvePlan<T>::build() {
   NT = (N + NB - 1) / NB;
   Np = NT * NB;
   dstride = ((NB + 31) / 32) * 32 * 32;
   TTGC_HIP(d_bt.reserve(sizeof(T) * (size_t)Np * nrhs));
+  TTGC_HIP(d_xt.reserve(sizeof(T) * (size_t)Np * nrhs));
   TTGC_HIP(d_dinv.reserve(sizeof(T) * (size_t)dstride * NT));
+  TTGC_HIP(d_linv.reserve(sizeof(T) * (size_t)NB * NB * NT));
   T *bt = static_cast<T *>(d_bt.ptr);
+  T *xt = static_cast<T *>(d_xt.ptr);
   T *dinv = static_cast<T *>(d_dinv.ptr);
+  T *linv = static_cast<T *>(d_linv.ptr);
   std::vector<PotrfOpT<T>> dj;
-  std::vector<TrsmOp<T>> trsm;
+  std::vector<TrsmOp<T>> inv;
   std::vector<UpdOp<T>> upd;
   auto tile = [&](int m, int n) { return tiles + tile_index(m, n, NT) * (long)NB * NB; };
+  auto li = [&](int i) { return linv + (long)i * NB * NB; };
   auto rows = [&](int i) { return bt + (long)i * NB * nrhs; };
+  auto xrows = [&](int i) { return xt + (long)i * NB * nrhs; };
   // actual (edge) sizes everywhere: the solve never reads tile padding
   auto dim = [&](int i) { return std::min(NB, N - i * NB); };
-  for (int k = 0; k < NT; ++k)
+  for (int k = 0; k < NT; ++k) {
     dj.push_back(PotrfOpT<T>{const_cast<T *>(tile(k, k)), dinv + (long)k * dstride, nullptr, dim(k), NB, 0, 0});
-  auto dinv_of = [&](int i) { return (const T *)(dinv + (long)i * dstride); };
-  // forward
+    // Linv_k := I L_kk^{-1}
+    inv.push_back(TrsmOp<T>{tile(k, k), li(k), dinv + (long)k * dstride, dim(k), dim(k), NB, NB});
+  }
+  // forward: apply (beta 0, B^T), then the updates (B^T)
   for (int i = 0; i < NT; ++i) {
-    fw_trsm.push_back((int)trsm.size());
-    trsm.push_back(TrsmOp<T>{tile(i, i), rows(i), dinv_of(i), nrhs, dim(i), NB, nrhs});
+    fw_app.push_back((int)upd.size());
+    upd.push_back(UpdOp<T>{rows(i), li(i), xrows(i), nrhs, dim(i), dim(i), nrhs, NB, nrhs, 0});
     fw_upd_off.push_back((int)upd.size());
     for (int m = i + 1; m < NT; ++m)
-      upd.push_back(UpdOp<T>{rows(i), tile(m, i), rows(m), nrhs, dim(m), dim(i), nrhs, NB, nrhs, 0});
+      upd.push_back(UpdOp<T>{xrows(i), tile(m, i), rows(m), nrhs, dim(m), dim(i), nrhs, NB, nrhs, 0});
     fw_upd_cnt.push_back((int)upd.size() - fw_upd_off.back());
   }
-  // backward
+  // backward: apply (beta 0, B as stored), then the updates (B as stored)
   for (int i = NT - 1; i >= 0; --i) {
-    bw_trsm.push_back((int)trsm.size());
-    trsm.push_back(TrsmOp<T>{tile(i, i), rows(i), dinv_of(i), nrhs, dim(i), NB, nrhs});
+    bw_app.push_back((int)upd.size());
+    upd.push_back(UpdOp<T>{xrows(i), li(i), rows(i), nrhs, dim(i), dim(i), nrhs, NB, nrhs, 0});
     bw_upd_off.push_back((int)upd.size());
     for (int m = 0; m < i; ++m)
-      upd.push_back(UpdOp<T>{rows(i), tile(i, m), rows(m), nrhs, dim(m), dim(i), nrhs, NB, nrhs, 0});
+      upd.push_back(UpdOp<T>{rows(i), tile(i, m), xrows(m), nrhs, dim(m), dim(i), nrhs, NB, nrhs, 0});
     bw_upd_cnt.push_back((int)upd.size() - bw_upd_off.back());
   }
   auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
   o_dj = 0;
-  o_trsm = align(dj.size() * sizeof(PotrfOpT<T>));
-  o_upd = align(o_trsm + trsm.size() * sizeof(TrsmOp<T>));
+  o_inv = align(dj.size() * sizeof(PotrfOpT<T>));
+  o_upd = align(o_inv + inv.size() * sizeof(TrsmOp<T>));
   const size_t total = align(o_upd + upd.size() * sizeof(UpdOp<T>));
   TTGC_HIP(d_ops.reserve(total));
   std::vector<char> host(total);
   memcpy(host.data() + o_dj, dj.data(), dj.size() * sizeof(PotrfOpT<T>));
-  memcpy(host.data() + o_trsm, trsm.data(), trsm.size() * sizeof(TrsmOp<T>));
+  memcpy(host.data() + o_inv, inv.data(), inv.size() * sizeof(TrsmOp<T>));
   if (!upd.empty()) memcpy(host.data() + o_upd, upd.data(), upd.size() * sizeof(UpdOp<T>));
   TTGC_HIP(hipMemcpy(d_ops.ptr, host.data(), total, hipMemcpyHostToDevice));
   TTGC_HIP(hipStreamCreateWithFlags(&s_main, hipStreamNonBlocking));
@@ -127,21 +150,29 @@ int SolvePlan<T>::build() {
 }
 
 template <typename T>
-int SolvePlan<T>::enqueue_sweeps(hipStream_t s) {
+int SolvePlan<T>::prepare(hipStream_t s) {
   PotrfOpT<T> pnone{};
   TrsmOp<T> tnone{};
-  UpdOp<T> unone{};
   TTGC_HIP(launch_dinv(s, dj_ops(), NT, pnone));
-  const T m1 = T(-1), p1 = T(1);
+  hipLaunchKernelGGL(identity_tiles_kernel<T>, dim3(64, NT), dim3(256), 0, s, N, NB, static_cast<T *>(d_linv.ptr));
+  TTGC_HIP(hipGetLastError());
+  TTGC_HIP(launch_trsm<T>(s, inv_ops(), NT, tnone, NB, nullptr, false));
+  return TTG_CHOL_OK;
+}
+
+template <typename T>
+int SolvePlan<T>::enqueue_sweeps(hipStream_t s) {
+  UpdOp<T> unone{};
+  const T m1 = T(-1), p1 = T(1), z = T(0);
   const bool fast = NB % 128 == 0 && N % NB == 0;
   for (int q = 0; q < NT; ++q) {
-    TTGC_HIP(launch_trsm<T>(s, trsm_ops() + fw_trsm[q], 1, tnone, nrhs, nullptr, true));
+    TTGC_HIP(launch_update<T>(s, upd_ops() + fw_app[q], 1, unone, nrhs, NB, p1, z, fast, nullptr, false));
     if (fw_upd_cnt[q])
       TTGC_HIP(launch_update<T>(s, upd_ops() + fw_upd_off[q], fw_upd_cnt[q], unone, nrhs, NB, m1, p1, fast, nullptr,
                                 false));
   }
   for (int q = 0; q < NT; ++q) {
-    TTGC_HIP(launch_trsm<T>(s, trsm_ops() + bw_trsm[q], 1, tnone, nrhs, nullptr, false));
+    TTGC_HIP(launch_update<T>(s, upd_ops() + bw_app[q], 1, unone, nrhs, NB, p1, z, fast, nullptr, true));
     if (bw_upd_cnt[q])
       TTGC_HIP(launch_update<T>(s, upd_ops() + bw_upd_off[q], bw_upd_cnt[q], unone, nrhs, NB, m1, p1, fast, nullptr,
                                 true));
@@ -156,7 +187,7 @@ SolvePlan<T>::~SolvePlan() {
   if (ev_in) (void)hipEventDestroy(ev_in);
   if (ev_out) (void)hipEventDestroy(ev_out);
   if (s_main) (void)hipStreamDestroy(s_main);
-  for (DevBuf *b : {&d_ops, &d_bt, &d_dinv})
+  for (DevBuf *b : {&d_ops, &d_bt, &d_xt, &d_dinv, &d_linv})
     if (b->ptr) (void)hipFree(b->ptr);
 }
 
@@ -388,6 +419,8 @@ int ttg_chol_dpotrs_tiled(void *stream, int N, int NB, const double *tiles, int 
   SolvePlan<double> *p = get_solve_plan<double>(N, NB, tiles, nrhs, &rc);
   if (!p) return rc;
   double *bt = static_cast<double *>(p->d_bt.ptr);
+  rc = p->prepare(s);
+  if (rc != TTG_CHOL_OK) return rc;
   TTGC_HIP(launch_to_rows<double, double>(s, N, p->Np, nrhs, B, ldb, bt));
   rc = p->run_sweeps(s);
   if (rc != TTG_CHOL_OK) return rc;

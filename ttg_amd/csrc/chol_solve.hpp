@@ -12,14 +12,18 @@ namespace ttgc {
 
 // Forward + backward sweeps for nrhs (<= 32) right-hand sides held
 // transposed in d_bt (nrhs x Np, Np = NT*NB, zero padded).  T is the
-// element type of the factor tiles.
+// element type of the factor tiles.  prepare() inverts every diagonal tile
+// of the factor (d_linv); the sweeps are then skinny GEMMs only:
+//   forward   Xt_i = Bt_i Linv_i^T,  Bt_m -= Xt_i L_mi^T  (m > i)
+//   backward  Bt_i = Xt_i Linv_i,    Xt_m -= Bt_i L_im    (m < i)
+// leaving the solution in d_bt.
 template <typename T>
 struct SolvePlan {
   int N = 0, NB = 0, NT = 0, Np = 0, nrhs = 0, dstride = 0;
   const T *tiles = nullptr;
-  DevBuf d_ops, d_bt, d_dinv;
-  size_t o_dj = 0, o_trsm = 0, o_upd = 0;
-  std::vector<int> fw_trsm, fw_upd_off, fw_upd_cnt, bw_trsm, bw_upd_off, bw_upd_cnt;
+  DevBuf d_ops, d_bt, d_xt, d_dinv, d_linv;
+  size_t o_dj = 0, o_inv = 0, o_upd = 0;
+  std::vector<int> fw_app, fw_upd_off, fw_upd_cnt, bw_app, bw_upd_off, bw_upd_cnt;
   hipStream_t s_main = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   hipGraph_t graph = nullptr;
@@ -27,10 +31,11 @@ struct SolvePlan {
 
   ~SolvePlan();
   int build();
+  int prepare(hipStream_t s);  // after every (re)factorization of the tiles
   int enqueue_sweeps(hipStream_t s);
   int run_sweeps(hipStream_t user);
   PotrfOpT<T> *dj_ops() { return reinterpret_cast<PotrfOpT<T> *>(static_cast<char *>(d_ops.ptr) + o_dj); }
-  TrsmOp<T> *trsm_ops() { return reinterpret_cast<TrsmOp<T> *>(static_cast<char *>(d_ops.ptr) + o_trsm); }
+  TrsmOp<T> *inv_ops() { return reinterpret_cast<TrsmOp<T> *>(static_cast<char *>(d_ops.ptr) + o_inv); }
   UpdOp<T> *upd_ops() { return reinterpret_cast<UpdOp<T> *>(static_cast<char *>(d_ops.ptr) + o_upd); }
   T *bt() { return static_cast<T *>(d_bt.ptr); }
 };

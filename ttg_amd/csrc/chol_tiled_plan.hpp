@@ -43,7 +43,7 @@ struct TiledPlanT {
   size_t o_potrf = 0, o_trsm = 0, o_upd = 0;
   std::vector<int> trsm_off, trsm_cnt, updA_off, updA_cnt, updB_off, updB_cnt;
   hipStream_t s_main = nullptr, s_look = nullptr;
-  hipEvent_t ev_in = nullptr, ev_out = nullptr, ev_panel = nullptr, ev_col = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr, ev_panel = nullptr, ev_col = nullptr, ev_diag = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int *bound_info = nullptr;  // the d_info the captured graph writes
@@ -52,7 +52,7 @@ struct TiledPlanT {
   ~TiledPlanT() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
-    for (hipEvent_t e : {ev_in, ev_out, ev_panel, ev_col})
+    for (hipEvent_t e : {ev_in, ev_out, ev_panel, ev_col, ev_diag})
       if (e) (void)hipEventDestroy(e);
     if (s_main) (void)hipStreamDestroy(s_main);
     if (s_look) (void)hipStreamDestroy(s_look);
@@ -110,7 +110,7 @@ struct TiledPlanT {
     }
     TTGC_HIP(hipStreamCreateWithFlags(&s_main, hipStreamNonBlocking));
     TTGC_HIP(hipStreamCreateWithFlags(&s_look, hipStreamNonBlocking));
-    for (hipEvent_t *e : {&ev_in, &ev_out, &ev_panel, &ev_col})
+    for (hipEvent_t *e : {&ev_in, &ev_out, &ev_panel, &ev_col, &ev_diag})
       TTGC_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
     return TTG_CHOL_OK;
   }
@@ -124,11 +124,11 @@ struct TiledPlanT {
     const size_t per = (size_t)NB * NB;  // halves per part
     const size_t slot_tiles = NT > 1 ? NT - 1 : 1;
     TTGC_HIP(d_panel.reserve(2 * slot_tiles * 2 * per * sizeof(uint16_t)));
-    TTGC_HIP(d_scale.reserve(2 * slot_tiles * sizeof(float)));
+    TTGC_HIP(d_scale.reserve(2 * slot_tiles * NB * sizeof(float)));
     uint16_t *panel = static_cast<uint16_t *>(d_panel.ptr);
     float *scale = static_cast<float *>(d_scale.ptr);
     auto hi = [&](int k, int m) { return panel + ((size_t)(k % 2) * slot_tiles + (m - k - 1)) * 2 * per; };
-    auto sc = [&](int k, int m) { return scale + (size_t)(k % 2) * slot_tiles + (m - k - 1); };
+    auto sc = [&](int k, int m) { return scale + ((size_t)(k % 2) * slot_tiles + (m - k - 1)) * NB; };
     std::vector<SplitOp> sp;
     std::vector<F16UpdOp> fu;
     for (int k = 0; k < NT; ++k) {
@@ -197,13 +197,22 @@ struct TiledPlanT {
       if (prof) prof_end(slot, s_main, PROF_UPDATE, (cnt - nsyrk) * 2.0 * b3 + nsyrk * (double)NB * NB * (NB + 1));
       return TTG_CHOL_OK;
     };
-    for (int k = 0; k < NT; ++k) {
-      if (la) TTGC_HIP(hipStreamWaitEvent(s_look, ev_col, 0));
-      int slot = prof ? prof_begin(sp) : -1;
+    auto potrf = [&](int k) -> int {
+      const int slot = prof ? prof_begin(sp) : -1;
       TTGC_HIP(launch_potrf<T>(sp, potrf_ops() + k, 1, pnone, info));
       if (prof) prof_end(slot, sp, PROF_POTRF, b3 / 3.0);
+      return TTG_CHOL_OK;
+    };
+    // Step k: TRSM(k) (+ split) on the panel stream, updA(k) (column k+1),
+    // then POTRF(k+1) -- which needs only updA(k) -- BEFORE updB(k) is
+    // released: the single-workgroup POTRF then finds free CUs instead of
+    // waiting behind thousands of update workgroups, and TRSM(k+1) overlaps
+    // updB(k).
+    if (la) TTGC_HIP(hipStreamWaitEvent(s_look, ev_col, 0));
+    if (int rc = potrf(0)) return rc;
+    for (int k = 0; k < NT; ++k) {
       if (trsm_cnt[k]) {
-        slot = prof ? prof_begin(sp) : -1;
+        const int slot = prof ? prof_begin(sp) : -1;
         TTGC_HIP(launch_trsm<T>(sp, trsm_ops() + trsm_off[k], trsm_cnt[k], tnone, NB, info));
         if (prof) prof_end(slot, sp, PROF_TRSM, trsm_cnt[k] * b3);
         if (f16x2) TTGC_HIP(launch_split(sp, split_ops() + split_off[k], split_cnt[k], NB));
@@ -216,7 +225,17 @@ struct TiledPlanT {
         int rc = upd(updA_off[k], updA_cnt[k], 1);
         if (rc) return rc;
       }
-      if (la) TTGC_HIP(hipEventRecord(ev_col, s_main));
+      if (k + 1 < NT) {
+        if (la) {
+          TTGC_HIP(hipEventRecord(ev_col, s_main));
+          TTGC_HIP(hipStreamWaitEvent(s_look, ev_col, 0));
+        }
+        if (int rc = potrf(k + 1)) return rc;
+        if (la && updB_cnt[k]) {
+          TTGC_HIP(hipEventRecord(ev_diag, s_look));
+          TTGC_HIP(hipStreamWaitEvent(s_main, ev_diag, 0));
+        }
+      }
       if (updB_cnt[k]) {
         int rc = upd(updB_off[k], updB_cnt[k], NT - k - 2);
         if (rc) return rc;
