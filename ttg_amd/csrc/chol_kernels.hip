@@ -25,10 +25,12 @@ constexpr int kMaxTile = 512;                 // largest single-tile POTRF/TRSM
 // blocks).  For block column j, the rows c0..n-1 of that column are held in
 // MFMA accumulators spread over the 4 waves (16-row blocks); they are updated
 // by every earlier block column in one streamed GEMM (operands straight from
-// L2), then the 32 x 32 diagonal block is factored and inverted in LDS and the
-// rows below are multiplied by the inverse.  Nothing but the 32 x 32 blocks
-// lives in LDS, so the kernel co-resides with the trailing-update kernels.
-// Float tiles are factored in an FP64 copy (op.pw, n x n) and rounded once.
+// L2, two k-steps in flight), then the 32 x 32 diagonal block is factored and
+// inverted by one wave in registers and the rows below are multiplied by the
+// inverse.  The waves run at raised priority: the diagonal tiles are the
+// critical path of the factorization.
+// Float tiles: the finished factor columns go to an FP64 workspace (op.pw,
+// n x n) for the later columns to read, and rounded into the tile at once.
 // ---------------------------------------------------------------------------
 constexpr int kRowBlocks = kMaxTile / 16 / 4;  // 16-row blocks per wave (<= 8)
 
@@ -124,21 +126,26 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOpT<TS> *__restri
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: lets the row-block tests branch on scalars
   const int lr = lane & 15, lk = lane >> 4;
   const int n = __builtin_amdgcn_readfirstlane(op.n);
-  // working FP64 matrix
+  // working FP64 matrix: the tile itself, or for float tiles an FP64
+  // workspace that receives the factor columns as they are finished (the
+  // float tile is read once, at the initialisation of each block column, and
+  // written once, rounded, next to the FP64 copy)
+  const long lda = __builtin_amdgcn_readfirstlane(op.lda);
   double *__restrict__ W;
   long ldw;
   if constexpr (sizeof(TS) == 8) {
     W = reinterpret_cast<double *>(op.A);
-    ldw = __builtin_amdgcn_readfirstlane(op.lda);
+    ldw = lda;
   } else {
     W = op.pw;
     ldw = n;
-    for (long e = tid; e < (long)n * n; e += 256) {
-      const int r = (int)(e % n), c = (int)(e / n);
-      if (r >= c) W[r + (long)c * ldw] = (double)op.A[r + (long)c * op.lda];
-    }
-    wg_sync_global();
   }
+  typedef __attribute__((address_space(1))) const TS gts;
+  const gts *Ag = (const gts *)op.A;
+  auto store = [&](long r, long c, double v) {
+    W[r + c * ldw] = v;
+    if constexpr (sizeof(TS) != 8) op.A[r + c * lda] = (TS)v;
+  };
   int fail = 0;
   const int nblk = (n + kPB - 1) / kPB;
 
@@ -160,9 +167,9 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOpT<TS> *__restri
 #pragma unroll
         for (int jj = 0; jj < 2; ++jj) {
           const int col = c0 + jj * 16 + lr;
-          const long oc = (long)min(col, n - 1) * ldw;
 #pragma unroll
-          for (int r = 0; r < 4; ++r) acc[t][jj][r] = Wg[min(c0 + q * 16 + Mfma<double>::drow(lk, r), n - 1) + oc];
+          for (int r = 0; r < 4; ++r)
+            acc[t][jj][r] = (double)Ag[min(c0 + q * 16 + Mfma<double>::drow(lk, r), n - 1) + (long)min(col, n - 1) * lda];
         }
 #pragma unroll
         for (int jj = 0; jj < 2; ++jj) {
@@ -260,7 +267,7 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOpT<TS> *__restri
     // L_D back to the working matrix; inverse blocks out for the TRSM
     for (int e = tid; e < kPB * kPB; e += 256) {
       const int r = e & 31, c = e >> 5;
-      if (r >= c && r < bs && c < bs) W[c0 + r + (long)(c0 + c) * ldw] = Ds[r][c];
+      if (r >= c && r < bs && c < bs) store(c0 + r, c0 + c, Ds[r][c]);
       if (op.dinv) op.dinv[jb * kPB * kPB + e] = (TS)Di[r][c];
     }
     // rows below the diagonal block: X = acc * Di^T (staged through LDS into
@@ -288,21 +295,13 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOpT<TS> *__restri
         for (int r = 0; r < 4; ++r) {
           const int row = c0 + q * 16 + Mfma<double>::drow(lk, r);
           if (row < n) {
-            if (lr < bs) W[row + (long)(c0 + lr) * ldw] = x0[r];
-            if (16 + lr < bs) W[row + (long)(c0 + 16 + lr) * ldw] = x1[r];
+            if (lr < bs) store(row, c0 + lr, x0[r]);
+            if (16 + lr < bs) store(row, c0 + 16 + lr, x1[r]);
           }
         }
       }
     }
     wg_sync_global();
-  }
-  if constexpr (sizeof(TS) != 8) {
-    // round the FP64 factor once into the float tile (columns done so far)
-    const int done = fail ? (fail - 1) / kPB * kPB : n;
-    for (long e = tid; e < (long)n * n; e += 256) {
-      const int r = (int)(e % n), c = (int)(e / n);
-      if (r >= c && c < done) op.A[r + (long)c * op.lda] = (TS)W[r + (long)c * ldw];
-    }
   }
   if (tid == 0 && op.info) {
     if (fail)

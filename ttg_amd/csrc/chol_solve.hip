@@ -12,8 +12,9 @@
 //             hipGraph.
 //   residual: R := B - A X with A symmetric from its lower tiles.  Each tile
 //             (m, n) is read once; a workgroup forms A_mn X_n and A_mn^T X_m
-//             from 64 x 64 LDS blocks and stores them to per-tile slots, and a
-//             second kernel sums the slots in a fixed order (deterministic).
+//             on FP64 MFMA from 64 x 64 LDS blocks and stores them to per-tile
+//             slots, and a second kernel sums the slots in a fixed order
+//             (deterministic).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,6 +25,7 @@
 #include <tuple>
 #include <vector>
 
+#include "chol_device.hpp"
 #include "chol_internal.hpp"
 #include "chol_kernels.hpp"
 #include "chol_solve.hpp"
@@ -254,11 +256,25 @@ template SolvePlan<float> *get_solve_plan<float>(int, int, const float *, int, i
 // ---------------------------------------------------------------------------
 constexpr int kRB = 64;  // LDS block of A
 
-// W layout: [tile][slot 0/1][kRhsChunk][NB]
+// X (column-major, ldx) columns j0 .. j0+nr-1 -> Xr (Np x kRhsChunk, row i at
+// i * kRhsChunk; zero beyond N and beyond nr)
+__global__ void residual_rows_kernel(int N, int Np, const double *__restrict__ X, long ldx, int nr,
+                                     double *__restrict__ Xr) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long)Np * kRhsChunk) return;
+  const int j = (int)(g % kRhsChunk), i = (int)(g / kRhsChunk);
+  Xr[g] = (i < N && j < nr) ? X[i + (long)j * ldx] : 0.0;
+}
+
+// One workgroup per lower tile (m, n):  P = A_mn X_n  (rows of block m) and
+// Q = A_mn^T X_m (rows of block n; strictly lower part on a diagonal tile),
+// both on FP64 MFMA from 64 x 64 blocks of A staged in LDS (each element of A
+// read from HBM once), stored row-major to the tile's slots W[t][0|1][NB][16].
+// Wave w owns rows w*16..w*16+15 of every 64-row block for P and of every
+// 64-column block for Q.
 template <bool ABS>
 __global__ __launch_bounds__(256) void residual_tile_kernel(int N, int NB, int nt, const double *__restrict__ tiles,
-                                                            const double *__restrict__ X, long ldx, int nr,
-                                                            double *__restrict__ W) {
+                                                            const double *__restrict__ Xr, double *__restrict__ W) {
   const long t = blockIdx.x;
   int m, n;
   {
@@ -276,85 +292,101 @@ __global__ __launch_bounds__(256) void residual_tile_kernel(int N, int NB, int n
   const double *A = tiles + t * (long)NB * NB;
   double *W0 = W + t * 2L * kRhsChunk * NB;
   double *W1 = W0 + (long)kRhsChunk * NB;
+  const double *Xn = Xr + (long)n * NB * kRhsChunk;
+  const double *Xm = Xr + (long)m * NB * kRhsChunk;
   __shared__ double blk[kRB][kRB + 1];  // blk[c][r]
-  __shared__ double xn[kRB][kRhsChunk];
-  __shared__ double xm[kRB][kRhsChunk];
-  const int tid = threadIdx.x, rl = tid & 63, jg = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lr = lane & 15, lk = lane >> 4;
   const bool diag = (m == n);
   const int nsb = (NB + kRB - 1) / kRB;
   const int mrows = min(NB, N - m * NB), ncols = min(NB, N - n * NB);
-  double pacc[8][4];  // nsb <= 8 (NB <= 512)
+  d4_t pacc[8];  // nsb <= 8 (NB <= 512)
 #pragma unroll
-  for (int a = 0; a < 8; ++a)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) pacc[a][q] = 0.0;
+  for (int a = 0; a < 8; ++a) pacc[a] = d4_t{0.0, 0.0, 0.0, 0.0};
   for (int cs = 0; cs < nsb; ++cs) {
-    double qacc[4] = {0.0, 0.0, 0.0, 0.0};
+    // B operand of P: X_n rows cs*64 + 4 ks + lk, right-hand side lr
+    double xb[kRB / 4];
 #pragma unroll
-    for (int rs = 0; rs < 8; ++rs) {
-      if (rs >= nsb) break;
-      if (diag && rs < cs) continue;  // strictly upper blocks of a diagonal tile are not stored
-      __syncthreads();
-      for (int e = tid; e < kRB * kRB; e += 256) {
-        const int r = e & 63, c = e >> 6;
-        const int gr = rs * kRB + r, gc = cs * kRB + c;
-        double v = 0.0;
-        if (gr < mrows && gc < ncols && (!diag || gr >= gc)) v = ABS ? fabs(A[gr + (long)gc * NB]) : A[gr + (long)gc * NB];
-        blk[c][r] = v;
-      }
-      for (int e = tid; e < kRB * kRhsChunk; e += 256) {
-        const int r = e % kRB, j = e / kRB;
-        const int gcol = n * NB + cs * kRB + r, grow = m * NB + rs * kRB + r;
-        xn[r][j] = (j < nr && gcol < N && cs * kRB + r < ncols) ? X[gcol + (long)j * ldx] : 0.0;
-        xm[r][j] = (j < nr && grow < N && rs * kRB + r < mrows) ? X[grow + (long)j * ldx] : 0.0;
-      }
-      __syncthreads();
-      // P: rows of this block times X_n
-#pragma unroll 8
-      for (int c = 0; c < kRB; ++c) {
-        const double a = blk[c][rl];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) pacc[rs][q] += a * xn[c][jg + 4 * q];
-      }
-      // Q: columns of this block times X_m (strictly lower part only on a diagonal tile)
-      const bool dblk = diag && rs == cs;
-#pragma unroll 8
-      for (int r = 0; r < kRB; ++r) {
-        double a = blk[rl][r];
-        if (dblk && r <= rl) a = 0.0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) qacc[q] += a * xm[r][jg + 4 * q];
-      }
+    for (int ks = 0; ks < kRB / 4; ++ks) {
+      const int k = cs * kRB + 4 * ks + lk;
+      xb[ks] = (k < NB) ? Xn[(long)k * kRhsChunk + lr] : 0.0;
     }
-    if (cs * kRB + rl < NB)
+    d4_t qacc = {0.0, 0.0, 0.0, 0.0};
+    // a runtime loop over the row blocks (nothing of later blocks is hoisted);
+    // the P accumulators rotate so that block rs is always pacc[0], and after
+    // the 8 steps every block is back in its own slot
+#pragma unroll 1
+    for (int rs = 0; rs < 8; ++rs) {
+      // strictly upper blocks of a diagonal tile are not stored
+      if (rs < nsb && !(diag && rs < cs)) {
+        __syncthreads();
+        for (int e = tid; e < kRB * kRB; e += 256) {
+          const int r = e & 63, c = e >> 6;
+          const int gr = rs * kRB + r, gc = cs * kRB + c;
+          double v = 0.0;
+          if (gr < mrows && gc < ncols && (!diag || gr >= gc)) v = A[gr + (long)gc * NB];
+          blk[c][r] = ABS ? fabs(v) : v;
+        }
+        double xm[kRB / 4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) W1[(long)(jg + 4 * q) * NB + cs * kRB + rl] = qacc[q];
+        for (int ks = 0; ks < kRB / 4; ++ks) {
+          const int k = rs * kRB + 4 * ks + lk;
+          xm[ks] = (k < NB) ? Xm[(long)k * kRhsChunk + lr] : 0.0;
+        }
+        __syncthreads();
+        // P rows rs*64 + w*16 + (0..15):  sum over this block's 64 columns
+#pragma unroll
+        for (int ks = 0; ks < kRB / 4; ++ks)
+          pacc[0] = Mfma<double>::mma(blk[4 * ks + lk][w * 16 + lr], xb[ks], pacc[0]);
+        if (diag && rs == cs) {  // Q takes the strictly lower part only
+          __syncthreads();
+          if (tid < kRB) blk[tid][tid] = 0.0;
+          __syncthreads();
+        }
+        // Q rows (= columns of A) cs*64 + w*16 + (0..15):  sum over this block's 64 rows
+#pragma unroll
+        for (int ks = 0; ks < kRB / 4; ++ks) qacc = Mfma<double>::mma(blk[w * 16 + lr][4 * ks + lk], xm[ks], qacc);
+      }
+      const d4_t first = pacc[0];
+#pragma unroll
+      for (int a = 0; a < 7; ++a) pacc[a] = pacc[a + 1];
+      pacc[7] = first;
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int row = cs * kRB + w * 16 + Mfma<double>::drow(lk, e);
+      if (row < NB) W1[(long)row * kRhsChunk + lr] = qacc[e];
+    }
   }
 #pragma unroll
   for (int rs = 0; rs < 8; ++rs) {
     if (rs >= nsb) break;
-    if (rs * kRB + rl < NB)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) W0[(long)(jg + 4 * q) * NB + rs * kRB + rl] = pacc[rs][q];
+    for (int e = 0; e < 4; ++e) {
+      const int row = rs * kRB + w * 16 + Mfma<double>::drow(lk, e);
+      if (row < NB) W0[(long)row * kRhsChunk + lr] = pacc[rs][e];
+    }
   }
 }
 
-// R(i, j) = B(i, j) - sum_{n <= m} W0(m, n) - sum_{n' >= m} W1(n', m),  m = i / NB
+// R(i, j) = B(i, j) - sum_{n <= m} P(m, n)[r][j] - sum_{n' >= m} Q(n', m)[r][j],
+// m = i / NB, r = i % NB: a fixed summation order (deterministic)
 __global__ void residual_reduce_kernel(int N, int NB, int nt, const double *__restrict__ W, int nr,
                                        const double *__restrict__ B, long ldb, double *__restrict__ R, long ldr,
                                        int j0) {
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= (long)N * nr) return;
-  const int i = (int)(g % N), j = (int)(g / N);
+  const int j = (int)(g % nr), i = (int)(g / nr);
   const int m = i / NB, r = i % NB;
   double s = 0.0;
   for (int n = 0; n <= m; ++n) {
     const long t = tile_index(m, n, nt);
-    s += W[t * 2L * kRhsChunk * NB + (long)j * NB + r];
+    s += W[t * 2L * kRhsChunk * NB + (long)r * kRhsChunk + j];
   }
   for (int n2 = m; n2 < nt; ++n2) {
     const long t = tile_index(n2, m, nt);
-    s += W[t * 2L * kRhsChunk * NB + (long)kRhsChunk * NB + (long)j * NB + r];
+    s += W[t * 2L * kRhsChunk * NB + (long)kRhsChunk * NB + (long)r * kRhsChunk + j];
   }
   R[i + (long)(j0 + j) * ldr] = B[i + (long)(j0 + j) * ldb] - s;
 }
@@ -365,6 +397,7 @@ static std::map<hipStream_t, std::unique_ptr<DevBuf>> g_res_ws;
 int residual_fp64(hipStream_t s, int N, int NB, const double *tiles, int nrhs, const double *X, long ldx,
                   const double *B, long ldb, double *R, long ldr, bool abs_a) {
   const int nt = (N + NB - 1) / NB;
+  const long Np = (long)nt * NB;
   const long ntiles = (long)nt * (nt + 1) / 2;
   DevBuf *ws;
   {
@@ -373,16 +406,20 @@ int residual_fp64(hipStream_t s, int N, int NB, const double *tiles, int nrhs, c
     if (!p) p.reset(new DevBuf);
     ws = p.get();
   }
-  TTGC_HIP(ws->reserve(sizeof(double) * (size_t)ntiles * 2 * kRhsChunk * NB));
+  const size_t wsz = (size_t)ntiles * 2 * kRhsChunk * NB;
+  TTGC_HIP(ws->reserve(sizeof(double) * (wsz + (size_t)Np * kRhsChunk)));
   double *W = static_cast<double *>(ws->ptr);
+  double *Xr = W + wsz;
   for (int j0 = 0; j0 < nrhs; j0 += kRhsChunk) {
     const int nr = std::min(kRhsChunk, nrhs - j0);
+    hipLaunchKernelGGL(residual_rows_kernel, dim3((unsigned)((Np * kRhsChunk + 255) / 256)), dim3(256), 0, s, N,
+                       (int)Np, X + (long)j0 * ldx, ldx, nr, Xr);
+    TTGC_HIP(hipGetLastError());
     if (abs_a)
-      hipLaunchKernelGGL(residual_tile_kernel<true>, dim3((unsigned)ntiles), dim3(256), 0, s, N, NB, nt, tiles,
-                         X + (long)j0 * ldx, ldx, nr, W);
+      hipLaunchKernelGGL(residual_tile_kernel<true>, dim3((unsigned)ntiles), dim3(256), 0, s, N, NB, nt, tiles, Xr, W);
     else
-      hipLaunchKernelGGL(residual_tile_kernel<false>, dim3((unsigned)ntiles), dim3(256), 0, s, N, NB, nt, tiles,
-                         X + (long)j0 * ldx, ldx, nr, W);
+      hipLaunchKernelGGL(residual_tile_kernel<false>, dim3((unsigned)ntiles), dim3(256), 0, s, N, NB, nt, tiles, Xr,
+                         W);
     TTGC_HIP(hipGetLastError());
     const long n = (long)N * nr;
     hipLaunchKernelGGL(residual_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, N, NB, nt, W, nr,
