@@ -304,6 +304,40 @@ __global__ __launch_bounds__(256) void residual_tile_kernel(int N, int NB, int n
   d4_t pacc[8];  // nsb <= 8 (NB <= 512)
 #pragma unroll
   for (int a = 0; a < 8; ++a) pacc[a] = d4_t{0.0, 0.0, 0.0, 0.0};
+  // blocks (cs, rs) in column-major order; a diagonal tile has none above
+  // its diagonal.  The next block's 16 elements per thread are loaded (from
+  // clamped, always valid addresses) while this block is multiplied, and
+  // masked when they are written to LDS.
+  auto active = [&](int cs, int rs) { return rs < nsb && !(diag && rs < cs); };
+  auto next_block = [&](int &cs, int &rs) {
+    do {
+      if (++rs == nsb) {
+        rs = 0;
+        ++cs;
+      }
+    } while (cs < nsb && !active(cs, rs));
+  };
+  double av[16];
+  auto fetch = [&](int cs, int rs) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int e = tid + 256 * i, r = e & 63, c = e >> 6;
+      const int gr = min(rs * kRB + r, mrows - 1), gc = min(cs * kRB + c, ncols - 1);
+      av[i] = A[gr + (long)gc * NB];
+    }
+  };
+  auto stash = [&](int cs, int rs) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int e = tid + 256 * i, r = e & 63, c = e >> 6;
+      const int gr = rs * kRB + r, gc = cs * kRB + c;
+      const bool ok = gr < mrows && gc < ncols && (!diag || gr >= gc);
+      blk[c][r] = ok ? (ABS ? fabs(av[i]) : av[i]) : 0.0;
+    }
+  };
+  int ncs = 0, nrs = 0;  // the block whose elements are in av
+  if (!active(0, 0)) next_block(ncs, nrs);
+  fetch(ncs, nrs);
   for (int cs = 0; cs < nsb; ++cs) {
     // B operand of P: X_n rows cs*64 + 4 ks + lk, right-hand side lr
     double xb[kRB / 4];
@@ -318,16 +352,11 @@ __global__ __launch_bounds__(256) void residual_tile_kernel(int N, int NB, int n
     // the 8 steps every block is back in its own slot
 #pragma unroll 1
     for (int rs = 0; rs < 8; ++rs) {
-      // strictly upper blocks of a diagonal tile are not stored
-      if (rs < nsb && !(diag && rs < cs)) {
+      if (active(cs, rs)) {
         __syncthreads();
-        for (int e = tid; e < kRB * kRB; e += 256) {
-          const int r = e & 63, c = e >> 6;
-          const int gr = rs * kRB + r, gc = cs * kRB + c;
-          double v = 0.0;
-          if (gr < mrows && gc < ncols && (!diag || gr >= gc)) v = A[gr + (long)gc * NB];
-          blk[c][r] = ABS ? fabs(v) : v;
-        }
+        stash(cs, rs);
+        next_block(ncs, nrs);
+        if (ncs < nsb) fetch(ncs, nrs);
         double xm[kRB / 4];
 #pragma unroll
         for (int ks = 0; ks < kRB / 4; ++ks) {
