@@ -212,6 +212,27 @@ int ttg_chol_profile_enable(int on);
  * (No counterpart in the reference.) */
 int ttg_chol_selftest_mfma_layout(void *stream, int kind, double *out);
 
+/* ---- batched variable-size solves (no reference counterpart: many
+ * independent small SPD systems in one launch) ------------------------------ */
+#define TTG_CHOL_BATCH_MAX_N 1024
+
+/* For i < count: factor the FP32 SPD matrix A[i] (n[i] x n[i], column-major,
+ * leading dimension lda[i] >= n[i], 1 <= n[i] <= 1024; lower triangle read,
+ * overwritten by L) and overwrite b[i] (n[i] floats) with the solution of
+ * A x = b.  n, A, lda, b are HOST arrays (A[i], b[i] are device pointers);
+ * d_info (device, count ints) receives 0 or the 1-based column of the first
+ * non-positive pivot (b[i] then unchanged).  One persistent launch: the
+ * matrices are dealt largest first from a device work counter.  cfg selects a
+ * launch configuration (0 .. ttg_chol_batched_num_configs()-1, see
+ * ttg_chol_batched_config), -1 the default; ttg_amd/autotune.py picks one per
+ * workload.  Stream-ordered; the descriptors are staged through pinned host
+ * memory (the next call waits for the previous copy only). */
+int ttg_chol_sposv_batched(void *stream, int count, const int *n, float *const *A, const int *lda, float *const *b,
+                           int *d_info, int cfg);
+int ttg_chol_batched_num_configs(void);
+/* Waves per workgroup and persistent workgroups per CU of configuration cfg. */
+int ttg_chol_batched_config(int cfg, int *waves, int *wg_per_cu);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,3 +79,20 @@ def test_graph_rejects_bad_arguments_without_gpu():
     assert G.lib().ttg_graph_dpotrf(0, 32, t.ctypes.data, None) == -1
     assert G.lib().ttg_graph_dpotrf(100, 1024, t.ctypes.data, None) == -1
     assert G.lib().ttg_graph_dplgsy(100, 0, 1.0, 1, t.ctypes.data) == -1
+
+
+def test_batched_rejects_bad_sizes_without_launch():
+    import ctypes
+
+    L = C.lib()
+    n = (ctypes.c_int * 2)(64, 2000)
+    lda = (ctypes.c_int * 2)(64, 2000)
+    ptrs = (ctypes.c_void_p * 2)(16, 16)
+    assert L.ttg_chol_sposv_batched(None, 2, n, ptrs, lda, ptrs, ctypes.c_void_p(16), -1) == C.EINVAL
+    n2 = (ctypes.c_int * 1)(64)
+    lda2 = (ctypes.c_int * 1)(32)
+    p1 = (ctypes.c_void_p * 1)(16)
+    assert L.ttg_chol_sposv_batched(None, 1, n2, p1, lda2, p1, ctypes.c_void_p(16), -1) == C.EINVAL
+    assert L.ttg_chol_sposv_batched(None, 1, n2, p1, n2, p1, ctypes.c_void_p(16), 99) == C.EINVAL
+    assert L.ttg_chol_sposv_batched(None, 0, None, None, None, None, None, -1) == C.OK
+    assert L.ttg_chol_batched_num_configs() >= 2

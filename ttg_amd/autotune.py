@@ -1,0 +1,159 @@
+"""Autotuner for the launch parameters of libttg_chol.
+
+Two tunables, both measured on the GPU that will run the work and cached per
+device and workload signature:
+
+* the batched solver's launch configuration (waves per workgroup and
+  persistent workgroups per CU, `ttg_chol_batched_config`) for a batch's
+  size distribution;
+* the tile size NB of the tiled mixed-precision solve for an order N.
+
+Each candidate runs `warmup` untimed and `reps` timed times (HIP events on the
+calling stream); the median wins.  Results go to a JSON cache
+($TTG_AMD_TUNING, default ~/.cache/ttg_amd/tuning.json) keyed by the device
+name and the workload signature, so later runs reuse them without measuring.
+"""
+import json
+import os
+import statistics
+
+_CACHE_ENV = "TTG_AMD_TUNING"
+
+
+def _cache_path():
+    return os.environ.get(_CACHE_ENV) or os.path.join(os.path.expanduser("~"), ".cache", "ttg_amd", "tuning.json")
+
+
+def _load():
+    try:
+        with open(_cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(key, value):
+    data = _load()
+    data[key] = value
+    path = _cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(data, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass  # a read-only home: tuning still returns its result
+
+
+def _device_name():
+    import torch
+
+    return torch.cuda.get_device_name(0)
+
+
+def batch_signature(sizes):
+    """Size distribution of a batch: count and a histogram in 64-wide bins."""
+    hist = [0] * 16
+    for n in sizes:
+        hist[min((int(n) - 1) // 64, 15)] += 1
+    return f"batched:count={len(sizes)}:hist={','.join(map(str, hist))}"
+
+
+def _time_events(fn, warmup, reps, reset=None):
+    import torch
+
+    ts = []
+    for i in range(warmup + reps):
+        if reset is not None:
+            reset()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        if i >= warmup:
+            ts.append(e0.elapsed_time(e1))
+    return statistics.median(ts)
+
+
+def synthetic_batch(sizes, seed=0, device="cuda"):
+    """FP32 SPD matrices B B^T / n + I (B uniform in [-1, 1]) packed back to
+    back (column-major, lda = n) and uniform right-hand sides, on the device."""
+    import torch
+
+    g = torch.Generator(device=device).manual_seed(seed)
+    total = sum(int(n) * int(n) for n in sizes)
+    A = torch.empty(total, dtype=torch.float32, device=device)
+    b = torch.empty(sum(int(n) for n in sizes), dtype=torch.float32, device=device)
+    oa = ob = 0
+    for n in sizes:
+        n = int(n)
+        B = torch.rand(n, n, generator=g, device=device, dtype=torch.float32) * 2 - 1
+        M = B @ B.t() / n + torch.eye(n, device=device, dtype=torch.float32)
+        A[oa:oa + n * n] = M.reshape(-1)
+        b[ob:ob + n] = torch.rand(n, generator=g, device=device, dtype=torch.float32) * 2 - 1
+        oa += n * n
+        ob += n
+    return A, b
+
+
+def tune_batched(sizes, A=None, b=None, reps=3, warmup=1, use_cache=True):
+    """Best launch configuration for a batch of these sizes: (cfg, {cfg: ms}).
+    A and b (packed, as synthetic_batch makes them) are used as given or made
+    synthetic; their contents are restored before every timed run."""
+    import torch
+
+    import ttg_amd.chol as C
+
+    key = f"{_device_name()}|{batch_signature(sizes)}"
+    if use_cache:
+        hit = _load().get(key)
+        if hit is not None:
+            return int(hit["cfg"]), {int(k): v for k, v in hit["ms"].items()}
+    if A is None:
+        A, b = synthetic_batch(sizes)
+    A0, b0 = A.clone(), b.clone()
+    info = torch.zeros(len(sizes), dtype=torch.int32, device=A.device)
+    batch = C.Batch.packed([int(n) for n in sizes], A, b)
+
+    def reset():
+        A.copy_(A0)
+        b.copy_(b0)
+
+    ms = {}
+    for cfg in range(len(C.batched_configs())):
+        ms[cfg] = _time_events(lambda: C.sposv_batched(batch, info, cfg=cfg), warmup, reps, reset)
+    reset()
+    best = min(ms, key=ms.get)
+    _store(key, {"cfg": best, "ms": ms, "configs": C.batched_configs()})
+    return best, ms
+
+
+def tune_tiled_nb(N, candidates=(256, 512), prec=None, nrhs=16, reps=1, warmup=1, use_cache=True, seed=7):
+    """Best tile size for the mixed-precision solve of order N: (NB, {NB: ms})."""
+    import torch
+
+    import ttg_amd.chol as C
+
+    prec = C.PREC_F16X2 if prec is None else prec
+    key = f"{_device_name()}|tiled:N={N}:prec={prec}:nrhs={nrhs}"
+    if use_cache:
+        hit = _load().get(key)
+        if hit is not None:
+            return int(hit["NB"]), {int(k): v for k, v in hit["ms"].items()}
+    ms = {}
+    for NB in candidates:
+        A = C.alloc_tiles(N, NB)
+        work = torch.empty(N * N, dtype=torch.float64, device="cuda")
+        C.spd_tiled(A, N, NB, 1e-3, seed, work)
+        del work
+        B = torch.rand(nrhs, N, dtype=torch.float64, device="cuda").t()
+        X = torch.empty_like(B.t()).t()
+        ms[NB] = _time_events(lambda: C.dposv_mixed(A, N, NB, B, X, prec=prec), warmup, reps)
+        del A, B, X
+        torch.cuda.empty_cache()
+    best = min(ms, key=ms.get)
+    _store(key, {"NB": best, "ms": ms})
+    return best, ms

@@ -19,6 +19,7 @@ SOURCES = [
     "chol_profile.hip",
     "chol_mixed.hip",
     "chol_f16.hip",
+    "chol_batched.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -43,6 +43,9 @@ SIGNATURES = {
     "ttg_chol_profile_read": (_i, [_i, _p, _p, _p]),
     "ttg_chol_profile_enable": (_i, [_i]),
     "ttg_chol_dposv_mixed": (_i, [_p, _i, _i, _p, _i, _p, _i, _p, _i, _i, _d, _i, _p, _p, _p]),
+    "ttg_chol_sposv_batched": (_i, [_p, _i, _p, _p, _p, _p, _p, _i]),
+    "ttg_chol_batched_num_configs": (_i, []),
+    "ttg_chol_batched_config": (_i, [_i, _p, _p]),
 }
 
 
@@ -259,3 +262,57 @@ def spd_tiled(tiles, N, NB, alpha, seed, work, stream=None):
     _require_cuda(tiles, work)
     _check(lib().ttg_chol_spd_tiled(_stream(stream), N, NB, float(alpha), seed, _ptr(tiles), _ptr(work)),
            "spd_tiled")
+
+
+class Batch:
+    """A batch of FP32 SPD systems for sposv_batched: host arrays of sizes,
+    leading dimensions and device pointers, built once and reusable."""
+
+    def __init__(self, mats, rhs):
+        if len(mats) != len(rhs):
+            raise ValueError("one right-hand side per matrix")
+        self.count = len(mats)
+        n = [int(m.shape[0]) for m in mats]
+        self.n = (ctypes.c_int * self.count)(*n)
+        self.lda = (ctypes.c_int * self.count)(*n)
+        self.A = (ctypes.c_void_p * self.count)(*[m.data_ptr() for m in mats])
+        self.b = (ctypes.c_void_p * self.count)(*[v.data_ptr() for v in rhs])
+
+    @classmethod
+    def packed(cls, sizes, A, b):
+        """Matrices packed back to back in one float32 tensor A (each n_i^2,
+        column-major, lda = n_i) and right-hand sides in b (each n_i)."""
+        self = cls.__new__(cls)
+        self.count = len(sizes)
+        self.n = (ctypes.c_int * self.count)(*sizes)
+        self.lda = (ctypes.c_int * self.count)(*sizes)
+        a0, b0 = A.data_ptr(), b.data_ptr()
+        offa, offb, pa, pb = 0, 0, [], []
+        for s in sizes:
+            pa.append(a0 + 4 * offa)
+            pb.append(b0 + 4 * offb)
+            offa += s * s
+            offb += s
+        if offa > A.numel() or offb > b.numel():
+            raise ValueError("packed batch larger than its buffers")
+        self.A = (ctypes.c_void_p * self.count)(*pa)
+        self.b = (ctypes.c_void_p * self.count)(*pb)
+        return self
+
+
+def sposv_batched(batch, info, cfg=-1, stream=None):
+    """Factor + solve every system of `batch` in one launch (info: int32 device
+    tensor of batch.count entries)."""
+    _require_cuda(info)
+    _check(lib().ttg_chol_sposv_batched(_stream(stream), batch.count, batch.n, batch.A, batch.lda, batch.b,
+                                        _ptr(info), cfg), "sposv_batched")
+
+
+def batched_configs():
+    """[(waves per workgroup, workgroups per CU)] of the batched launch configurations."""
+    out = []
+    for c in range(lib().ttg_chol_batched_num_configs()):
+        w, g = ctypes.c_int(), ctypes.c_int()
+        _check(lib().ttg_chol_batched_config(c, ctypes.byref(w), ctypes.byref(g)), "batched_config")
+        out.append((w.value, g.value))
+    return out

@@ -60,6 +60,91 @@ struct Vec16<float> {
   static constexpr int n = 4;
 };
 
+// ----------------------------------------------------------------------------
+// One-wave factor and inverse of a 32 x 32 SPD block (POTRF and the batched
+// solver).
+// ----------------------------------------------------------------------------
+// Lane l's value of v, for a wave-uniform l.
+__device__ __forceinline__ double rdlane(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float rdlane(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// LDS written by some lanes of this wave becomes visible to its other lanes
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Cholesky factor and inverse of the 32 x 32 block in Ds (lower part read)
+// by ONE wave, without barriers.  Lane i (mod 32) holds row i of the block in
+// registers and factors it left-looking, column by column
+//   L(i,j) = (A(i,j) - sum_{k<j} L(i,k) L(j,k)) / L(j,j),
+// storing each finished column to Ds so that row j's entries are read back
+// as LDS broadcasts; the pivot comes from lane j by readlane.  Then lane c
+// computes column c of inv(L) by forward substitution
+//   X(r,c) = -(sum_{c<=k<r} L(r,k) X(k,c)) / L(r,r),  X(c,c) = 1 / L(c,c),
+// and stores it to Di.  Returns the 1-based column of the first pivot that
+// is not positive (NaN included), 0 if none; on failure Ds and Di are left
+// unspecified.  The upper part of Ds must be zero on entry.
+template <typename T>
+__device__ __forceinline__ int diag_chol_inv_wave(T (*Ds)[33], T (*Di)[33], int lane) {
+  constexpr int kB = 32;
+  // an opaque copy of the lane index: keeps the compiler from hoisting the
+  // 100+ lane masks below out of the caller's loop (and spilling them)
+  asm volatile("" : "+v"(lane));
+  const int i = lane & 31;
+  const bool writer = lane < kB;
+  int fail = 0;
+  T a[kB];
+#pragma unroll
+  for (int k = 0; k < kB; ++k) a[k] = (k <= i) ? Ds[i][k] : T(0);
+#pragma unroll
+  for (int j = 0; j < kB; ++j) {
+    T s0 = a[j], s1 = T(0);
+#pragma unroll
+    for (int k = 0; k < j; k += 2) {
+      s0 -= a[k] * Ds[j][k];
+      if (k + 1 < j) s1 -= a[k + 1] * Ds[j][k + 1];
+    }
+    const T s = s0 + s1;
+    const T djj = rdlane(s, j);
+    fail = (fail == 0 && !(djj > T(0))) ? j + 1 : fail;  // keep going: the caller discards the block
+    const T d = sqrt(djj);
+    if (i == j)
+      a[j] = d;
+    else if (i > j)
+      a[j] = s / d;
+    if (writer) Ds[i][j] = a[j];
+    wave_lds_sync();
+  }
+  if (fail) return fail;
+  const int c = lane & 31;
+  T x[kB];
+#pragma unroll
+  for (int r = 0; r < kB; ++r) {
+    T t0 = T(0), t1 = T(0);
+#pragma unroll
+    for (int k = 0; k < r; k += 2) {
+      t0 += Ds[r][k] * x[k];
+      if (k + 1 < r) t1 += Ds[r][k + 1] * x[k + 1];
+    }
+    const T lrr = Ds[r][r];
+    x[r] = (r < c) ? T(0) : (r == c ? T(1) / lrr : -(t0 + t1) / lrr);
+  }
+  if (writer) {
+#pragma unroll
+    for (int r = 0; r < kB; ++r) Di[r][c] = x[r];
+  }
+  return 0;
+}
+
 // Workgroup b of a grid of G is placed on XCD b % 8.  Return a logical index
 // such that consecutive logical indices run on the same XCD (and share its
 // L2), while the map stays a bijection on [0, G).
@@ -70,9 +155,12 @@ __device__ __forceinline__ int xcd_remap(int b, int G) {
 }
 
 // Make this workgroup's global stores visible to its own later global loads
-// (the POTRF and TRSM kernels re-read data they wrote).
+// (the POTRF, TRSM and batched kernels re-read data they wrote).  All waves
+// of a workgroup run on one CU and share its vector L1, so a workgroup-scope
+// fence (wait for the stores) is enough; an agent-scope fence would write
+// back this XCD's whole L2 every time.
 __device__ __forceinline__ void wg_sync_global() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
   __syncthreads();
 }
 

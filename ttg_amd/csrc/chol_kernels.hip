@@ -34,81 +34,6 @@ constexpr int kMaxTile = 512;                 // largest single-tile POTRF/TRSM
 // ---------------------------------------------------------------------------
 constexpr int kRowBlocks = kMaxTile / 16 / 4;  // 16-row blocks per wave (<= 8)
 
-// Lane l's value of v, for a wave-uniform l (two 32-bit readlanes).
-__device__ __forceinline__ double rdlane(double v, int l) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)b, l);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
-// Cholesky factor and inverse of the 32 x 32 block in Ds (lower part read)
-// by ONE wave, without barriers.  Lane i (mod 32) holds row i of the block in
-// registers and factors it left-looking, column by column
-//   L(i,j) = (A(i,j) - sum_{k<j} L(i,k) L(j,k)) / L(j,j),
-// storing each finished column to Ds so that row j's entries are read back
-// as LDS broadcasts; the pivot comes from lane j by readlane.  Then lane c
-// computes column c of inv(L) by forward substitution
-//   X(r,c) = -(sum_{c<=k<r} L(r,k) X(k,c)) / L(r,r),  X(c,c) = 1 / L(c,c),
-// and stores it to Di.  Returns the 1-based column of the first pivot that
-// is not positive (NaN included), 0 if none; on failure Ds and Di are left
-// unspecified.
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ int diag_chol_inv_wave(double (*Ds)[kPB + 1], double (*Di)[kPB + 1], int lane) {
-  // an opaque copy of the lane index: keeps the compiler from hoisting the
-  // 100+ lane masks below out of the caller's loop (and spilling them)
-  asm volatile("" : "+v"(lane));
-  const int i = lane & 31;
-  const bool writer = lane < kPB;
-  int fail = 0;
-  double a[kPB];
-#pragma unroll
-  for (int k = 0; k < kPB; ++k) a[k] = (k <= i) ? Ds[i][k] : 0.0;
-#pragma unroll
-  for (int j = 0; j < kPB; ++j) {
-    double s0 = a[j], s1 = 0.0;
-#pragma unroll
-    for (int k = 0; k < j; k += 2) {
-      s0 -= a[k] * Ds[j][k];
-      if (k + 1 < j) s1 -= a[k + 1] * Ds[j][k + 1];
-    }
-    const double s = s0 + s1;
-    const double djj = rdlane(s, j);
-    fail = (fail == 0 && !(djj > 0.0)) ? j + 1 : fail;  // keep going: the caller discards the block
-    const double d = sqrt(djj);
-    if (i == j)
-      a[j] = d;
-    else if (i > j)
-      a[j] = s / d;
-    if (writer) Ds[i][j] = a[j];
-    wave_lds_sync();
-  }
-  if (fail) return fail;
-  const int c = lane & 31;
-  double x[kPB];
-#pragma unroll
-  for (int r = 0; r < kPB; ++r) {
-    double t0 = 0.0, t1 = 0.0;
-#pragma unroll
-    for (int k = 0; k < r; k += 2) {
-      t0 += Ds[r][k] * x[k];
-      if (k + 1 < r) t1 += Ds[r][k + 1] * x[k + 1];
-    }
-    const double lrr = Ds[r][r];
-    x[r] = (r < c) ? 0.0 : (r == c ? 1.0 / lrr : -(t0 + t1) / lrr);
-  }
-  if (lane < kPB) {
-#pragma unroll
-    for (int r = 0; r < kPB; ++r) Di[r][c] = x[r];
-  }
-  return 0;
-}
-
 template <typename TS>
 __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOpT<TS> *__restrict__ ops, PotrfOpT<TS> one,
                                                     const int *__restrict__ d_abort) {
