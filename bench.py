@@ -44,11 +44,14 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--n", type=int, default=65536)
+    p.add_argument("--preset", choices=["metric", "low-precision", "fp64"], default="metric",
+                   help="metric: N=65536, 16 RHS, kappa~1e4 (the headline); low-precision: N=98304, 8 RHS, "
+                        "kappa~1e6; fp64: plain FP64 factor + solve at N=32768 (explicit flags override)")
+    p.add_argument("--n", type=int, default=None)
     p.add_argument("--nb", type=int, default=512)
-    p.add_argument("--nrhs", type=int, default=16)
-    p.add_argument("--kappa", type=float, default=1e4)
-    p.add_argument("--mode", choices=["f16x2", "fp32", "fp64"], default="f16x2",
+    p.add_argument("--nrhs", type=int, default=None)
+    p.add_argument("--kappa", type=float, default=None)
+    p.add_argument("--mode", choices=["f16x2", "fp32", "fp64", "f16", "bf16"], default=None,
                    help="f16x2: FP32 factor with split-FP16 MFMA trailing updates + FP64 refinement (the metric); "
                         "fp32: FP32 MFMA updates; fp64: plain FP64 factor")
     p.add_argument("--max-iter", type=int, default=30)
@@ -65,7 +68,26 @@ def parse():
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="only run a plain device copy of MIB MiB three times (calibrates the HBM byte counters) "
                         "and exit")
-    return p.parse_args()
+    a = p.parse_args()
+    preset = {"metric": (65536, 16, 1e4, "f16x2"), "low-precision": (98304, 8, 1e6, "f16x2"),
+              "fp64": (32768, 16, 1e4, "fp64")}[a.preset]
+    a.n = preset[0] if a.n is None else a.n
+    a.nrhs = preset[1] if a.nrhs is None else a.nrhs
+    a.kappa = preset[2] if a.kappa is None else a.kappa
+    a.mode = preset[3] if a.mode is None else a.mode
+    return a
+
+
+METRIC = "Cholesky solve TFLOP/s to 1e-12 backward err, N=65536: mixed-prec vs FP64"
+
+
+def metric_name(a):
+    if (a.n, a.nrhs, a.kappa, a.mode) == (65536, 16, 1e4, "f16x2"):
+        return METRIC
+    what = {"f16x2": "mixed-prec (split-FP16 updates) vs FP64", "fp32": "mixed-prec (FP32 updates) vs FP64",
+            "fp64": "plain FP64", "f16": "mixed-prec (FP16 updates) vs FP64",
+            "bf16": "mixed-prec (BF16 updates) vs FP64"}[a.mode]
+    return f"Cholesky solve TFLOP/s to {a.tol:g} backward err, N={a.n}, {a.nrhs} RHS, kappa~{a.kappa:g}: {what}"
 
 
 def copy_probe(mib):
@@ -184,7 +206,8 @@ def run_batched(a):
 def read_traffic(mode, N, NB, nrhs):
     """HBM bytes per launch of the dominant kernel from the committed PMC
     summary for this workload (profiles/pmc_<kernel>_N<N>.txt), or None."""
-    kern = {"f16x2": "update_f16x2", "fp32": "update_fp32", "fp64": "update_fp64"}[mode]
+    kern = {"f16x2": "update_f16x2", "fp32": "update_fp32", "fp64": "update_fp64", "f16": "update_f16",
+            "bf16": "update_bf16"}[mode]
     path = os.path.join(ROOT, "profiles", f"pmc_{kern}_N{N}_NB{NB}.txt")
     if not os.path.exists(path):
         return None
@@ -265,7 +288,7 @@ def main():
     anorm = float((-rs).max().item())
     del Aabs
 
-    prec = {"f16x2": C.PREC_F16X2, "fp32": C.PREC_FP32, "fp64": C.PREC_FP64}
+    prec = {"f16x2": C.PREC_F16X2, "fp32": C.PREC_FP32, "fp64": C.PREC_FP64, "f16": C.PREC_F16, "bf16": C.PREC_BF16}
     iters = {}
 
     def make_step(mode):
@@ -327,14 +350,17 @@ def main():
         ms_u, n_u, fl_u = C.profile_read(C.PROF_UPDATE)
         ms_t, n_t, fl_t = C.profile_read(C.PROF_TRSM)
         ms_p, n_p, fl_p = C.profile_read(C.PROF_POTRF)
-        peak = {"f16x2": FP16_PEAK / 3.0, "fp32": FP32_PEAK, "fp64": FP64_PEAK}[a.mode]
+        peak = {"f16x2": FP16_PEAK / 3.0, "fp32": FP32_PEAK, "fp64": FP64_PEAK, "f16": FP16_PEAK,
+                "bf16": FP16_PEAK}[a.mode]
         ach = fl_u / (ms_u * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": round(ach, 3), "peak": round(peak / 1e12, 1), "unit": "TFLOP/s",
                 "frac": round(ach / (peak / 1e12), 4), "traffic": read_traffic(a.mode, N, NB, nrhs),
                 "kernel": {"f16x2": "update_f16x2_kernel (batched SYRK+GEMM, 3 FP16 MFMA products per output; "
                                     "peak = FP16 dense / 3, flops counted once)",
                            "fp32": "update_kernel<float> (batched SYRK+GEMM trailing update)",
-                           "fp64": "update_kernel<double> (batched SYRK+GEMM trailing update)"}[a.mode],
+                           "fp64": "update_kernel<double> (batched SYRK+GEMM trailing update)",
+                           "f16": "update_f16x2_kernel<FP16, one product> (batched SYRK+GEMM)",
+                           "bf16": "update_f16x2_kernel<BF16, one product> (batched SYRK+GEMM)"}[a.mode],
                 "launches": n_u, "avg_launch_ms": round(ms_u / max(n_u, 1), 4),
                 "flops_per_launch": fl_u / max(n_u, 1),
                 "others": {"trsm_ms": round(ms_t, 2), "trsm_tflops": round(fl_t / max(ms_t, 1e-9) / 1e9, 2),
@@ -345,7 +371,7 @@ def main():
         cpu = cpu_baseline(a.cpu_n, 256, nrhs)
 
     out = {
-        "metric": "Cholesky solve TFLOP/s to 1e-12 backward err, N=65536: mixed-prec vs FP64",
+        "metric": metric_name(a),
         "value": round(value, 3),
         "unit": "TFLOP/s",
         "n_gpus": 1,
@@ -355,9 +381,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"f16x2": "f16x2+f32+f64", "fp32": "f32+f64", "fp64": "f64"}[a.mode],
+        "dtype": {"f16x2": "f16x2+f32+f64", "fp32": "f32+f64", "fp64": "f64", "f16": "f16+f32+f64",
+                  "bf16": "bf16+f32+f64"}[a.mode],
         "data": "synthetic: A = B B^T/N + alpha I, B(i,j) = k/127 with k uniform integer in [-127,127] "
-                "(seeded LCG), alpha for kappa ~ 1e4; 16 RHS uniform[-1,1]",
+                f"(seeded LCG), alpha for kappa ~ {a.kappa:g}; {nrhs} RHS uniform[-1,1]",
         "config": {"workload": f"SPD solve N={N} NB={NB} nrhs={nrhs} kappa~{a.kappa:g} (tiled POTRF graph)",
                    "N": N, "NB": NB, "nrhs": nrhs, "mode": a.mode, "parallelism": "single GPU"},
         "backward_error": berr,

@@ -141,9 +141,13 @@ int ttg_chol_dresidual_tiled(void *stream, int N, int NB, const double *tiles, i
 /* Working precisions of ttg_chol_dposv_mixed. */
 #define TTG_CHOL_PREC_FP64 0 /* FP64 factor (time-to-solution reference) */
 #define TTG_CHOL_PREC_FP32 1 /* FP32 TRSM + trailing updates, FP64 diagonal POTRF */
-#define TTG_CHOL_PREC_F16X2 2 /* FP32 TRSM; trailing updates on FP16 MFMA with each panel tile split into
-                                 hi + lo FP16 parts under a per-tile power-of-two scale (3 products,
+#define TTG_CHOL_PREC_F16X2 2 /* FP32 TRSM; trailing updates on FP16 MFMA with each panel row split into
+                                 hi + lo FP16 parts under a per-row power-of-two scale (3 products,
                                  FP32 accumulation); FP64 diagonal POTRF; NB % 128 == 0 */
+#define TTG_CHOL_PREC_F16 3   /* as F16X2 with the hi part only: one FP16 product per update (operands
+                                 rounded to FP16 after row scaling, FP32 accumulation); for systems
+                                 with kappa * 2^-11 well below 1 */
+#define TTG_CHOL_PREC_BF16 4  /* one BF16 product per update (FP32 accumulation); kappa * 2^-8 << 1 */
 
 /* Solve A X = B (A SPD, FP64 lower tiles, left unchanged; B, X N x nrhs,
  * nrhs <= 32) with a factorization in working precision `prec` followed by

@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "libttg_chol.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "ttg_chol.h")
 
 OK, EINVAL, EHIP, ENOTSUP, ENOMEM, ESTATE, ENOCONV = 0, -1, -2, -3, -4, -5, -6
-PREC_FP64, PREC_FP32, PREC_F16X2 = 0, 1, 2
+PREC_FP64, PREC_FP32, PREC_F16X2, PREC_F16, PREC_BF16 = 0, 1, 2, 3, 4
 F_LOOKAHEAD, F_GRAPH, F_PROFILE = 0x1, 0x2, 0x4
 PROF_UPDATE, PROF_TRSM, PROF_POTRF = 0, 1, 2
 F_DEFAULT = F_LOOKAHEAD | F_GRAPH

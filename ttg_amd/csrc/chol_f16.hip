@@ -23,6 +23,7 @@ typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
 // ---------------------------------------------------------------------------
 // split: one workgroup per 64-row stripe of a panel tile
 // ---------------------------------------------------------------------------
+template <int MODE>
 __global__ __launch_bounds__(256) void split_kernel(const SplitOp *__restrict__ ops, int NB) {
   __builtin_amdgcn_s_setprio(2);  // panel work: ahead of co-resident update waves
   const int nst = NB / 64;
@@ -71,11 +72,14 @@ __global__ __launch_bounds__(256) void split_kernel(const SplitOp *__restrict__ 
     for (int e = tid; e < 64 * 64; e += 256) {
       const int c = e & 63, r = e >> 6;  // row-major write, k (= c) contiguous
       const float v = blk[c][r] * rsc[r];
-      const _Float16 h = (_Float16)v;
-      const _Float16 l = (_Float16)(v - (float)h);
       const long o = (long)(r0 + r) * NB + bc * 64 + c;
-      reinterpret_cast<_Float16 *>(op.hi)[o] = h;
-      reinterpret_cast<_Float16 *>(op.lo)[o] = l;
+      if constexpr (MODE == kLowBF16) {
+        reinterpret_cast<__bf16 *>(op.hi)[o] = (__bf16)v;
+      } else {
+        const _Float16 h = (_Float16)v;
+        reinterpret_cast<_Float16 *>(op.hi)[o] = h;
+        if constexpr (MODE == kLowF16x2) reinterpret_cast<_Float16 *>(op.lo)[o] = (_Float16)(v - (float)h);
+      }
     }
   }
 }
@@ -91,43 +95,53 @@ constexpr int kF16BM = 128, kF16BK = 32;  // LDS rows of 32 halves = 4 chunks of
 // distinct 16-byte bank groups.
 __device__ __forceinline__ int swz(int row, int chunk) { return chunk ^ ((row >> 2) & 3); }
 
+template <int NP>  // parts per operand: 2 (hi, lo) or 1 (hi)
 struct F16Smem {
-  h8_t a[2][2][kF16BM][kF16BK / 8];  // [buf][hi/lo][row][chunk]
-  h8_t b[2][2][kF16BM][kF16BK / 8];
+  h8_t a[2][NP][kF16BM][kF16BK / 8];  // [buf][part][row][chunk]
+  h8_t b[2][NP][kF16BM][kF16BK / 8];
 };
 
-__device__ __forceinline__ void f16_load(const F16UpdOp &op, int i0, int j0, int k0, int tid, h8_t (&ra)[4],
-                                         h8_t (&rb)[4]) {
-  // 128 rows x 32 k x {hi, lo} per operand = 1024 16-byte chunks; 4 per thread
+template <int NP>
+__device__ __forceinline__ void f16_load(const F16UpdOp &op, int i0, int j0, int k0, int tid, h8_t (&ra)[2 * NP],
+                                         h8_t (&rb)[2 * NP]) {
+  // 128 rows x 32 k per part and operand = 512 16-byte chunks; 2 per thread
   const int v = tid & 3, r = tid >> 2;  // chunk in row, row 0..63
-  const h8_t *ahi = reinterpret_cast<const h8_t *>(op.ahi), *alo = reinterpret_cast<const h8_t *>(op.alo);
-  const h8_t *bhi = reinterpret_cast<const h8_t *>(op.bhi), *blo = reinterpret_cast<const h8_t *>(op.blo);
+  const h8_t *ap[2] = {reinterpret_cast<const h8_t *>(op.ahi), reinterpret_cast<const h8_t *>(op.alo)};
+  const h8_t *bp[2] = {reinterpret_cast<const h8_t *>(op.bhi), reinterpret_cast<const h8_t *>(op.blo)};
 #pragma unroll
   for (int p = 0; p < 2; ++p) {
     const int row = r + 64 * p;
     const long oa = ((long)(i0 + row) * op.ld + k0) / 8 + v;
     const long ob = ((long)(j0 + row) * op.ld + k0) / 8 + v;
-    ra[2 * p + 0] = ahi[oa];
-    ra[2 * p + 1] = alo[oa];
-    rb[2 * p + 0] = bhi[ob];
-    rb[2 * p + 1] = blo[ob];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+      ra[NP * p + q] = ap[q][oa];
+      rb[NP * p + q] = bp[q][ob];
+    }
   }
 }
 
-__device__ __forceinline__ void f16_store(F16Smem &sm, int buf, int tid, const h8_t (&ra)[4], const h8_t (&rb)[4]) {
+template <int NP>
+__device__ __forceinline__ void f16_store(F16Smem<NP> &sm, int buf, int tid, const h8_t (&ra)[2 * NP],
+                                          const h8_t (&rb)[2 * NP]) {
   const int v = tid & 3, r = tid >> 2;
 #pragma unroll
   for (int p = 0; p < 2; ++p) {
     const int row = r + 64 * p, c = swz(row, v);
-    sm.a[buf][0][row][c] = ra[2 * p + 0];
-    sm.a[buf][1][row][c] = ra[2 * p + 1];
-    sm.b[buf][0][row][c] = rb[2 * p + 0];
-    sm.b[buf][1][row][c] = rb[2 * p + 1];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+      sm.a[buf][q][row][c] = ra[NP * p + q];
+      sm.b[buf][q][row][c] = rb[NP * p + q];
+    }
   }
 }
 
+typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
+
+template <int MODE>
 __global__ __launch_bounds__(256) void update_f16x2_kernel(const F16UpdOp *__restrict__ ops, int bpo, int nbn,
                                                            const int *__restrict__ d_abort) {
+  constexpr int NP = (MODE == kLowF16x2) ? 2 : 1;
   if (d_abort && *d_abort) return;
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const int o = b / bpo, rr = b % bpo, bi = rr / nbn, bj = rr % nbn;
@@ -135,7 +149,7 @@ __global__ __launch_bounds__(256) void update_f16x2_kernel(const F16UpdOp *__res
   if (op.lower && bi < bj) return;
   const int i0 = bi * kF16BM, j0 = bj * kF16BM;
   if (i0 >= op.n || j0 >= op.n) return;
-  __shared__ __align__(16) F16Smem sm;
+  __shared__ __align__(16) F16Smem<NP> sm;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int lr = lane & 15, lk = lane >> 4;
@@ -145,37 +159,45 @@ __global__ __launch_bounds__(256) void update_f16x2_kernel(const F16UpdOp *__res
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
 
-  h8_t ra[4], rb[4];
+  h8_t ra[2 * NP], rb[2 * NP];
   const int nk = op.k / kF16BK;
-  f16_load(op, i0, j0, 0, tid, ra, rb);
-  f16_store(sm, 0, tid, ra, rb);
+  f16_load<NP>(op, i0, j0, 0, tid, ra, rb);
+  f16_store<NP>(sm, 0, tid, ra, rb);
   __syncthreads();
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
     const bool more = kt + 1 < nk;
-    if (more) f16_load(op, i0, j0, (kt + 1) * kF16BK, tid, ra, rb);
-    h8_t ah[4], al[4], bh[4], bl[4];
+    if (more) f16_load<NP>(op, i0, j0, (kt + 1) * kF16BK, tid, ra, rb);
+    h8_t ah[NP][4], bh[NP][4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int row = wm * 64 + i * 16 + lr, c = swz(row, lk);
-      ah[i] = sm.a[cur][0][row][c];
-      al[i] = sm.a[cur][1][row][c];
-    }
+    for (int q = 0; q < NP; ++q) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int row = wn * 64 + j * 16 + lr, c = swz(row, lk);
-      bh[j] = sm.b[cur][0][row][c];
-      bl[j] = sm.b[cur][1][row][c];
+      for (int i = 0; i < 4; ++i) {
+        const int row = wm * 64 + i * 16 + lr;
+        ah[q][i] = sm.a[cur][q][row][swz(row, lk)];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int row = wn * 64 + j * 16 + lr;
+        bh[q][j] = sm.b[cur][q][row][swz(row, lk)];
+      }
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        if constexpr (MODE == kLowF16x2) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[1][i], bh[0][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0][i], bh[1][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0][i], bh[0][j], acc[i][j], 0, 0, 0);
+        } else if constexpr (MODE == kLowF16) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0][i], bh[0][j], acc[i][j], 0, 0, 0);
+        } else {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, ah[0][i]),
+                                                               __builtin_bit_cast(b8_t, bh[0][j]), acc[i][j], 0, 0, 0);
+        }
       }
-    if (more) f16_store(sm, cur ^ 1, tid, ra, rb);
+    if (more) f16_store<NP>(sm, cur ^ 1, tid, ra, rb);
     __syncthreads();
   }
   const bool diag = op.lower && bi == bj;
@@ -199,18 +221,30 @@ __global__ __launch_bounds__(256) void update_f16x2_kernel(const F16UpdOp *__res
   }
 }
 
-hipError_t launch_split(hipStream_t s, const SplitOp *ops, int nops, int NB) {
+hipError_t launch_split(hipStream_t s, const SplitOp *ops, int nops, int NB, int mode) {
   if (nops <= 0) return hipSuccess;
-  hipLaunchKernelGGL(split_kernel, dim3(nops * (NB / 64)), dim3(256), 0, s, ops, NB);
+  const dim3 g(nops * (NB / 64)), b(256);
+  if (mode == kLowF16x2)
+    hipLaunchKernelGGL(split_kernel<kLowF16x2>, g, b, 0, s, ops, NB);
+  else if (mode == kLowF16)
+    hipLaunchKernelGGL(split_kernel<kLowF16>, g, b, 0, s, ops, NB);
+  else
+    hipLaunchKernelGGL(split_kernel<kLowBF16>, g, b, 0, s, ops, NB);
   return hipGetLastError();
 }
 
-hipError_t launch_update_f16x2(hipStream_t s, const F16UpdOp *ops, int nops, int NB, const int *d_abort) {
+hipError_t launch_update_f16x2(hipStream_t s, const F16UpdOp *ops, int nops, int NB, const int *d_abort, int mode) {
   if (nops <= 0) return hipSuccess;
   const int nbn = NB / kF16BM, bpo = nbn * nbn;
   const long grid = (long)nops * bpo;
   if (grid > 0x7fffffffL) return hipErrorInvalidConfiguration;
-  hipLaunchKernelGGL(update_f16x2_kernel, dim3((unsigned)grid), dim3(256), 0, s, ops, bpo, nbn, d_abort);
+  const dim3 g((unsigned)grid), b(256);
+  if (mode == kLowF16x2)
+    hipLaunchKernelGGL(update_f16x2_kernel<kLowF16x2>, g, b, 0, s, ops, bpo, nbn, d_abort);
+  else if (mode == kLowF16)
+    hipLaunchKernelGGL(update_f16x2_kernel<kLowF16>, g, b, 0, s, ops, bpo, nbn, d_abort);
+  else
+    hipLaunchKernelGGL(update_f16x2_kernel<kLowBF16>, g, b, 0, s, ops, bpo, nbn, d_abort);
   return hipGetLastError();
 }
 

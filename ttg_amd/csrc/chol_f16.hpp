@@ -26,7 +26,12 @@ struct F16UpdOp {
   int lower;
 };
 
-hipError_t launch_split(hipStream_t s, const SplitOp *ops, int nops, int NB);
-hipError_t launch_update_f16x2(hipStream_t s, const F16UpdOp *ops, int nops, int NB, const int *d_abort);
+// low-precision trailing-update modes
+constexpr int kLowF16x2 = 1;  // FP16 hi + lo split, three MFMA products (~FP32 accuracy)
+constexpr int kLowF16 = 2;    // one FP16 product (operands rounded to FP16 after row scaling)
+constexpr int kLowBF16 = 3;   // one BF16 product
+
+hipError_t launch_split(hipStream_t s, const SplitOp *ops, int nops, int NB, int mode);
+hipError_t launch_update_f16x2(hipStream_t s, const F16UpdOp *ops, int nops, int NB, const int *d_abort, int mode);
 
 }  // namespace ttgc

@@ -20,6 +20,7 @@
 #include <tuple>
 #include <vector>
 
+#include "chol_f16.hpp"
 #include "chol_internal.hpp"
 #include "chol_kernels.hpp"
 #include "chol_solve.hpp"
@@ -100,7 +101,7 @@ struct MixedPlanBase {
 template <typename T>
 struct MixedPlan : MixedPlanBase {
   int N = 0, NB = 0, nrhs = 0;
-  bool f16x2 = false;
+  int lowp = 0;  // 0: FP32 updates, else kLowF16x2 / kLowF16 / kLowBF16
   DevBuf W;      // working factor tiles (T)
   DevBuf R;      // residual, N x nrhs (FP64)
   DevBuf stats;  // anorm, berr[nrhs]
@@ -133,7 +134,7 @@ struct MixedPlan : MixedPlanBase {
     fplan->NB = NB;
     fplan->tiles = w();
     fplan->flags = TTG_CHOL_F_DEFAULT;
-    fplan->f16x2 = f16x2;
+    fplan->lowp = lowp;
     int rc = fplan->build();
     if (rc) return rc;
     splan = get_solve_plan<T>(N, NB, w(), nrhs, &rc);
@@ -232,7 +233,8 @@ extern "C" int ttg_chol_dposv_mixed(void *stream, int N, int NB, const double *A
     set_error("dposv_mixed: bad argument (N=%d NB=%d nrhs=%d ldb=%d ldx=%d)", N, NB, nrhs, ldb, ldx);
     return TTG_CHOL_EINVAL;
   }
-  if (prec != TTG_CHOL_PREC_FP64 && prec != TTG_CHOL_PREC_FP32 && prec != TTG_CHOL_PREC_F16X2) {
+  if (prec != TTG_CHOL_PREC_FP64 && prec != TTG_CHOL_PREC_FP32 && prec != TTG_CHOL_PREC_F16X2 &&
+      prec != TTG_CHOL_PREC_F16 && prec != TTG_CHOL_PREC_BF16) {
     set_error("dposv_mixed: precision %d not supported", prec);
     return TTG_CHOL_ENOTSUP;
   }
@@ -255,8 +257,8 @@ extern "C" int ttg_chol_dposv_mixed(void *stream, int N, int NB, const double *A
     }
     return static_cast<MixedPlan<double> *>(p.get())->run(s, A, B, ldb, X, ldx, tol, max_iter, d_info, iters, berr);
   }
-  if (prec == TTG_CHOL_PREC_F16X2 && NB % 128 != 0) {
-    set_error("dposv_mixed: split-FP16 updates need NB %% 128 == 0 (NB=%d)", NB);
+  if (prec >= TTG_CHOL_PREC_F16X2 && NB % 128 != 0) {
+    set_error("dposv_mixed: FP16/BF16 updates need NB %% 128 == 0 (NB=%d)", NB);
     return TTG_CHOL_ENOTSUP;
   }
   if (!p) {
@@ -265,7 +267,10 @@ extern "C" int ttg_chol_dposv_mixed(void *stream, int N, int NB, const double *A
     q->N = N;
     q->NB = NB;
     q->nrhs = nrhs;
-    q->f16x2 = prec == TTG_CHOL_PREC_F16X2;
+    q->lowp = prec == TTG_CHOL_PREC_F16X2 ? kLowF16x2
+              : prec == TTG_CHOL_PREC_F16 ? kLowF16
+              : prec == TTG_CHOL_PREC_BF16 ? kLowBF16
+                                            : 0;
     int rc = q->build();
     if (rc) {
       g_mixed.erase(key);

@@ -35,7 +35,7 @@ struct TiledPlanT {
   int N = 0, NB = 0, NT = 0;
   T *tiles = nullptr;
   unsigned flags = 0;
-  bool f16x2 = false;  // float tiles only: split-FP16 trailing updates
+  int lowp = 0;  // float tiles only: 0 FP32 updates, else kLowF16x2 / kLowF16 / kLowBF16
   DevBuf d_ops, d_dinv, d_info, d_pw;
   DevBuf d_panel, d_scale, d_f16ops;  // split-FP16 panels (two slots), scales, descriptors
   std::vector<int> split_off, split_cnt;
@@ -104,7 +104,7 @@ struct TiledPlanT {
     if (!trsm.empty()) memcpy(host.data() + o_trsm, trsm.data(), trsm.size() * sizeof(TrsmOp<T>));
     if (!upd.empty()) memcpy(host.data() + o_upd, upd.data(), upd.size() * sizeof(UpdOp<T>));
     TTGC_HIP(hipMemcpy(d_ops.ptr, host.data(), total, hipMemcpyHostToDevice));
-    if (f16x2) {
+    if (lowp) {
       int rc = build_f16(upd);
       if (rc) return rc;
     }
@@ -190,8 +190,8 @@ struct TiledPlanT {
     if (la) TTGC_HIP(hipEventRecord(ev_col, s_main));
     auto upd = [&](int off, int cnt, int nsyrk) -> int {
       const int slot = prof ? prof_begin(s_main) : -1;
-      if (f16x2)
-        TTGC_HIP(launch_update_f16x2(s_main, f16_ops() + off, cnt, NB, info));
+      if (lowp)
+        TTGC_HIP(launch_update_f16x2(s_main, f16_ops() + off, cnt, NB, info, lowp));
       else
         TTGC_HIP(launch_update<T>(s_main, upd_ops() + off, cnt, unone, NB, NB, m1, p1, fast, info));
       if (prof) prof_end(slot, s_main, PROF_UPDATE, (cnt - nsyrk) * 2.0 * b3 + nsyrk * (double)NB * NB * (NB + 1));
@@ -215,7 +215,7 @@ struct TiledPlanT {
         const int slot = prof ? prof_begin(sp) : -1;
         TTGC_HIP(launch_trsm<T>(sp, trsm_ops() + trsm_off[k], trsm_cnt[k], tnone, NB, info));
         if (prof) prof_end(slot, sp, PROF_TRSM, trsm_cnt[k] * b3);
-        if (f16x2) TTGC_HIP(launch_split(sp, split_ops() + split_off[k], split_cnt[k], NB));
+        if (lowp) TTGC_HIP(launch_split(sp, split_ops() + split_off[k], split_cnt[k], NB, lowp));
       }
       if (la) {
         TTGC_HIP(hipEventRecord(ev_panel, s_look));
