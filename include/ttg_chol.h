@@ -209,10 +209,11 @@ int ttg_chol_profile_read(int kind, double *ms_total, long long *launches, doubl
 int ttg_chol_profile_enable(int on);
 
 /* Lane-layout self test of the MFMA instructions the kernels use
- * (kind 0: FP64 16x16x4, kind 1: FP32 16x16x4, kind 2: FP16 16x16x32).
- * `out` (device; 64*64*4 doubles for kinds 0-1, 512*64*4 for kind 2)
- * receives, for each one-hot B position, the four outputs of every lane.  The kernels'
- * operand/result index maps are checked against it by the test suite.
+ * (kind 0: FP64 16x16x4, kind 1: FP32 16x16x4, kind 2: FP16 16x16x32,
+ * kind 3: FP16 32x32x16).  `out` (device; 64*64*4 doubles for kinds 0-1,
+ * 512*64*4 for kind 2, 512*64*16 for kind 3) receives, for each one-hot B
+ * position, the outputs of every lane.  The kernels' operand/result index
+ * maps are checked against it by the test suite.
  * (No counterpart in the reference.) */
 int ttg_chol_selftest_mfma_layout(void *stream, int kind, double *out);
 

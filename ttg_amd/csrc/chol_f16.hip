@@ -4,13 +4,17 @@
 // power of two s_i (max_j |A_mk(i,j)| s_i in [2^12, 2^13)) and split into two
 // FP16 parts
 //   hi = fp16(A s),  lo = fp16(A s - hi),
-// stored row-major (k contiguous).  The update of tile (m, n) is then
+// stored k-slab-major: [k / 32][row][k % 32], so that the 32-k slab of 128
+// consecutive rows a workgroup loads is one contiguous 8 KB block.  The update of tile (m, n) is then
 //   C -= diag(1/s_m) (hi_m hi_n^T + hi_m lo_n^T + lo_m hi_n^T) diag(1/s_n)
 // with three FP16 MFMAs (v_mfma_f32_16x16x32_f16) accumulating in FP32.  The
 // dropped lo lo^T term and the FP16 rounding of lo bound the representation
 // error of each operand entry by ~2^-22 of its row's largest entry, close to
 // FP32 (2^-24), at a third of the FP16 MFMA rate instead of the FP32 one.
 #include <hip/hip_runtime.h>
+
+#include <cstdlib>
+#include <cstring>
 
 #include "chol_device.hpp"
 #include "chol_f16.hpp"
@@ -72,7 +76,8 @@ __global__ __launch_bounds__(256) void split_kernel(const SplitOp *__restrict__ 
     for (int e = tid; e < 64 * 64; e += 256) {
       const int c = e & 63, r = e >> 6;  // row-major write, k (= c) contiguous
       const float v = blk[c][r] * rsc[r];
-      const long o = (long)(r0 + r) * NB + bc * 64 + c;
+      const int k = bc * 64 + c;
+      const long o = ((long)(k >> 5) * NB + (r0 + r)) * 32 + (k & 31);  // k-slab-major: [k / 32][row][k % 32]
       if constexpr (MODE == kLowBF16) {
         reinterpret_cast<__bf16 *>(op.hi)[o] = (__bf16)v;
       } else {
@@ -90,15 +95,16 @@ __global__ __launch_bounds__(256) void split_kernel(const SplitOp *__restrict__ 
 // ---------------------------------------------------------------------------
 constexpr int kF16BM = 128, kF16BK = 32;  // LDS rows of 32 halves = 4 chunks of 16 bytes
 
-// 16-byte chunk c of row r is stored at chunk c ^ ((r >> 2) & 3): the 16 lanes
-// of a quarter-wave that read rows r..r+15 at the same chunk then hit 16
-// distinct 16-byte bank groups.
-__device__ __forceinline__ int swz(int row, int chunk) { return chunk ^ ((row >> 2) & 3); }
+// 16-byte chunk c of row r (64-byte rows) is stored at chunk c ^ ((r >> 1) & 3):
+// the 8 lanes that one LDS cycle serves for a 16-byte read (8 consecutive
+// rows, same logical chunk) then cover 8 distinct 16-byte slots of the
+// 128-byte bank window (rows alternate halves, the XOR walks the chunks).
+__device__ __forceinline__ int swz(int row, int chunk) { return chunk ^ ((row >> 1) & 3); }
 
-template <int NP>  // parts per operand: 2 (hi, lo) or 1 (hi)
+template <int NP, int NST = 2>  // parts per operand: 2 (hi, lo) or 1 (hi); LDS stages
 struct F16Smem {
-  h8_t a[2][NP][kF16BM][kF16BK / 8];  // [buf][part][row][chunk]
-  h8_t b[2][NP][kF16BM][kF16BK / 8];
+  h8_t a[NST][NP][kF16BM][kF16BK / 8];  // [buf][part][row][chunk]
+  h8_t b[NST][NP][kF16BM][kF16BK / 8];
 };
 
 template <int NP>
@@ -111,8 +117,8 @@ __device__ __forceinline__ void f16_load(const F16UpdOp &op, int i0, int j0, int
 #pragma unroll
   for (int p = 0; p < 2; ++p) {
     const int row = r + 64 * p;
-    const long oa = ((long)(i0 + row) * op.ld + k0) / 8 + v;
-    const long ob = ((long)(j0 + row) * op.ld + k0) / 8 + v;
+    const long oa = ((long)(k0 >> 5) * op.ld + (i0 + row)) * 4 + v;  // [k / 32][row][k % 32]
+    const long ob = ((long)(k0 >> 5) * op.ld + (j0 + row)) * 4 + v;
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
       ra[NP * p + q] = ap[q][oa];
@@ -121,8 +127,8 @@ __device__ __forceinline__ void f16_load(const F16UpdOp &op, int i0, int j0, int
   }
 }
 
-template <int NP>
-__device__ __forceinline__ void f16_store(F16Smem<NP> &sm, int buf, int tid, const h8_t (&ra)[2 * NP],
+template <int NP, int NST>
+__device__ __forceinline__ void f16_store(F16Smem<NP, NST> &sm, int buf, int tid, const h8_t (&ra)[2 * NP],
                                           const h8_t (&rb)[2 * NP]) {
   const int v = tid & 3, r = tid >> 2;
 #pragma unroll
@@ -138,7 +144,7 @@ __device__ __forceinline__ void f16_store(F16Smem<NP> &sm, int buf, int tid, con
 
 typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
 
-template <int MODE>
+template <int MODE, int PF>  // PF: slabs of operands in flight (1 or 2)
 __global__ __launch_bounds__(256) void update_f16x2_kernel(const F16UpdOp *__restrict__ ops, int bpo, int nbn,
                                                            const int *__restrict__ d_abort) {
   constexpr int NP = (MODE == kLowF16x2) ? 2 : 1;
@@ -149,7 +155,7 @@ __global__ __launch_bounds__(256) void update_f16x2_kernel(const F16UpdOp *__res
   if (op.lower && bi < bj) return;
   const int i0 = bi * kF16BM, j0 = bj * kF16BM;
   if (i0 >= op.n || j0 >= op.n) return;
-  __shared__ __align__(16) F16Smem<NP> sm;
+  __shared__ __align__(16) F16Smem<NP, 2> sm;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int lr = lane & 15, lk = lane >> 4;
@@ -159,15 +165,7 @@ __global__ __launch_bounds__(256) void update_f16x2_kernel(const F16UpdOp *__res
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
 
-  h8_t ra[2 * NP], rb[2 * NP];
-  const int nk = op.k / kF16BK;
-  f16_load<NP>(op, i0, j0, 0, tid, ra, rb);
-  f16_store<NP>(sm, 0, tid, ra, rb);
-  __syncthreads();
-  for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    const bool more = kt + 1 < nk;
-    if (more) f16_load<NP>(op, i0, j0, (kt + 1) * kF16BK, tid, ra, rb);
+  auto compute = [&](int cur) {
     h8_t ah[NP][4], bh[NP][4];
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
@@ -197,8 +195,38 @@ __global__ __launch_bounds__(256) void update_f16x2_kernel(const F16UpdOp *__res
                                                                __builtin_bit_cast(b8_t, bh[0][j]), acc[i][j], 0, 0, 0);
         }
       }
-    if (more) f16_store<NP>(sm, cur ^ 1, tid, ra, rb);
+  };
+  const int nk = op.k / kF16BK;
+  h8_t ra0[2 * NP], rb0[2 * NP];
+  f16_load<NP>(op, i0, j0, 0, tid, ra0, rb0);
+  f16_store<NP, 2>(sm, 0, tid, ra0, rb0);
+  if constexpr (PF == 1) {
     __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+      const int cur = kt & 1;
+      const bool more = kt + 1 < nk;
+      if (more) f16_load<NP>(op, i0, j0, (kt + 1) * kF16BK, tid, ra0, rb0);
+      compute(cur);
+      if (more) f16_store<NP, 2>(sm, cur ^ 1, tid, ra0, rb0);
+      __syncthreads();
+    }
+  } else {
+    // two slabs in flight: register set 1 holds slab kt+1 while slab kt+2
+    // loads into set 0 (and the other way round on odd steps)
+    h8_t ra1[2 * NP], rb1[2 * NP];
+    if (nk > 1) f16_load<NP>(op, i0, j0, kF16BK, tid, ra1, rb1);
+    __syncthreads();
+    for (int kt = 0; kt < nk; kt += 2) {
+      if (kt + 2 < nk) f16_load<NP>(op, i0, j0, (kt + 2) * kF16BK, tid, ra0, rb0);
+      compute(0);
+      if (kt + 1 < nk) f16_store<NP, 2>(sm, 1, tid, ra1, rb1);
+      __syncthreads();
+      if (kt + 1 >= nk) break;
+      if (kt + 3 < nk) f16_load<NP>(op, i0, j0, (kt + 3) * kF16BK, tid, ra1, rb1);
+      compute(1);
+      if (kt + 2 < nk) f16_store<NP, 2>(sm, 0, tid, ra0, rb0);
+      __syncthreads();
+    }
   }
   const bool diag = op.lower && bi == bj;
   float rrow[4][4];  // row factors (powers of two: the products below are exact)
@@ -221,6 +249,154 @@ __global__ __launch_bounds__(256) void update_f16x2_kernel(const F16UpdOp *__res
   }
 }
 
+// ---------------------------------------------------------------------------
+// update, large block: 256 x 256 of C per workgroup, 4 waves of 128 x 128,
+// v_mfma_f32_32x32x16 (FP16 / BF16), k slabs of 32 staged in LDS (double
+// buffered, 64 KB per part and stage pair).  Twice the flops per byte of
+// L2 and LDS traffic of the 128 x 128 kernel.  The MFMA takes the C-column
+// side (tile n) as its A operand and the C-row side (tile m) as its B
+// operand, so each lane's results are C^T columns: 32 lanes store 32
+// consecutive rows of one C column.
+// ---------------------------------------------------------------------------
+constexpr int kBigBM = 256;
+typedef float f16v_t __attribute__((ext_vector_type(16)));
+
+template <int NP>
+struct BigSmem {
+  h8_t a[2][NP][kBigBM][kF16BK / 8];  // rows of tile m (C rows): [buf][part][row][chunk]
+  h8_t b[2][NP][kBigBM][kF16BK / 8];  // rows of tile n (C columns)
+};
+
+template <int MODE>
+__device__ __forceinline__ f16v_t mfma32(h8_t a, h8_t b, f16v_t c) {
+  if constexpr (MODE == kLowBF16)
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b8_t, a), __builtin_bit_cast(b8_t, b), c, 0,
+                                                    0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+}
+
+template <int MODE, int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void update_lp_big_kernel(const F16UpdOp *__restrict__ ops, int bpo,
+                                                                    int nbn, const int *__restrict__ d_abort) {
+  constexpr int NP = (MODE == kLowF16x2) ? 2 : 1;
+  constexpr int NTH = WAVES * 64;
+  constexpr int WN = WAVES / 2;          // waves along the C columns (2 along the rows)
+  constexpr int TN = kBigBM / WN / 32;   // 32-column tiles per wave
+  constexpr int RPT = kBigBM / (NTH / 4);  // rows loaded per thread and operand
+  if (d_abort && *d_abort) return;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int o = b / bpo, rr = b % bpo, bi = rr / nbn, bj = rr % nbn;
+  const F16UpdOp op = ops[o];
+  if (op.lower && bi < bj) return;
+  const int i0 = bi * kBigBM, j0 = bj * kBigBM;
+  if (i0 >= op.n || j0 >= op.n) return;
+  __shared__ __align__(16) BigSmem<NP> sm;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave / WN, wn = wave % WN;
+  const int l32 = lane & 31, lh = lane >> 5;
+  const h8_t *ap[2] = {reinterpret_cast<const h8_t *>(op.ahi), reinterpret_cast<const h8_t *>(op.alo)};
+  const h8_t *bp[2] = {reinterpret_cast<const h8_t *>(op.bhi), reinterpret_cast<const h8_t *>(op.blo)};
+
+  // global -> registers -> LDS: thread t moves chunk t & 3 of rows (t >> 2) + (NTH / 4) p
+  h8_t ra[RPT][NP], rb[RPT][NP];
+  const int cv = tid & 3, r0 = tid >> 2;
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int p = 0; p < RPT; ++p) {
+      const long oa = ((long)(k0 >> 5) * op.ld + (i0 + r0 + (NTH / 4) * p)) * 4 + cv;  // [k / 32][row][k % 32]
+      const long ob = ((long)(k0 >> 5) * op.ld + (j0 + r0 + (NTH / 4) * p)) * 4 + cv;
+#pragma unroll
+      for (int q = 0; q < NP; ++q) {
+        ra[p][q] = ap[q][oa];
+        rb[p][q] = bp[q][ob];
+      }
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < RPT; ++p) {
+      const int row = r0 + (NTH / 4) * p, c = swz(row, cv);
+#pragma unroll
+      for (int q = 0; q < NP; ++q) {
+        sm.a[buf][q][row][c] = ra[p][q];
+        sm.b[buf][q][row][c] = rb[p][q];
+      }
+    }
+  };
+
+  f16v_t acc[4][TN];  // [C-row tile tm][C-column tile tn]
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = f16v_t{};
+
+  // fragments of one 16-wide k step: fn (C-column side), fm (C-row side);
+  // two sets, so that the next step's LDS reads overlap this step's MFMAs
+  h8_t fn[2][TN][NP], fm[2][4][NP];
+  auto fread = [&](int f, int buf, int ks) {
+    const int ch = ks * 2 + lh;  // this lane's 8 k values
+#pragma unroll
+    for (int t = 0; t < TN; ++t) {
+      const int rn = wn * (TN * 32) + t * 32 + l32;
+#pragma unroll
+      for (int q = 0; q < NP; ++q) fn[f][t][q] = sm.b[buf][q][rn][swz(rn, ch)];
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int rm = wm * 128 + t * 32 + l32;
+#pragma unroll
+      for (int q = 0; q < NP; ++q) fm[f][t][q] = sm.a[buf][q][rm][swz(rm, ch)];
+    }
+  };
+  auto fmma = [&](int f) {
+#pragma unroll
+    for (int tm = 0; tm < 4; ++tm)
+#pragma unroll
+      for (int tn = 0; tn < TN; ++tn) {
+        if constexpr (NP == 2) {
+          acc[tm][tn] = mfma32<MODE>(fn[f][tn][1], fm[f][tm][0], acc[tm][tn]);
+          acc[tm][tn] = mfma32<MODE>(fn[f][tn][0], fm[f][tm][1], acc[tm][tn]);
+        }
+        acc[tm][tn] = mfma32<MODE>(fn[f][tn][0], fm[f][tm][0], acc[tm][tn]);
+      }
+  };
+  const int nk = op.k / kF16BK;  // slabs of 32 = two k steps
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  fread(0, 0, 0);
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < nk;
+    if (more) gload((kt + 1) * kF16BK);  // in flight across both k steps
+    fread(1, cur, 1);
+    fmma(0);
+    if (more) lstore(cur ^ 1);  // that buffer's last readers passed the previous barrier
+    fmma(1);
+    __syncthreads();
+    if (more) fread(0, cur ^ 1, 0);
+  }
+  // C(row, col) -= ra[row] rb[col] D(col, row): lane -> row, element -> column
+  const bool diag = op.lower && bi == bj;
+#pragma unroll
+  for (int tm = 0; tm < 4; ++tm) {
+    const int row = i0 + wm * 128 + tm * 32 + l32;
+    const float fr = op.ra[row];
+#pragma unroll
+    for (int tn = 0; tn < TN; ++tn) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int col = j0 + wn * (TN * 32) + tn * 32 + 8 * (r / 4) + 4 * lh + (r % 4);
+        if (!diag || row >= col) {
+          float *pc = op.C + row + (long)col * op.ldc;
+          *pc -= acc[tm][tn][r] * (fr * op.rb[col]);
+        }
+      }
+    }
+  }
+}
+
 hipError_t launch_split(hipStream_t s, const SplitOp *ops, int nops, int NB, int mode) {
   if (nops <= 0) return hipSuccess;
   const dim3 g(nops * (NB / 64)), b(256);
@@ -235,16 +411,53 @@ hipError_t launch_split(hipStream_t s, const SplitOp *ops, int nops, int NB, int
 
 hipError_t launch_update_f16x2(hipStream_t s, const F16UpdOp *ops, int nops, int NB, const int *d_abort, int mode) {
   if (nops <= 0) return hipSuccess;
+  // block shape: 128 x 128 (default: measured fastest) or 256 x 256 with 4 or 8 waves
+  const char *blk = getenv("TTG_CHOL_F16_BLOCK");
+  const int shape = !blk ? 0 : (!strcmp(blk, "256x4") ? 4 : (!strcmp(blk, "256x8") ? 8 : 0));
+  if (NB % kBigBM == 0 && shape != 0) {
+    const int nbn = NB / kBigBM, bpo = nbn * nbn;
+    const long grid = (long)nops * bpo;
+    if (grid > 0x7fffffffL) return hipErrorInvalidConfiguration;
+    const dim3 g((unsigned)grid);
+    if (shape == 4) {
+      if (mode == kLowF16x2)
+        hipLaunchKernelGGL((update_lp_big_kernel<kLowF16x2, 4>), g, dim3(256), 0, s, ops, bpo, nbn, d_abort);
+      else if (mode == kLowF16)
+        hipLaunchKernelGGL((update_lp_big_kernel<kLowF16, 4>), g, dim3(256), 0, s, ops, bpo, nbn, d_abort);
+      else
+        hipLaunchKernelGGL((update_lp_big_kernel<kLowBF16, 4>), g, dim3(256), 0, s, ops, bpo, nbn, d_abort);
+    } else {
+      if (mode == kLowF16x2)
+        hipLaunchKernelGGL((update_lp_big_kernel<kLowF16x2, 8>), g, dim3(512), 0, s, ops, bpo, nbn, d_abort);
+      else if (mode == kLowF16)
+        hipLaunchKernelGGL((update_lp_big_kernel<kLowF16, 8>), g, dim3(512), 0, s, ops, bpo, nbn, d_abort);
+      else
+        hipLaunchKernelGGL((update_lp_big_kernel<kLowBF16, 8>), g, dim3(512), 0, s, ops, bpo, nbn, d_abort);
+    }
+    return hipGetLastError();
+  }
   const int nbn = NB / kF16BM, bpo = nbn * nbn;
   const long grid = (long)nops * bpo;
   if (grid > 0x7fffffffL) return hipErrorInvalidConfiguration;
   const dim3 g((unsigned)grid), b(256);
+  // slabs of operands in flight: 1 (default: measured fastest, 2 workgroups per
+  // CU) or 2 (TTG_CHOL_F16_PREFETCH=2: more registers, one workgroup per CU)
+  const char *pf = getenv("TTG_CHOL_F16_PREFETCH");
+  if (!pf || strcmp(pf, "2")) {
+    if (mode == kLowF16x2)
+      hipLaunchKernelGGL((update_f16x2_kernel<kLowF16x2, 1>), g, b, 0, s, ops, bpo, nbn, d_abort);
+    else if (mode == kLowF16)
+      hipLaunchKernelGGL((update_f16x2_kernel<kLowF16, 1>), g, b, 0, s, ops, bpo, nbn, d_abort);
+    else
+      hipLaunchKernelGGL((update_f16x2_kernel<kLowBF16, 1>), g, b, 0, s, ops, bpo, nbn, d_abort);
+    return hipGetLastError();
+  }
   if (mode == kLowF16x2)
-    hipLaunchKernelGGL(update_f16x2_kernel<kLowF16x2>, g, b, 0, s, ops, bpo, nbn, d_abort);
+    hipLaunchKernelGGL((update_f16x2_kernel<kLowF16x2, 2>), g, b, 0, s, ops, bpo, nbn, d_abort);
   else if (mode == kLowF16)
-    hipLaunchKernelGGL(update_f16x2_kernel<kLowF16>, g, b, 0, s, ops, bpo, nbn, d_abort);
+    hipLaunchKernelGGL((update_f16x2_kernel<kLowF16, 2>), g, b, 0, s, ops, bpo, nbn, d_abort);
   else
-    hipLaunchKernelGGL(update_f16x2_kernel<kLowBF16>, g, b, 0, s, ops, bpo, nbn, d_abort);
+    hipLaunchKernelGGL((update_f16x2_kernel<kLowBF16, 2>), g, b, 0, s, ops, bpo, nbn, d_abort);
   return hipGetLastError();
 }
 

@@ -7,8 +7,9 @@
 namespace ttgc {
 
 // Split one FP32 panel tile (NB x NB, column-major, ld NB) into FP16 hi / lo
-// parts (row-major, NB x NB each), each row i scaled by its own power of two
-// s_i; rinv[i] = 1 / s_i (exact).
+// parts (NB x NB each, k-slab-major: element (i, k) at (k / 32) NB 32 + 32 i
+// + k % 32), each row i scaled by its own power of two s_i; rinv[i] = 1 / s_i
+// (exact).
 struct SplitOp {
   const float *src;
   uint16_t *hi, *lo;  // FP16 bit patterns
@@ -16,8 +17,8 @@ struct SplitOp {
 };
 
 // C (FP32 tile, column-major) -= diag(ra) (A_hi B_hi^T + A_hi B_lo^T + A_lo B_hi^T) diag(rb)
-// for n x n tiles with inner dimension k; A/B parts are row-major with row
-// stride ld (halves); ra / rb are the row factors of the A and B splits.
+// for n x n tiles with inner dimension k; A/B parts are k-slab-major with ld
+// rows per slab; ra / rb are the row factors of the A and B splits.
 struct F16UpdOp {
   const uint16_t *ahi, *alo, *bhi, *blo;
   const float *ra, *rb;

@@ -200,6 +200,25 @@ __global__ __launch_bounds__(64) void mfma_probe_f16_kernel(double *out) {
   for (int r = 0; r < 4; ++r) out[(q * 64 + l) * 4 + r] = (double)d[r];
 }
 
+// FP16 32x32x16 probe: A lane l element e holds l*8 + e + 1; block q sets B
+// lane q / 8, element q % 8 to 1.  out[(q * 64 + l) * 16 + r] = D element r
+// of lane l.
+__global__ __launch_bounds__(64) void mfma_probe_f16_32_kernel(double *out) {
+  typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+  typedef float f16v __attribute__((ext_vector_type(16)));
+  const int q = blockIdx.x, l = threadIdx.x;
+  h8 a, b;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    a[e] = (_Float16)(float)(l * 8 + e + 1);
+    b[e] = (_Float16)((l == q / 8 && e == q % 8) ? 1.0f : 0.0f);
+  }
+  f16v d = {};
+  d = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, d, 0, 0, 0);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) out[(q * 64 + l) * 16 + r] = (double)d[r];
+}
+
 template <typename T>
 hipError_t launch_prepare_t(hipStream_t s, int N, int NB, T *tiles, int *d_info) {
   const int nt = (N + NB - 1) / NB;
@@ -294,7 +313,7 @@ int ttg_chol_spd_tiled(void *stream, int N, int NB, double alpha, unsigned long 
 
 int ttg_chol_selftest_mfma_layout(void *stream, int kind, double *out) {
   clear_error();
-  if (!out || kind < 0 || kind > 2) {
+  if (!out || kind < 0 || kind > 3) {
     set_error("selftest_mfma_layout: bad argument (kind=%d)", kind);
     return TTG_CHOL_EINVAL;
   }
@@ -302,8 +321,10 @@ int ttg_chol_selftest_mfma_layout(void *stream, int kind, double *out) {
     hipLaunchKernelGGL(mfma_probe_kernel<double>, dim3(64), dim3(64), 0, as_stream(stream), out);
   else if (kind == 1)
     hipLaunchKernelGGL(mfma_probe_kernel<float>, dim3(64), dim3(64), 0, as_stream(stream), out);
-  else
+  else if (kind == 2)
     hipLaunchKernelGGL(mfma_probe_f16_kernel, dim3(512), dim3(64), 0, as_stream(stream), out);
+  else
+    hipLaunchKernelGGL(mfma_probe_f16_32_kernel, dim3(512), dim3(64), 0, as_stream(stream), out);
   TTGC_HIP(hipGetLastError());
   return TTG_CHOL_OK;
 }
