@@ -6,6 +6,7 @@
 #pragma once
 
 #include <cstddef>
+#include <cstring>
 #include <memory>
 #include <vector>
 
@@ -23,7 +24,13 @@ class TiledMatrix {
   TiledMatrix(std::size_t N, std::size_t NB, T *storage = nullptr)
       : N_(N), NB_(NB), nt_((N + NB - 1) / NB) {
     if (!storage) {
-      own_.reset(new T[num_tiles() * NB * NB]());
+      // pinned host memory: the tiles move between host and HBM at full
+      // link speed (the push-out of every final tile, the loads)
+      void *p = nullptr;
+      const std::size_t bytes = num_tiles() * NB * NB * sizeof(T);
+      if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) ttg::detail::fatal("hipHostMalloc");
+      std::memset(p, 0, bytes);
+      own_.reset(static_cast<T *>(p));
       storage = own_.get();
     }
     data_ = storage;
@@ -64,8 +71,11 @@ class TiledMatrix {
   }
 
  private:
+  struct HostFree {
+    void operator()(T *p) const { (void)hipHostFree(p); }
+  };
   std::size_t N_, NB_, nt_;
-  std::unique_ptr<T[]> own_;
+  std::unique_ptr<T, HostFree> own_;
   T *data_ = nullptr;
 };
 
