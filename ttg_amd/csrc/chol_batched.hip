@@ -105,7 +105,10 @@ __device__ int batched_factor(float *__restrict__ A, int lda, int n, BatchSmem<W
         int oa[TA];
 #pragma unroll
         for (int t = 0; t < TA; ++t) oa[t] = min(c0 + (wave + WAVES * t) * 16 + lr, n - 1);
-        float bv[2][2], av[2][TA];
+        // operand slots in flight: more when few row blocks leave little
+        // MFMA work per k-step to cover the load latency
+        constexpr int SL = TA >= 2 ? 2 : 4;
+        float bv[SL][2], av[SL][TA];
         auto load = [&](int sl, int k) {
           const gfloat *Ak = Ag + (long)k * lda;
           bv[sl][0] = Ak[ob0];
@@ -120,13 +123,14 @@ __device__ int batched_factor(float *__restrict__ A, int lda, int n, BatchSmem<W
             acc[t][1] = Mfma<float>::mma(av[sl][t], bv[sl][1], acc[t][1]);
           }
         };
-        load(0, lk);
-        load(1, 4 + lk);  // c0 is a multiple of 32
-        for (int k0 = 0; k0 < c0; k0 += 8) {
-          mma(0);
-          if (k0 + 8 < c0) load(0, k0 + 8 + lk);
-          mma(1);
-          if (k0 + 12 < c0) load(1, k0 + 12 + lk);
+#pragma unroll
+        for (int sl = 0; sl < SL; ++sl) load(sl, 4 * sl + lk);  // c0 is a multiple of 32 >= 4 SL
+        for (int k0 = 0; k0 < c0; k0 += 4 * SL) {
+#pragma unroll
+          for (int sl = 0; sl < SL; ++sl) {
+            mma(sl);
+            if (k0 + 4 * (SL + sl) < c0) load(sl, k0 + 4 * (SL + sl) + lk);
+          }
         }
       };
       static_assert(RB <= 8, "at most 8 row blocks per wave");

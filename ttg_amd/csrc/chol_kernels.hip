@@ -124,7 +124,10 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOpT<TS> *__restri
         int oa[TA];
 #pragma unroll
         for (int t = 0; t < TA; ++t) oa[t] = min(c0 + (wave + 4 * t) * 16 + lr, n - 1);
-        double bv[2][2], av[2][TA];
+        // operand slots in flight: more when few row blocks leave little
+        // MFMA work per k-step to cover the load latency
+        constexpr int SL = TA >= 2 ? 2 : 4;
+        double bv[SL][2], av[SL][TA];
         auto load = [&](int sl, int k) {
           const gdouble *Wk = Wg + (long)k * ldw;
           bv[sl][0] = Wk[ob0];
@@ -139,13 +142,14 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOpT<TS> *__restri
             acc[t][1] = Mfma<double>::mma(av[sl][t], bv[sl][1], acc[t][1]);
           }
         };
-        load(0, lk);
-        load(1, 4 + lk);  // c0 is a multiple of 32
-        for (int k0 = 0; k0 < c0; k0 += 8) {
-          mma(0);
-          if (k0 + 8 < c0) load(0, k0 + 8 + lk);
-          mma(1);
-          if (k0 + 12 < c0) load(1, k0 + 12 + lk);
+#pragma unroll
+        for (int sl = 0; sl < SL; ++sl) load(sl, 4 * sl + lk);  // c0 is a multiple of 32 >= 4 SL
+        for (int k0 = 0; k0 < c0; k0 += 4 * SL) {
+#pragma unroll
+          for (int sl = 0; sl < SL; ++sl) {
+            mma(sl);
+            if (k0 + 4 * (SL + sl) < c0) load(sl, k0 + 4 * (SL + sl) + lk);
+          }
         }
       };
       switch (tact) {
