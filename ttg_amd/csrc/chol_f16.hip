@@ -164,6 +164,15 @@ __global__ __launch_bounds__(256) void update_f16x2_kernel(const F16UpdOp *__res
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+  // this lane's C values (4 consecutive rows of one column per (i, j)),
+  // loaded now so that their latency hides behind the k loop
+  f4_t cv[4][4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const float *pc = op.C + (long)(j0 + wn * 64 + j * 16 + lr) * op.ldc + i0 + wm * 64 + 4 * lk;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) cv[i][j] = *reinterpret_cast<const f4_t *>(pc + i * 16);
+  }
 
   auto compute = [&](int cur) {
     h8_t ah[NP][4], bh[NP][4];
@@ -229,7 +238,97 @@ __global__ __launch_bounds__(256) void update_f16x2_kernel(const F16UpdOp *__res
     }
   }
   const bool diag = op.lower && bi == bj;
-  float rrow[4][4];  // row factors (powers of two: the products below are exact)
+  f4_t rrow[4];  // row factors (powers of two: the products below are exact)
+#pragma unroll
+  for (int i = 0; i < 4; ++i) rrow[i] = *reinterpret_cast<const f4_t *>(op.ra + i0 + wm * 64 + i * 16 + 4 * lk);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int col = j0 + wn * 64 + j * 16 + lr;
+    const float rc = op.rb[col];
+    float *pc = op.C + (long)col * op.ldc + i0 + wm * 64 + 4 * lk;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const f4_t v = cv[i][j] - acc[i][j] * (rrow[i] * rc);
+      const int row = i0 + wm * 64 + i * 16 + 4 * lk;  // rows row .. row + 3
+      if (!diag || row >= col) {
+        *reinterpret_cast<f4_t *>(pc + i * 16) = v;
+      } else if (row + 3 >= col) {  // the diagonal crosses these 4 rows: lower part only
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (row + r >= col) pc[i * 16 + r] = v[r];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// update, operands straight from L2: the 128 x 128 block and wave layout of
+// update_f16x2_kernel, but every wave loads its own MFMA fragments from the
+// k-slab-major panel (one contiguous 1 KB block per 16-row tile and part)
+// into registers, the next slab while this one is multiplied.  No LDS and no
+// barriers: the waves of a workgroup run independently.
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(256, 2) void update_lp_direct_kernel(const F16UpdOp *__restrict__ ops, int bpo, int nbn,
+                                                               const int *__restrict__ d_abort) {
+  constexpr int NP = (MODE == kLowF16x2) ? 2 : 1;
+  if (d_abort && *d_abort) return;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int o = b / bpo, rr = b % bpo, bi = rr / nbn, bj = rr % nbn;
+  const F16UpdOp op = ops[o];
+  if (op.lower && bi < bj) return;
+  const int i0 = bi * kF16BM, j0 = bj * kF16BM;
+  if (i0 >= op.n || j0 >= op.n) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int lr = lane & 15, lk = lane >> 4;
+  const h8_t *ap[2] = {reinterpret_cast<const h8_t *>(op.ahi), reinterpret_cast<const h8_t *>(op.alo)};
+  const h8_t *bp[2] = {reinterpret_cast<const h8_t *>(op.bhi), reinterpret_cast<const h8_t *>(op.blo)};
+  const long ld = op.ld;
+  f4_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+  h8_t fa[2][NP][4], fb[2][NP][4];
+  auto load = [&](int sl, int kt) {
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) fa[sl][q][i] = ap[q][((long)kt * ld + (i0 + wm * 64 + i * 16 + lr)) * 4 + lk];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) fb[sl][q][j] = bp[q][((long)kt * ld + (j0 + wn * 64 + j * 16 + lr)) * 4 + lk];
+    }
+  };
+  auto mma = [&](int sl) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if constexpr (MODE == kLowF16x2) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[sl][1][i], fb[sl][0][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[sl][0][i], fb[sl][1][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[sl][0][i], fb[sl][0][j], acc[i][j], 0, 0, 0);
+        } else if constexpr (MODE == kLowF16) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[sl][0][i], fb[sl][0][j], acc[i][j], 0, 0, 0);
+        } else {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, fa[sl][0][i]),
+                                                               __builtin_bit_cast(b8_t, fb[sl][0][j]), acc[i][j], 0, 0,
+                                                               0);
+        }
+      }
+  };
+  const int nk = op.k / kF16BK;
+  load(0, 0);
+  for (int kt = 0; kt < nk; kt += 2) {
+    if (kt + 1 < nk) load(1, kt + 1);
+    mma(0);
+    if (kt + 1 >= nk) break;
+    if (kt + 2 < nk) load(0, kt + 2);
+    mma(1);
+  }
+  const bool diag = op.lower && bi == bj;
+  float rrow[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -440,6 +539,16 @@ hipError_t launch_update_f16x2(hipStream_t s, const F16UpdOp *ops, int nops, int
   const long grid = (long)nops * bpo;
   if (grid > 0x7fffffffL) return hipErrorInvalidConfiguration;
   const dim3 g((unsigned)grid), b(256);
+  const char *dir = getenv("TTG_CHOL_F16_DIRECT");
+  if (dir && !strcmp(dir, "1")) {
+    if (mode == kLowF16x2)
+      hipLaunchKernelGGL(update_lp_direct_kernel<kLowF16x2>, g, b, 0, s, ops, bpo, nbn, d_abort);
+    else if (mode == kLowF16)
+      hipLaunchKernelGGL(update_lp_direct_kernel<kLowF16>, g, b, 0, s, ops, bpo, nbn, d_abort);
+    else
+      hipLaunchKernelGGL(update_lp_direct_kernel<kLowBF16>, g, b, 0, s, ops, bpo, nbn, d_abort);
+    return hipGetLastError();
+  }
   // slabs of operands in flight: 1 (default: measured fastest, 2 workgroups per
   // CU) or 2 (TTG_CHOL_F16_PREFETCH=2: more registers, one workgroup per CU)
   const char *pf = getenv("TTG_CHOL_F16_PREFETCH");
