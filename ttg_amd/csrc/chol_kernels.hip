@@ -396,14 +396,50 @@ __device__ __forceinline__ void update_body(const UpdOp<T> *__restrict__ ops, Up
   const int i0 = bi * BM, j0 = bj * BN;
   if (i0 >= op.m || j0 >= op.n) return;
   __shared__ __align__(16) typename Core::Smem sm;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int WN = SKINNY ? 4 : 2;
+  const int wm = wave / WN, wn = wave % WN;
+  // full float tiles: this lane's C values are loaded before the k loop, so
+  // that their latency hides behind it instead of trailing it (measured: a
+  // gain for FP32, a loss for FP64, whose 64 doubles per lane crowd the
+  // register file)
+  constexpr bool PRE = FAST && !SKINNY && sizeof(T) == 4;
+  T cpre[PRE ? Core::MI : 1][PRE ? Core::NI : 1][4];
+  if constexpr (PRE) {
+    if (beta != T(0)) {
+#pragma unroll
+      for (int i = 0; i < Core::MI; ++i)
+#pragma unroll
+        for (int j = 0; j < Core::NI; ++j) {
+          const T *pc = op.C + (long)(j0 + Core::col_of(wn, j, lane)) * op.ldc;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) cpre[i][j][r] = pc[i0 + Core::row_of(wm, i, lane, r)];
+        }
+    }
+  }
   Core core;
   core.zero();
   core.template run2<FAST && !SKINNY, FAST>(sm, op.A, op.lda, op.B, op.ldb, i0, j0, 0, op.k, op.m, op.n);
 
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  constexpr int WN = SKINNY ? 4 : 2;
-  const int wm = wave / WN, wn = wave % WN;
   const bool diag = op.lower && bi == bj;
+  if constexpr (PRE) {
+#pragma unroll
+    for (int i = 0; i < Core::MI; ++i)
+#pragma unroll
+      for (int j = 0; j < Core::NI; ++j) {
+        const int col = j0 + Core::col_of(wn, j, lane);
+        T *pc = op.C + (long)col * op.ldc;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = i0 + Core::row_of(wm, i, lane, r);
+          if (!diag || row >= col) {
+            const T v = alpha * core.acc[i][j][r];
+            pc[row] = (beta == T(0)) ? v : v + beta * cpre[i][j][r];
+          }
+        }
+      }
+    return;
+  }
   const bool full = FAST && !SKINNY && !diag;
 #pragma unroll
   for (int i = 0; i < Core::MI; ++i)
