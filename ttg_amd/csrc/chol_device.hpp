@@ -82,13 +82,27 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// 1/sqrt(x) for x > 0: the hardware estimate refined by one Newton step
+// (relative error ~ the estimate's squared, below 1 ulp of double/float).
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x * y;
+  return y + y * (0.5 - h * y);
+}
+__device__ __forceinline__ float rsqrt_nr(float x) {
+  float y = __builtin_amdgcn_rsqf(x);
+  const float h = 0.5f * x * y;
+  return y + y * (0.5f - h * y);
+}
+
 // Cholesky factor and inverse of the 32 x 32 block in Ds (lower part read)
 // by ONE wave, without barriers.  Lane i (mod 32) holds row i of the block in
 // registers and factors it left-looking, column by column
 //   L(i,j) = (A(i,j) - sum_{k<j} L(i,k) L(j,k)) / L(j,j),
 // storing each finished column to Ds so that row j's entries are read back
-// as LDS broadcasts; the pivot comes from lane j by readlane.  Then lane c
-// computes column c of inv(L) by forward substitution
+// as LDS broadcasts; the pivot comes from lane j by readlane, and 1 / L(j,j)
+// from a refined reciprocal square root (no divisions on the chain).  Then
+// lane c computes column c of inv(L) by forward substitution
 //   X(r,c) = -(sum_{c<=k<r} L(r,k) X(k,c)) / L(r,r),  X(c,c) = 1 / L(c,c),
 // and stores it to Di.  Returns the 1-based column of the first pivot that
 // is not positive (NaN included), 0 if none; on failure Ds and Di are left
@@ -105,6 +119,7 @@ __device__ __forceinline__ int diag_chol_inv_wave(T (*Ds)[33], T (*Di)[33], int 
   T a[kB];
 #pragma unroll
   for (int k = 0; k < kB; ++k) a[k] = (k <= i) ? Ds[i][k] : T(0);
+  T rinv = T(0);  // lane i: 1 / L(i,i)
 #pragma unroll
   for (int j = 0; j < kB; ++j) {
     T s0 = a[j], s1 = T(0);
@@ -116,31 +131,33 @@ __device__ __forceinline__ int diag_chol_inv_wave(T (*Ds)[33], T (*Di)[33], int 
     const T s = s0 + s1;
     const T djj = rdlane(s, j);
     fail = (fail == 0 && !(djj > T(0))) ? j + 1 : fail;  // keep going: the caller discards the block
-    const T d = sqrt(djj);
-    if (i == j)
-      a[j] = d;
-    else if (i > j)
-      a[j] = s / d;
+    const T r = rsqrt_nr(djj);
+    if (i == j) {
+      a[j] = djj * r;  // sqrt(djj)
+      rinv = r;
+    } else if (i > j) {
+      a[j] = s * r;
+    }
     if (writer) Ds[i][j] = a[j];
     wave_lds_sync();
   }
   if (fail) return fail;
-  const int c = lane & 31;
+  const int c = i;
   T x[kB];
 #pragma unroll
-  for (int r = 0; r < kB; ++r) {
+  for (int rr = 0; rr < kB; ++rr) {
     T t0 = T(0), t1 = T(0);
 #pragma unroll
-    for (int k = 0; k < r; k += 2) {
-      t0 += Ds[r][k] * x[k];
-      if (k + 1 < r) t1 += Ds[r][k + 1] * x[k + 1];
+    for (int k = 0; k < rr; k += 2) {
+      t0 += Ds[rr][k] * x[k];
+      if (k + 1 < rr) t1 += Ds[rr][k + 1] * x[k + 1];
     }
-    const T lrr = Ds[r][r];
-    x[r] = (r < c) ? T(0) : (r == c ? T(1) / lrr : -(t0 + t1) / lrr);
+    const T rl = rdlane(rinv, rr);
+    x[rr] = (rr < c) ? T(0) : (rr == c ? rl : -(t0 + t1) * rl);
   }
   if (writer) {
 #pragma unroll
-    for (int r = 0; r < kB; ++r) Di[r][c] = x[r];
+    for (int rr = 0; rr < kB; ++rr) Di[rr][c] = x[rr];
   }
   return 0;
 }
