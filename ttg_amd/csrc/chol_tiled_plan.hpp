@@ -12,10 +12,19 @@
 // updA(k) is done, overlapping updB(k).  Per tile the update order over k is
 // the one the reference graph fixes (GEMM(m,n,0), GEMM(m,n,1), ...).  The
 // launch sequence is captured once into a hipGraph and replayed.
+//
+// Reserved CUs (TTG_CHOL_POTRF_CUS, default 8): the single-workgroup POTRF
+// gets a stream of its own whose queue may only use the first R CUs of the
+// device mask, and the update stream the others.  KFD interleaves the mask
+// bits over the XCDs (bit b -> XCD b % 8), so R = 8 is one CU per XCD.
+// POTRF(k+1) then runs beside updB(k) at full speed instead of after it.
+// The masks live on the HSA queues, which a replayed hipGraph does not use,
+// so this schedule is launched directly, without graph capture.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -42,7 +51,7 @@ struct TiledPlanT {
   size_t o_split = 0, o_f16 = 0;
   size_t o_potrf = 0, o_trsm = 0, o_upd = 0;
   std::vector<int> trsm_off, trsm_cnt, updA_off, updA_cnt, updB_off, updB_cnt;
-  hipStream_t s_main = nullptr, s_look = nullptr;
+  hipStream_t s_main = nullptr, s_look = nullptr, s_diag = nullptr;  // s_diag: POTRF on reserved CUs
   hipEvent_t ev_in = nullptr, ev_out = nullptr, ev_panel = nullptr, ev_col = nullptr, ev_diag = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -56,6 +65,7 @@ struct TiledPlanT {
       if (e) (void)hipEventDestroy(e);
     if (s_main) (void)hipStreamDestroy(s_main);
     if (s_look) (void)hipStreamDestroy(s_look);
+    if (s_diag) (void)hipStreamDestroy(s_diag);
     for (DevBuf *b : {&d_ops, &d_dinv, &d_info, &d_pw, &d_panel, &d_scale, &d_f16ops})
       if (b->ptr) (void)hipFree(b->ptr);
   }
@@ -108,11 +118,36 @@ struct TiledPlanT {
       int rc = build_f16(upd);
       if (rc) return rc;
     }
-    TTGC_HIP(hipStreamCreateWithFlags(&s_main, hipStreamNonBlocking));
+    if (flags & TTG_CHOL_F_LOOKAHEAD) reserve_cus();
+    if (!s_main) TTGC_HIP(hipStreamCreateWithFlags(&s_main, hipStreamNonBlocking));
     TTGC_HIP(hipStreamCreateWithFlags(&s_look, hipStreamNonBlocking));
     for (hipEvent_t *e : {&ev_in, &ev_out, &ev_panel, &ev_col, &ev_diag})
       TTGC_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
     return TTG_CHOL_OK;
+  }
+
+  // POTRF stream on CUs 0 .. R-1 of the mask, update stream on the rest; on
+  // any failure both stay unset and the plain two-stream schedule is used.
+  void reserve_cus() {
+    const char *e = getenv("TTG_CHOL_POTRF_CUS");
+    const int want = e ? atoi(e) : 8;
+    int dev = 0, ncu = 0;
+    if (want <= 0 || hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || want * 4 > ncu)
+      return;
+    const int words = (ncu + 31) / 32;
+    std::vector<uint32_t> mdiag(words, 0u), mrest(words, 0u);
+    for (int c = 0; c < ncu; ++c) (c < want ? mdiag : mrest)[c / 32] |= 1u << (c % 32);
+    hipStream_t d = nullptr, m = nullptr;
+    if (hipExtStreamCreateWithCUMask(&d, words, mdiag.data()) != hipSuccess ||
+        hipExtStreamCreateWithCUMask(&m, words, mrest.data()) != hipSuccess) {
+      if (d) (void)hipStreamDestroy(d);
+      if (m) (void)hipStreamDestroy(m);
+      (void)hipGetLastError();
+      return;
+    }
+    s_diag = d;
+    s_main = m;
   }
 
   // Split-FP16 descriptors: panel k lives in slot k % 2 (tile m at position
@@ -186,7 +221,9 @@ struct TiledPlanT {
     PotrfOpT<T> pnone{};
     TTGC_HIP(launch_prepare_t<T>(s_main, N, NB, tiles, info));
     const bool la = (flags & TTG_CHOL_F_LOOKAHEAD) != 0;
+    const bool rsv = la && s_diag;  // POTRF on its reserved CUs
     hipStream_t sp = la ? s_look : s_main;
+    hipStream_t sd = rsv ? s_diag : sp;
     if (la) TTGC_HIP(hipEventRecord(ev_col, s_main));
     auto upd = [&](int off, int cnt, int nsyrk) -> int {
       const int slot = prof ? prof_begin(s_main) : -1;
@@ -198,17 +235,21 @@ struct TiledPlanT {
       return TTG_CHOL_OK;
     };
     auto potrf = [&](int k) -> int {
-      const int slot = prof ? prof_begin(sp) : -1;
-      TTGC_HIP(launch_potrf<T>(sp, potrf_ops() + k, 1, pnone, info));
-      if (prof) prof_end(slot, sp, PROF_POTRF, b3 / 3.0);
+      const int slot = prof ? prof_begin(sd) : -1;
+      TTGC_HIP(launch_potrf<T>(sd, potrf_ops() + k, 1, pnone, info));
+      if (prof) prof_end(slot, sd, PROF_POTRF, b3 / 3.0);
+      if (rsv) {  // TRSM(k) follows on the panel stream
+        TTGC_HIP(hipEventRecord(ev_diag, s_diag));
+        TTGC_HIP(hipStreamWaitEvent(s_look, ev_diag, 0));
+      }
       return TTG_CHOL_OK;
     };
     // Step k: TRSM(k) (+ split) on the panel stream, updA(k) (column k+1),
-    // then POTRF(k+1) -- which needs only updA(k) -- BEFORE updB(k) is
-    // released: the single-workgroup POTRF then finds free CUs instead of
-    // waiting behind thousands of update workgroups, and TRSM(k+1) overlaps
-    // updB(k).
-    if (la) TTGC_HIP(hipStreamWaitEvent(s_look, ev_col, 0));
+    // then POTRF(k+1), which needs only updA(k).  On shared CUs updB(k) is
+    // released only after POTRF(k+1): the single-workgroup POTRF then finds
+    // free CUs instead of waiting behind thousands of update workgroups.  On
+    // reserved CUs the two run side by side.  TRSM(k+1) overlaps updB(k).
+    if (la) TTGC_HIP(hipStreamWaitEvent(sd, ev_col, 0));
     if (int rc = potrf(0)) return rc;
     for (int k = 0; k < NT; ++k) {
       if (trsm_cnt[k]) {
@@ -228,10 +269,10 @@ struct TiledPlanT {
       if (k + 1 < NT) {
         if (la) {
           TTGC_HIP(hipEventRecord(ev_col, s_main));
-          TTGC_HIP(hipStreamWaitEvent(s_look, ev_col, 0));
+          TTGC_HIP(hipStreamWaitEvent(sd, ev_col, 0));
         }
         if (int rc = potrf(k + 1)) return rc;
-        if (la && updB_cnt[k]) {
+        if (la && !rsv && updB_cnt[k]) {
           TTGC_HIP(hipEventRecord(ev_diag, s_look));
           TTGC_HIP(hipStreamWaitEvent(s_main, ev_diag, 0));
         }
@@ -252,7 +293,7 @@ struct TiledPlanT {
     int rc = bind_info(info);
     if (rc) return rc;
     const bool prof = (flags & TTG_CHOL_F_PROFILE) != 0 || prof_enabled();
-    if ((flags & TTG_CHOL_F_GRAPH) && !prof) {
+    if ((flags & TTG_CHOL_F_GRAPH) && !prof && !s_diag) {
       if (!gexec || bound_info != info) {
         if (gexec) {
           TTGC_HIP(hipGraphExecDestroy(gexec));
