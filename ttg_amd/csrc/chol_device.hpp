@@ -14,6 +14,21 @@ namespace ttgc {
 constexpr int kWave = 64;
 constexpr int kXcds = 8;  // MI355X: 8 XCDs, workgroups are dealt round-robin
 
+// Global-memory views of generic pointers.  A load through a generic pointer
+// compiles to a FLAT instruction, which counts on both vmcnt and lgkmcnt, so
+// every wait for LDS reads (lgkmcnt) also waits for the global loads in
+// flight -- a prefetch issued before the LDS reads of a k-slab then completes
+// before the slab's first MFMA.  Through these views the accesses are GLOBAL
+// instructions, counted on vmcnt only.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gld(const T *p) {
+  return (const __attribute__((address_space(1))) T *)p;
+}
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gst(T *p) {
+  return (__attribute__((address_space(1))) T *)p;
+}
+
 typedef double d2_t __attribute__((ext_vector_type(2)));
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
@@ -211,11 +226,11 @@ struct StageLoader {
     for (int p = 0; p < PASSES; ++p) {
       const int col = col0 + c + p * CPP;
       if constexpr (FAST) {
-        r[p] = *reinterpret_cast<const vec_t *>(base + row + (long)col * ld);
+        r[p] = *gld(reinterpret_cast<const vec_t *>(base + row + (long)col * ld));
       } else {
 #pragma unroll
         for (int e = 0; e < V; ++e)
-          r[p][e] = (row + e < rlim && col < clim) ? base[row + e + (long)col * ld] : T(0);
+          r[p][e] = (row + e < rlim && col < clim) ? gld(base)[row + e + (long)col * ld] : T(0);
       }
     }
   }
@@ -255,11 +270,11 @@ struct StageLoaderT {
     for (int p = 0; p < PASSES; ++p) {
       const int row = row0 + j + p * RPP;
       if constexpr (FAST) {
-        r[p] = *reinterpret_cast<const vec_t *>(base + k + (long)row * ld);
+        r[p] = *gld(reinterpret_cast<const vec_t *>(base + k + (long)row * ld));
       } else {
 #pragma unroll
         for (int e = 0; e < V; ++e)
-          r[p][e] = (row < rlim && k + e < clim) ? base[k + e + (long)row * ld] : T(0);
+          r[p][e] = (row < rlim && k + e < clim) ? gld(base)[k + e + (long)row * ld] : T(0);
       }
     }
   }

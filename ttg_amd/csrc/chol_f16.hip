@@ -13,9 +13,6 @@
 // FP32 (2^-24), at a third of the FP16 MFMA rate instead of the FP32 one.
 #include <hip/hip_runtime.h>
 
-#include <cstdlib>
-#include <cstring>
-
 #include "chol_device.hpp"
 #include "chol_f16.hpp"
 #include "chol_internal.hpp"
@@ -43,11 +40,11 @@ __global__ __launch_bounds__(256) void split_kernel(const SplitOp *__restrict__ 
     const int v = tid & 15, c = tid >> 4;
     float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f;
     for (int col = c; col < NB; col += 16) {
-      const float4 x = *reinterpret_cast<const float4 *>(op.src + r0 + 4 * v + (long)col * NB);
-      m0 = fmaxf(m0, fabsf(x.x));
-      m1 = fmaxf(m1, fabsf(x.y));
-      m2 = fmaxf(m2, fabsf(x.z));
-      m3 = fmaxf(m3, fabsf(x.w));
+      const f4_t x = *gld(reinterpret_cast<const f4_t *>(op.src + r0 + 4 * v + (long)col * NB));
+      m0 = fmaxf(m0, fabsf(x[0]));
+      m1 = fmaxf(m1, fabsf(x[1]));
+      m2 = fmaxf(m2, fabsf(x[2]));
+      m3 = fmaxf(m3, fabsf(x[3]));
     }
     red[c][4 * v + 0] = m0;
     red[c][4 * v + 1] = m1;
@@ -63,14 +60,14 @@ __global__ __launch_bounds__(256) void split_kernel(const SplitOp *__restrict__ 
     int e = 13;
     if (mx > 0.f && isfinite(mx)) frexpf(mx, &e);
     rsc[tid] = ldexpf(1.f, 13 - e);
-    op.rinv[r0 + tid] = ldexpf(1.f, e - 13);
+    gst(op.rinv)[r0 + tid] = ldexpf(1.f, e - 13);
   }
   // 2. 64 x 64 blocks: transpose through LDS, split, write row-major
   for (int bc = 0; bc < NB / 64; ++bc) {
     __syncthreads();
     for (int e = tid; e < 64 * 64; e += 256) {
       const int r = e & 63, c = e >> 6;  // coalesced column-major read
-      blk[c][r] = op.src[(r0 + r) + (long)(bc * 64 + c) * NB];
+      blk[c][r] = gld(op.src)[(r0 + r) + (long)(bc * 64 + c) * NB];
     }
     __syncthreads();
     for (int e = tid; e < 64 * 64; e += 256) {
@@ -79,11 +76,11 @@ __global__ __launch_bounds__(256) void split_kernel(const SplitOp *__restrict__ 
       const int k = bc * 64 + c;
       const long o = ((long)(k >> 5) * NB + (r0 + r)) * 32 + (k & 31);  // k-slab-major: [k / 32][row][k % 32]
       if constexpr (MODE == kLowBF16) {
-        reinterpret_cast<__bf16 *>(op.hi)[o] = (__bf16)v;
+        gst(reinterpret_cast<__bf16 *>(op.hi))[o] = (__bf16)v;
       } else {
         const _Float16 h = (_Float16)v;
-        reinterpret_cast<_Float16 *>(op.hi)[o] = h;
-        if constexpr (MODE == kLowF16x2) reinterpret_cast<_Float16 *>(op.lo)[o] = (_Float16)(v - (float)h);
+        gst(reinterpret_cast<_Float16 *>(op.hi))[o] = h;
+        if constexpr (MODE == kLowF16x2) gst(reinterpret_cast<_Float16 *>(op.lo))[o] = (_Float16)(v - (float)h);
       }
     }
   }
@@ -121,8 +118,8 @@ __device__ __forceinline__ void f16_load(const F16UpdOp &op, int i0, int j0, int
     const long ob = ((long)(k0 >> 5) * op.ld + (j0 + row)) * 4 + v;
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
-      ra[NP * p + q] = ap[q][oa];
-      rb[NP * p + q] = bp[q][ob];
+      ra[NP * p + q] = gld(ap[q])[oa];
+      rb[NP * p + q] = gld(bp[q])[ob];
     }
   }
 }
@@ -144,7 +141,7 @@ __device__ __forceinline__ void f16_store(F16Smem<NP, NST> &sm, int buf, int tid
 
 typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
 
-template <int MODE, int PF>  // PF: slabs of operands in flight (1 or 2)
+template <int MODE>
 __global__ __launch_bounds__(256) void update_f16x2_kernel(const F16UpdOp *__restrict__ ops, int bpo, int nbn,
                                                            const int *__restrict__ d_abort) {
   constexpr int NP = (MODE == kLowF16x2) ? 2 : 1;
@@ -164,14 +161,17 @@ __global__ __launch_bounds__(256) void update_f16x2_kernel(const F16UpdOp *__res
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
-  // this lane's C values (4 consecutive rows of one column per (i, j)),
-  // loaded now so that their latency hides behind the k loop
+  // the first slab's operands go out first, then this lane's C values (4
+  // consecutive rows of one column per (i, j)), whose latency then hides
+  // behind the first slabs instead of delaying them
+  h8_t ra0[2 * NP], rb0[2 * NP];
+  f16_load<NP>(op, i0, j0, 0, tid, ra0, rb0);
   f4_t cv[4][4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const float *pc = op.C + (long)(j0 + wn * 64 + j * 16 + lr) * op.ldc + i0 + wm * 64 + 4 * lk;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) cv[i][j] = *reinterpret_cast<const f4_t *>(pc + i * 16);
+    for (int i = 0; i < 4; ++i) cv[i][j] = *gld(reinterpret_cast<const f4_t *>(pc + i * 16));
   }
 
   auto compute = [&](int cur) {
@@ -206,52 +206,31 @@ __global__ __launch_bounds__(256) void update_f16x2_kernel(const F16UpdOp *__res
       }
   };
   const int nk = op.k / kF16BK;
-  h8_t ra0[2 * NP], rb0[2 * NP];
-  f16_load<NP>(op, i0, j0, 0, tid, ra0, rb0);
   f16_store<NP, 2>(sm, 0, tid, ra0, rb0);
-  if constexpr (PF == 1) {
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < nk;
+    if (more) f16_load<NP>(op, i0, j0, (kt + 1) * kF16BK, tid, ra0, rb0);
+    compute(cur);
+    if (more) f16_store<NP, 2>(sm, cur ^ 1, tid, ra0, rb0);
     __syncthreads();
-    for (int kt = 0; kt < nk; ++kt) {
-      const int cur = kt & 1;
-      const bool more = kt + 1 < nk;
-      if (more) f16_load<NP>(op, i0, j0, (kt + 1) * kF16BK, tid, ra0, rb0);
-      compute(cur);
-      if (more) f16_store<NP, 2>(sm, cur ^ 1, tid, ra0, rb0);
-      __syncthreads();
-    }
-  } else {
-    // two slabs in flight: register set 1 holds slab kt+1 while slab kt+2
-    // loads into set 0 (and the other way round on odd steps)
-    h8_t ra1[2 * NP], rb1[2 * NP];
-    if (nk > 1) f16_load<NP>(op, i0, j0, kF16BK, tid, ra1, rb1);
-    __syncthreads();
-    for (int kt = 0; kt < nk; kt += 2) {
-      if (kt + 2 < nk) f16_load<NP>(op, i0, j0, (kt + 2) * kF16BK, tid, ra0, rb0);
-      compute(0);
-      if (kt + 1 < nk) f16_store<NP, 2>(sm, 1, tid, ra1, rb1);
-      __syncthreads();
-      if (kt + 1 >= nk) break;
-      if (kt + 3 < nk) f16_load<NP>(op, i0, j0, (kt + 3) * kF16BK, tid, ra1, rb1);
-      compute(1);
-      if (kt + 2 < nk) f16_store<NP, 2>(sm, 0, tid, ra0, rb0);
-      __syncthreads();
-    }
   }
   const bool diag = op.lower && bi == bj;
   f4_t rrow[4];  // row factors (powers of two: the products below are exact)
 #pragma unroll
-  for (int i = 0; i < 4; ++i) rrow[i] = *reinterpret_cast<const f4_t *>(op.ra + i0 + wm * 64 + i * 16 + 4 * lk);
+  for (int i = 0; i < 4; ++i) rrow[i] = *gld(reinterpret_cast<const f4_t *>(op.ra + i0 + wm * 64 + i * 16 + 4 * lk));
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int col = j0 + wn * 64 + j * 16 + lr;
-    const float rc = op.rb[col];
-    float *pc = op.C + (long)col * op.ldc + i0 + wm * 64 + 4 * lk;
+    const float rc = gld(op.rb)[col];
+    auto pc = gst(op.C + (long)col * op.ldc + i0 + wm * 64 + 4 * lk);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const f4_t v = cv[i][j] - acc[i][j] * (rrow[i] * rc);
       const int row = i0 + wm * 64 + i * 16 + 4 * lk;  // rows row .. row + 3
       if (!diag || row >= col) {
-        *reinterpret_cast<f4_t *>(pc + i * 16) = v;
+        *reinterpret_cast<__attribute__((address_space(1))) f4_t *>(pc + i * 16) = v;
       } else if (row + 3 >= col) {  // the diagonal crosses these 4 rows: lower part only
 #pragma unroll
         for (int r = 0; r < 4; ++r)
@@ -261,240 +240,6 @@ __global__ __launch_bounds__(256) void update_f16x2_kernel(const F16UpdOp *__res
   }
 }
 
-// ---------------------------------------------------------------------------
-// update, operands straight from L2: the 128 x 128 block and wave layout of
-// update_f16x2_kernel, but every wave loads its own MFMA fragments from the
-// k-slab-major panel (one contiguous 1 KB block per 16-row tile and part)
-// into registers, the next slab while this one is multiplied.  No LDS and no
-// barriers: the waves of a workgroup run independently.
-// ---------------------------------------------------------------------------
-template <int MODE>
-__global__ __launch_bounds__(256, 2) void update_lp_direct_kernel(const F16UpdOp *__restrict__ ops, int bpo, int nbn,
-                                                               const int *__restrict__ d_abort) {
-  constexpr int NP = (MODE == kLowF16x2) ? 2 : 1;
-  if (d_abort && *d_abort) return;
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int o = b / bpo, rr = b % bpo, bi = rr / nbn, bj = rr % nbn;
-  const F16UpdOp op = ops[o];
-  if (op.lower && bi < bj) return;
-  const int i0 = bi * kF16BM, j0 = bj * kF16BM;
-  if (i0 >= op.n || j0 >= op.n) return;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  const int lr = lane & 15, lk = lane >> 4;
-  const h8_t *ap[2] = {reinterpret_cast<const h8_t *>(op.ahi), reinterpret_cast<const h8_t *>(op.alo)};
-  const h8_t *bp[2] = {reinterpret_cast<const h8_t *>(op.bhi), reinterpret_cast<const h8_t *>(op.blo)};
-  const long ld = op.ld;
-  f4_t acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
-  h8_t fa[2][NP][4], fb[2][NP][4];
-  auto load = [&](int sl, int kt) {
-#pragma unroll
-    for (int q = 0; q < NP; ++q) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) fa[sl][q][i] = ap[q][((long)kt * ld + (i0 + wm * 64 + i * 16 + lr)) * 4 + lk];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) fb[sl][q][j] = bp[q][((long)kt * ld + (j0 + wn * 64 + j * 16 + lr)) * 4 + lk];
-    }
-  };
-  auto mma = [&](int sl) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        if constexpr (MODE == kLowF16x2) {
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[sl][1][i], fb[sl][0][j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[sl][0][i], fb[sl][1][j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[sl][0][i], fb[sl][0][j], acc[i][j], 0, 0, 0);
-        } else if constexpr (MODE == kLowF16) {
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[sl][0][i], fb[sl][0][j], acc[i][j], 0, 0, 0);
-        } else {
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, fa[sl][0][i]),
-                                                               __builtin_bit_cast(b8_t, fb[sl][0][j]), acc[i][j], 0, 0,
-                                                               0);
-        }
-      }
-  };
-  const int nk = op.k / kF16BK;
-  load(0, 0);
-  for (int kt = 0; kt < nk; kt += 2) {
-    if (kt + 1 < nk) load(1, kt + 1);
-    mma(0);
-    if (kt + 1 >= nk) break;
-    if (kt + 2 < nk) load(0, kt + 2);
-    mma(1);
-  }
-  const bool diag = op.lower && bi == bj;
-  float rrow[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) rrow[i][r] = op.ra[i0 + wm * 64 + i * 16 + Mfma<float>::drow(lk, r)];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int col = j0 + wn * 64 + j * 16 + lr;
-    const float rc = op.rb[col];
-    float *pc = op.C + (long)col * op.ldc;
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = i0 + wm * 64 + i * 16 + Mfma<float>::drow(lk, r);
-        if (!diag || row >= col) pc[row] -= acc[i][j][r] * (rrow[i][r] * rc);
-      }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// update, large block: 256 x 256 of C per workgroup, 4 waves of 128 x 128,
-// v_mfma_f32_32x32x16 (FP16 / BF16), k slabs of 32 staged in LDS (double
-// buffered, 64 KB per part and stage pair).  Twice the flops per byte of
-// L2 and LDS traffic of the 128 x 128 kernel.  The MFMA takes the C-column
-// side (tile n) as its A operand and the C-row side (tile m) as its B
-// operand, so each lane's results are C^T columns: 32 lanes store 32
-// consecutive rows of one C column.
-// ---------------------------------------------------------------------------
-constexpr int kBigBM = 256;
-typedef float f16v_t __attribute__((ext_vector_type(16)));
-
-template <int NP>
-struct BigSmem {
-  h8_t a[2][NP][kBigBM][kF16BK / 8];  // rows of tile m (C rows): [buf][part][row][chunk]
-  h8_t b[2][NP][kBigBM][kF16BK / 8];  // rows of tile n (C columns)
-};
-
-template <int MODE>
-__device__ __forceinline__ f16v_t mfma32(h8_t a, h8_t b, f16v_t c) {
-  if constexpr (MODE == kLowBF16)
-    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b8_t, a), __builtin_bit_cast(b8_t, b), c, 0,
-                                                    0, 0);
-  else
-    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
-}
-
-template <int MODE, int WAVES>
-__global__ __launch_bounds__(WAVES * 64) void update_lp_big_kernel(const F16UpdOp *__restrict__ ops, int bpo,
-                                                                    int nbn, const int *__restrict__ d_abort) {
-  constexpr int NP = (MODE == kLowF16x2) ? 2 : 1;
-  constexpr int NTH = WAVES * 64;
-  constexpr int WN = WAVES / 2;          // waves along the C columns (2 along the rows)
-  constexpr int TN = kBigBM / WN / 32;   // 32-column tiles per wave
-  constexpr int RPT = kBigBM / (NTH / 4);  // rows loaded per thread and operand
-  if (d_abort && *d_abort) return;
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int o = b / bpo, rr = b % bpo, bi = rr / nbn, bj = rr % nbn;
-  const F16UpdOp op = ops[o];
-  if (op.lower && bi < bj) return;
-  const int i0 = bi * kBigBM, j0 = bj * kBigBM;
-  if (i0 >= op.n || j0 >= op.n) return;
-  __shared__ __align__(16) BigSmem<NP> sm;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave / WN, wn = wave % WN;
-  const int l32 = lane & 31, lh = lane >> 5;
-  const h8_t *ap[2] = {reinterpret_cast<const h8_t *>(op.ahi), reinterpret_cast<const h8_t *>(op.alo)};
-  const h8_t *bp[2] = {reinterpret_cast<const h8_t *>(op.bhi), reinterpret_cast<const h8_t *>(op.blo)};
-
-  // global -> registers -> LDS: thread t moves chunk t & 3 of rows (t >> 2) + (NTH / 4) p
-  h8_t ra[RPT][NP], rb[RPT][NP];
-  const int cv = tid & 3, r0 = tid >> 2;
-  auto gload = [&](int k0) {
-#pragma unroll
-    for (int p = 0; p < RPT; ++p) {
-      const long oa = ((long)(k0 >> 5) * op.ld + (i0 + r0 + (NTH / 4) * p)) * 4 + cv;  // [k / 32][row][k % 32]
-      const long ob = ((long)(k0 >> 5) * op.ld + (j0 + r0 + (NTH / 4) * p)) * 4 + cv;
-#pragma unroll
-      for (int q = 0; q < NP; ++q) {
-        ra[p][q] = ap[q][oa];
-        rb[p][q] = bp[q][ob];
-      }
-    }
-  };
-  auto lstore = [&](int buf) {
-#pragma unroll
-    for (int p = 0; p < RPT; ++p) {
-      const int row = r0 + (NTH / 4) * p, c = swz(row, cv);
-#pragma unroll
-      for (int q = 0; q < NP; ++q) {
-        sm.a[buf][q][row][c] = ra[p][q];
-        sm.b[buf][q][row][c] = rb[p][q];
-      }
-    }
-  };
-
-  f16v_t acc[4][TN];  // [C-row tile tm][C-column tile tn]
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j) acc[i][j] = f16v_t{};
-
-  // fragments of one 16-wide k step: fn (C-column side), fm (C-row side);
-  // two sets, so that the next step's LDS reads overlap this step's MFMAs
-  h8_t fn[2][TN][NP], fm[2][4][NP];
-  auto fread = [&](int f, int buf, int ks) {
-    const int ch = ks * 2 + lh;  // this lane's 8 k values
-#pragma unroll
-    for (int t = 0; t < TN; ++t) {
-      const int rn = wn * (TN * 32) + t * 32 + l32;
-#pragma unroll
-      for (int q = 0; q < NP; ++q) fn[f][t][q] = sm.b[buf][q][rn][swz(rn, ch)];
-    }
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int rm = wm * 128 + t * 32 + l32;
-#pragma unroll
-      for (int q = 0; q < NP; ++q) fm[f][t][q] = sm.a[buf][q][rm][swz(rm, ch)];
-    }
-  };
-  auto fmma = [&](int f) {
-#pragma unroll
-    for (int tm = 0; tm < 4; ++tm)
-#pragma unroll
-      for (int tn = 0; tn < TN; ++tn) {
-        if constexpr (NP == 2) {
-          acc[tm][tn] = mfma32<MODE>(fn[f][tn][1], fm[f][tm][0], acc[tm][tn]);
-          acc[tm][tn] = mfma32<MODE>(fn[f][tn][0], fm[f][tm][1], acc[tm][tn]);
-        }
-        acc[tm][tn] = mfma32<MODE>(fn[f][tn][0], fm[f][tm][0], acc[tm][tn]);
-      }
-  };
-  const int nk = op.k / kF16BK;  // slabs of 32 = two k steps
-  gload(0);
-  lstore(0);
-  __syncthreads();
-  fread(0, 0, 0);
-  for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    const bool more = kt + 1 < nk;
-    if (more) gload((kt + 1) * kF16BK);  // in flight across both k steps
-    fread(1, cur, 1);
-    fmma(0);
-    if (more) lstore(cur ^ 1);  // that buffer's last readers passed the previous barrier
-    fmma(1);
-    __syncthreads();
-    if (more) fread(0, cur ^ 1, 0);
-  }
-  // C(row, col) -= ra[row] rb[col] D(col, row): lane -> row, element -> column
-  const bool diag = op.lower && bi == bj;
-#pragma unroll
-  for (int tm = 0; tm < 4; ++tm) {
-    const int row = i0 + wm * 128 + tm * 32 + l32;
-    const float fr = op.ra[row];
-#pragma unroll
-    for (int tn = 0; tn < TN; ++tn) {
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int col = j0 + wn * (TN * 32) + tn * 32 + 8 * (r / 4) + 4 * lh + (r % 4);
-        if (!diag || row >= col) {
-          float *pc = op.C + row + (long)col * op.ldc;
-          *pc -= acc[tm][tn][r] * (fr * op.rb[col]);
-        }
-      }
-    }
-  }
-}
 
 hipError_t launch_split(hipStream_t s, const SplitOp *ops, int nops, int NB, int mode) {
   if (nops <= 0) return hipSuccess;
@@ -510,63 +255,16 @@ hipError_t launch_split(hipStream_t s, const SplitOp *ops, int nops, int NB, int
 
 hipError_t launch_update_f16x2(hipStream_t s, const F16UpdOp *ops, int nops, int NB, const int *d_abort, int mode) {
   if (nops <= 0) return hipSuccess;
-  // block shape: 128 x 128 (default: measured fastest) or 256 x 256 with 4 or 8 waves
-  const char *blk = getenv("TTG_CHOL_F16_BLOCK");
-  const int shape = !blk ? 0 : (!strcmp(blk, "256x4") ? 4 : (!strcmp(blk, "256x8") ? 8 : 0));
-  if (NB % kBigBM == 0 && shape != 0) {
-    const int nbn = NB / kBigBM, bpo = nbn * nbn;
-    const long grid = (long)nops * bpo;
-    if (grid > 0x7fffffffL) return hipErrorInvalidConfiguration;
-    const dim3 g((unsigned)grid);
-    if (shape == 4) {
-      if (mode == kLowF16x2)
-        hipLaunchKernelGGL((update_lp_big_kernel<kLowF16x2, 4>), g, dim3(256), 0, s, ops, bpo, nbn, d_abort);
-      else if (mode == kLowF16)
-        hipLaunchKernelGGL((update_lp_big_kernel<kLowF16, 4>), g, dim3(256), 0, s, ops, bpo, nbn, d_abort);
-      else
-        hipLaunchKernelGGL((update_lp_big_kernel<kLowBF16, 4>), g, dim3(256), 0, s, ops, bpo, nbn, d_abort);
-    } else {
-      if (mode == kLowF16x2)
-        hipLaunchKernelGGL((update_lp_big_kernel<kLowF16x2, 8>), g, dim3(512), 0, s, ops, bpo, nbn, d_abort);
-      else if (mode == kLowF16)
-        hipLaunchKernelGGL((update_lp_big_kernel<kLowF16, 8>), g, dim3(512), 0, s, ops, bpo, nbn, d_abort);
-      else
-        hipLaunchKernelGGL((update_lp_big_kernel<kLowBF16, 8>), g, dim3(512), 0, s, ops, bpo, nbn, d_abort);
-    }
-    return hipGetLastError();
-  }
   const int nbn = NB / kF16BM, bpo = nbn * nbn;
   const long grid = (long)nops * bpo;
   if (grid > 0x7fffffffL) return hipErrorInvalidConfiguration;
   const dim3 g((unsigned)grid), b(256);
-  const char *dir = getenv("TTG_CHOL_F16_DIRECT");
-  if (dir && !strcmp(dir, "1")) {
-    if (mode == kLowF16x2)
-      hipLaunchKernelGGL(update_lp_direct_kernel<kLowF16x2>, g, b, 0, s, ops, bpo, nbn, d_abort);
-    else if (mode == kLowF16)
-      hipLaunchKernelGGL(update_lp_direct_kernel<kLowF16>, g, b, 0, s, ops, bpo, nbn, d_abort);
-    else
-      hipLaunchKernelGGL(update_lp_direct_kernel<kLowBF16>, g, b, 0, s, ops, bpo, nbn, d_abort);
-    return hipGetLastError();
-  }
-  // slabs of operands in flight: 1 (default: measured fastest, 2 workgroups per
-  // CU) or 2 (TTG_CHOL_F16_PREFETCH=2: more registers, one workgroup per CU)
-  const char *pf = getenv("TTG_CHOL_F16_PREFETCH");
-  if (!pf || strcmp(pf, "2")) {
-    if (mode == kLowF16x2)
-      hipLaunchKernelGGL((update_f16x2_kernel<kLowF16x2, 1>), g, b, 0, s, ops, bpo, nbn, d_abort);
-    else if (mode == kLowF16)
-      hipLaunchKernelGGL((update_f16x2_kernel<kLowF16, 1>), g, b, 0, s, ops, bpo, nbn, d_abort);
-    else
-      hipLaunchKernelGGL((update_f16x2_kernel<kLowBF16, 1>), g, b, 0, s, ops, bpo, nbn, d_abort);
-    return hipGetLastError();
-  }
   if (mode == kLowF16x2)
-    hipLaunchKernelGGL((update_f16x2_kernel<kLowF16x2, 2>), g, b, 0, s, ops, bpo, nbn, d_abort);
+    hipLaunchKernelGGL(update_f16x2_kernel<kLowF16x2>, g, b, 0, s, ops, bpo, nbn, d_abort);
   else if (mode == kLowF16)
-    hipLaunchKernelGGL((update_f16x2_kernel<kLowF16, 2>), g, b, 0, s, ops, bpo, nbn, d_abort);
+    hipLaunchKernelGGL(update_f16x2_kernel<kLowF16>, g, b, 0, s, ops, bpo, nbn, d_abort);
   else
-    hipLaunchKernelGGL((update_f16x2_kernel<kLowBF16, 2>), g, b, 0, s, ops, bpo, nbn, d_abort);
+    hipLaunchKernelGGL(update_f16x2_kernel<kLowBF16>, g, b, 0, s, ops, bpo, nbn, d_abort);
   return hipGetLastError();
 }
 

@@ -68,8 +68,8 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOpT<TS> *__restri
   typedef __attribute__((address_space(1))) const TS gts;
   const gts *Ag = (const gts *)op.A;
   auto store = [&](long r, long c, double v) {
-    W[r + c * ldw] = v;
-    if constexpr (sizeof(TS) != 8) op.A[r + c * lda] = (TS)v;
+    gst(W)[r + c * ldw] = v;
+    if constexpr (sizeof(TS) != 8) gst(op.A)[r + c * lda] = (TS)v;
   };
   int fail = 0;
   const int nblk = (n + kPB - 1) / kPB;
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOpT<TS> *__restri
     for (int e = tid; e < kPB * kPB; e += 256) {
       const int r = e & 31, c = e >> 5;
       if (r >= c && r < bs && c < bs) store(c0 + r, c0 + c, Ds[r][c]);
-      if (op.dinv) op.dinv[jb * kPB * kPB + e] = (TS)Di[r][c];
+      if (op.dinv) gst(op.dinv)[jb * kPB * kPB + e] = (TS)Di[r][c];
     }
     // rows below the diagonal block: X = acc * Di^T (staged through LDS into
     // MFMA A-operand layout, one 16 x 32 block at a time per wave)
@@ -324,14 +324,14 @@ __global__ __launch_bounds__(256) void trsm_kernel(const TrsmOp<T> *__restrict__
                                    min(kPB, op.n - c0));
       }
     }
-    for (int e = tid; e < kPB * kPB; e += 256) Dis[e & 31][e >> 5] = op.dinv[j * kPB * kPB + e];
+    for (int e = tid; e < kPB * kPB; e += 256) Dis[e & 31][e >> 5] = gld(op.dinv)[j * kPB * kPB + e];
 #pragma unroll
     for (int jj = 0; jj < 2; ++jj)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int rl = wave * 16 + Mfma<T>::drow(lk, r), cl = jj * 16 + lr;
         const int row = r0 + rl, col = c0 + cl;
-        const T bv = (row < op.m && col < op.n) ? op.B[row + (long)col * op.ldb] : T(0);
+        const T bv = (row < op.m && col < op.n) ? gld(op.B)[row + (long)col * op.ldb] : T(0);
         Ts[cl][rl] = bv - core.acc[0][jj][r];
       }
     __syncthreads();
@@ -351,7 +351,7 @@ __global__ __launch_bounds__(256) void trsm_kernel(const TrsmOp<T> *__restrict__
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = r0 + wave * 16 + Mfma<T>::drow(lk, r), col = c0 + jj * 16 + lr;
-        if (row < op.m && col < op.n) op.B[row + (long)col * op.ldb] = x[jj][r];
+        if (row < op.m && col < op.n) gst(op.B)[row + (long)col * op.ldb] = x[jj][r];
       }
     wg_sync_global();
   }
@@ -411,7 +411,7 @@ __device__ __forceinline__ void update_body(const UpdOp<T> *__restrict__ ops, Up
       for (int i = 0; i < Core::MI; ++i)
 #pragma unroll
         for (int j = 0; j < Core::NI; ++j) {
-          const T *pc = op.C + (long)(j0 + Core::col_of(wn, j, lane)) * op.ldc;
+          auto pc = gld(op.C + (long)(j0 + Core::col_of(wn, j, lane)) * op.ldc);
 #pragma unroll
           for (int r = 0; r < 4; ++r) cpre[i][j][r] = pc[i0 + Core::row_of(wm, i, lane, r)];
         }
@@ -428,7 +428,7 @@ __device__ __forceinline__ void update_body(const UpdOp<T> *__restrict__ ops, Up
 #pragma unroll
       for (int j = 0; j < Core::NI; ++j) {
         const int col = j0 + Core::col_of(wn, j, lane);
-        T *pc = op.C + (long)col * op.ldc;
+        auto pc = gst(op.C + (long)col * op.ldc);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int row = i0 + Core::row_of(wm, i, lane, r);
@@ -446,7 +446,7 @@ __device__ __forceinline__ void update_body(const UpdOp<T> *__restrict__ ops, Up
 #pragma unroll
     for (int j = 0; j < Core::NI; ++j) {
       const int col = j0 + Core::col_of(wn, j, lane);
-      T *pc = op.C + (long)col * op.ldc;
+      auto pc = gst(op.C + (long)col * op.ldc);
       if (full) {
         if (beta == T(0)) {
 #pragma unroll
