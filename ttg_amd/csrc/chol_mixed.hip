@@ -29,12 +29,76 @@
 
 namespace ttgc {
 
-// Working copy of the lower tiles in precision T (padding is set later by
-// the factorization's prepare step).
+// Working copy and ||A||_inf in one pass over the FP64 tiles: one workgroup
+// per lower tile (m, n) converts it and writes |A| partial sums, S[t][0][r] =
+// sum over the tile's row r (lower part on a diagonal tile) and S[t][1][c] =
+// sum over its column c (strictly lower part on a diagonal tile: the
+// transpose's contribution to row n*NB + c).  Thread t owns rows t and
+// t + 256 (NB <= 512); each wave reduces its rows of a column by shuffles in
+// a fixed order, and the four wave sums are added in wave order.
 template <typename T>
-__global__ void convert_tiles_kernel(long n, const double *__restrict__ A, T *__restrict__ W) {
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    W[i] = (T)A[i];
+__global__ __launch_bounds__(256) void convert_norm_kernel(int N, int NB, int nt, const double *__restrict__ A,
+                                                           T *__restrict__ W, double *__restrict__ S) {
+  const long t = blockIdx.x;
+  int m, n;
+  {
+    int lo = 0, hi = nt - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) / 2;
+      if ((long)mid * nt - (long)mid * (mid - 1) / 2 <= t)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    n = lo;
+    m = n + (int)(t - ((long)n * nt - (long)n * (n - 1) / 2));
+  }
+  const double *a = A + t * (long)NB * NB;
+  T *w = W + t * (long)NB * NB;
+  double *rs = S + t * 2L * NB, *cs = rs + NB;
+  __shared__ double csum[4][512];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int mrows = min(NB, N - m * NB), ncols = min(NB, N - n * NB);
+  const bool diag = m == n;
+  const int r0 = tid, r1 = tid + 256;
+  double ra0 = 0.0, ra1 = 0.0;
+  for (int c = 0; c < NB; ++c) {
+    const long o = (long)c * NB;
+    double u = 0.0;
+    if (r0 < NB) {
+      const double v = a[o + r0];
+      w[o + r0] = (T)v;
+      const bool in = r0 < mrows && c < ncols;
+      if (in && (!diag || r0 >= c)) ra0 += fabs(v);
+      if (in && (!diag || r0 > c)) u += fabs(v);
+    }
+    if (r1 < NB) {
+      const double v = a[o + r1];
+      w[o + r1] = (T)v;
+      const bool in = r1 < mrows && c < ncols;
+      if (in && (!diag || r1 >= c)) ra1 += fabs(v);
+      if (in && (!diag || r1 > c)) u += fabs(v);
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) u += __shfl_xor(u, d);
+    if (lane == 0) csum[wave][c] = u;
+  }
+  if (r0 < NB) rs[r0] = ra0;
+  if (r1 < NB) rs[r1] = ra1;
+  __syncthreads();
+  for (int c = tid; c < NB; c += 256) cs[c] = ((csum[0][c] + csum[1][c]) + csum[2][c]) + csum[3][c];
+}
+
+// -rowsum_i of |A| from the tile partial sums, in a fixed order (row block m:
+// tiles (m, 0..m) by rows, then tiles (m..nt-1, m) by columns).
+__global__ void norm_rows_kernel(int N, int NB, int nt, const double *__restrict__ S, double *__restrict__ rows) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int m = i / NB, r = i % NB;
+  double s = 0.0;
+  for (int n = 0; n <= m; ++n) s += S[tile_index(m, n, nt) * 2L * NB + r];
+  for (int n2 = m; n2 < nt; ++n2) s += S[tile_index(n2, m, nt) * 2L * NB + NB + r];
+  rows[i] = -s;
 }
 
 // Per right-hand side j: berr_j = max|R_j| / (anorm * max|X_j| + max|B_j|).
@@ -105,7 +169,7 @@ struct MixedPlan : MixedPlanBase {
   DevBuf W;      // working factor tiles (T)
   DevBuf R;      // residual, N x nrhs (FP64)
   DevBuf stats;  // anorm, berr[nrhs]
-  DevBuf ones, zeros, rows;
+  DevBuf rows, norm_parts;  // -rowsum of |A|; per-tile |A| partial sums
   std::unique_ptr<TiledPlanT<T>> fplan;
   SolvePlan<T> *splan = nullptr;
   std::vector<double> hstats;
@@ -120,15 +184,8 @@ struct MixedPlan : MixedPlanBase {
     TTGC_HIP(W.reserve(sizeof(T) * ntile));
     TTGC_HIP(R.reserve(sizeof(double) * (size_t)N * nrhs));
     TTGC_HIP(stats.reserve(sizeof(double) * (1 + nrhs)));
-    TTGC_HIP(ones.reserve(sizeof(double) * (size_t)N));
-    TTGC_HIP(zeros.reserve(sizeof(double) * (size_t)N));
     TTGC_HIP(rows.reserve(sizeof(double) * (size_t)N));
-    {
-      std::vector<double> h(N, 1.0);
-      TTGC_HIP(hipMemcpy(ones.ptr, h.data(), sizeof(double) * N, hipMemcpyHostToDevice));
-      std::fill(h.begin(), h.end(), 0.0);
-      TTGC_HIP(hipMemcpy(zeros.ptr, h.data(), sizeof(double) * N, hipMemcpyHostToDevice));
-    }
+    TTGC_HIP(norm_parts.reserve(sizeof(double) * (size_t)nt * (nt + 1) / 2 * 2 * NB));
     fplan.reset(new TiledPlanT<T>);
     fplan->N = N;
     fplan->NB = NB;
@@ -163,19 +220,16 @@ struct MixedPlan : MixedPlanBase {
   int run(hipStream_t s, const double *A, const double *B, long ldb, double *X, long ldx, double tol, int max_iter,
           int *d_info, int *iters, double *berr) {
     const int nt = (N + NB - 1) / NB;
-    const long ntile = (long)nt * (nt + 1) / 2 * NB * NB;
-    // 1. working copy
+    // 1. working copy (padding is set later by the factorization's prepare
+    //    step) and ||A||_inf in one pass over A
     {
-      const long blocks = std::min<long>((ntile + 255) / 256, 65536L);
-      hipLaunchKernelGGL(convert_tiles_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, s, ntile, A, w());
+      const long ntiles = (long)nt * (nt + 1) / 2;
+      double *parts = static_cast<double *>(norm_parts.ptr), *rw = static_cast<double *>(rows.ptr);
+      hipLaunchKernelGGL(convert_norm_kernel<T>, dim3((unsigned)ntiles), dim3(256), 0, s, N, NB, nt, A, w(), parts);
       TTGC_HIP(hipGetLastError());
-    }
-    // ||A||_inf: rows = 0 - |A| 1
-    {
-      int rc = residual_fp64(s, N, NB, A, 1, static_cast<double *>(ones.ptr), N, static_cast<double *>(zeros.ptr), N,
-                             static_cast<double *>(rows.ptr), N, true);
-      if (rc) return rc;
-      hipLaunchKernelGGL(max_neg_kernel, dim3(1), dim3(1024), 0, s, N, static_cast<double *>(rows.ptr), st());
+      hipLaunchKernelGGL(norm_rows_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, N, NB, nt, parts, rw);
+      TTGC_HIP(hipGetLastError());
+      hipLaunchKernelGGL(max_neg_kernel, dim3(1), dim3(1024), 0, s, N, rw, st());
       TTGC_HIP(hipGetLastError());
     }
     // 2. factor
