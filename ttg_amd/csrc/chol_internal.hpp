@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <string>
 
@@ -27,6 +28,34 @@ inline int hip_fail(hipError_t e, const char *what) {
   } while (0)
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Two streams whose HSA queues split the CUs of the current device: `few` may
+// only use CUs 0 .. want-1 of the device mask, `rest` the others (KFD deals
+// the mask bits round-robin over the XCDs, so want = 8 is one CU per XCD).
+// Work on `few` then never waits behind thousands of workgroups on `rest`.
+// Returns false (and creates nothing) if the device is too small for the
+// split or the runtime refuses the masks; the caller then uses plain streams.
+inline bool make_split_streams(int want, hipStream_t *few, hipStream_t *rest) {
+  int dev = 0, ncu = 0;
+  if (want <= 0 || hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || want * 4 > ncu)
+    return false;
+  const int words = (ncu + 31) / 32;
+  uint32_t mfew[16] = {0}, mrest[16] = {0};
+  if (words > 16) return false;
+  for (int c = 0; c < ncu; ++c) (c < want ? mfew : mrest)[c / 32] |= 1u << (c % 32);
+  hipStream_t a = nullptr, b = nullptr;
+  if (hipExtStreamCreateWithCUMask(&a, words, mfew) != hipSuccess ||
+      hipExtStreamCreateWithCUMask(&b, words, mrest) != hipSuccess) {
+    if (a) (void)hipStreamDestroy(a);
+    if (b) (void)hipStreamDestroy(b);
+    (void)hipGetLastError();
+    return false;
+  }
+  *few = a;
+  *rest = b;
+  return true;
+}
 
 // Device buffer that grows on demand (never shrinks); used for descriptor
 // staging and workspaces owned by the library.

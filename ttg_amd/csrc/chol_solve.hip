@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -89,6 +90,127 @@ __global__ void identity_tiles_kernel(int N, int NB, T *__restrict__ linv) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// The two GEMMs on the critical path of a sweep step, split over k:
+//   P[y][c][r] = sum_{kk in chunk y} A(r, kk) B(c, kk),   A (nr x k), B (n x k),
+// B(c, kk) = B[c + kk ldb] (TB: B[kk + c ldb]).  Workgroup (x, y) takes the
+// 64 columns x and the 64-k chunk y, so a 512 x 512 tile is streamed by 64
+// workgroups with all their loads in flight at once.  With SUB, the A operand
+// is first reduced from an earlier product:  A(r, kk) - sum_q Q[q][kk][r]
+// (the next-block update folded into the inverse apply that consumes it).
+// apply_reduce_kernel adds the chunks in order (deterministic).
+// ---------------------------------------------------------------------------
+constexpr int kApB = 64;
+constexpr int kMaxChunks = 8;  // k chunks of a product: NB <= 512
+
+template <typename T, bool TB, bool SUB>
+__global__ __launch_bounds__(256) void apply_partial_kernel(int nr, int n, int k, const T *__restrict__ A, int lda,
+                                                            const T *__restrict__ B, int ldb, const T *__restrict__ Q,
+                                                            int nq, T *__restrict__ P) {
+  const int c0 = blockIdx.x * kApB, k0 = blockIdx.y * kApB;
+  __shared__ T bs[kApB][kApB + 1];  // bs[kk][cc] = B(c0 + cc, k0 + kk)
+  __shared__ T as[kApB][17];        // as[kk][r] = A(r0 + r, k0 + kk), 16 right-hand sides at a time
+  const int tid = threadIdx.x, cc = tid & 63, rg = tid >> 6;
+  T bv[kApB * kApB / 256];
+#pragma unroll
+  for (int i = 0; i < kApB * kApB / 256; ++i) {
+    const int e = tid + 256 * i, fast = e & 63, slow = e >> 6;  // fast runs along contiguous memory
+    const int c = TB ? slow : fast, kk = TB ? fast : slow;
+    const bool ok = c0 + c < n && k0 + kk < k;
+    bv[i] = ok ? (TB ? B[(k0 + kk) + (long)(c0 + c) * ldb] : B[(c0 + c) + (long)(k0 + kk) * ldb]) : T(0);
+  }
+#pragma unroll
+  for (int i = 0; i < kApB * kApB / 256; ++i) {
+    const int e = tid + 256 * i, fast = e & 63, slow = e >> 6;
+    bs[TB ? fast : slow][TB ? slow : fast] = bv[i];
+  }
+  for (int r0 = 0; r0 < nr; r0 += 16) {
+    __syncthreads();
+    T av[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = tid + 256 * i, r = e & 15, kk = e >> 4;
+      const bool ok = r0 + r < nr && k0 + kk < k;
+      T v = ok ? A[(r0 + r) + (long)(k0 + kk) * lda] : T(0);
+      if constexpr (SUB) {
+        // all chunks loaded before the first subtraction (nq <= 8), then
+        // subtracted in chunk order
+        T qv[kMaxChunks];
+#pragma unroll
+        for (int q = 0; q < kMaxChunks; ++q) qv[q] = (ok && q < nq) ? Q[((long)q * k + k0 + kk) * nr + r0 + r] : T(0);
+#pragma unroll
+        for (int q = 0; q < kMaxChunks; ++q) v -= qv[q];
+      }
+      av[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = tid + 256 * i;
+      as[e >> 4][e & 15] = av[i];
+    }
+    __syncthreads();
+    T acc[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll 8
+    for (int kk = 0; kk < kApB; ++kk) {
+      const T bb = bs[kk][cc];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[j] += as[kk][rg * 4 + j] * bb;
+    }
+    if (c0 + cc < n) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int r = r0 + rg * 4 + j;
+        if (r < nr) P[((long)blockIdx.y * n + c0 + cc) * nr + r] = acc[j];
+      }
+    }
+  }
+}
+
+template <typename T>
+__global__ void apply_reduce_kernel(int nr, int n, int nkc, const T *__restrict__ P, T *__restrict__ C, int ldc) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nr * n) return;
+  const int r = g % nr, c = g / nr;
+  T pv[kMaxChunks];
+#pragma unroll
+  for (int q = 0; q < kMaxChunks; ++q) pv[q] = q < nkc ? P[((long)q * n + c) * nr + r] : T(0);
+  T s = T(0);
+#pragma unroll
+  for (int q = 0; q < kMaxChunks; ++q) s += pv[q];
+  C[(long)c * ldc + r] = s;
+}
+
+// P := the k-split product of op (A B^T); with Q, A is op.A - sum of the nq
+// chunks of Q first.
+template <typename T>
+static hipError_t launch_partial(hipStream_t s, const UpdOp<T> &op, bool tb, const T *Q, int nq, T *P) {
+  const dim3 g((op.n + kApB - 1) / kApB, (op.k + kApB - 1) / kApB);
+  if (tb) {
+    if (Q)
+      hipLaunchKernelGGL((apply_partial_kernel<T, true, true>), g, dim3(256), 0, s, op.m, op.n, op.k, op.A, op.lda,
+                         op.B, op.ldb, Q, nq, P);
+    else
+      hipLaunchKernelGGL((apply_partial_kernel<T, true, false>), g, dim3(256), 0, s, op.m, op.n, op.k, op.A, op.lda,
+                         op.B, op.ldb, Q, nq, P);
+  } else {
+    if (Q)
+      hipLaunchKernelGGL((apply_partial_kernel<T, false, true>), g, dim3(256), 0, s, op.m, op.n, op.k, op.A, op.lda,
+                         op.B, op.ldb, Q, nq, P);
+    else
+      hipLaunchKernelGGL((apply_partial_kernel<T, false, false>), g, dim3(256), 0, s, op.m, op.n, op.k, op.A, op.lda,
+                         op.B, op.ldb, Q, nq, P);
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_reduce(hipStream_t s, const UpdOp<T> &op, const T *P) {
+  const int tot = op.m * op.n;
+  hipLaunchKernelGGL(apply_reduce_kernel<T>, dim3((tot + 255) / 256), dim3(256), 0, s, op.m, op.n,
+                     (op.k + kApB - 1) / kApB, P, op.C, op.ldc);
+  return hipGetLastError();
+}
+
 template <typename T>
 int SolvePlan<T>::build() {
   NT = (N + NB - 1) / NB;
@@ -98,6 +220,7 @@ int SolvePlan<T>::build() {
   TTGC_HIP(d_xt.reserve(sizeof(T) * (size_t)Np * nrhs));
   TTGC_HIP(d_dinv.reserve(sizeof(T) * (size_t)dstride * NT));
   TTGC_HIP(d_linv.reserve(sizeof(T) * (size_t)NB * NB * NT));
+  TTGC_HIP(d_part.reserve(sizeof(T) * 2 * (size_t)((NB + kApB - 1) / kApB) * NB * nrhs));
   T *bt = static_cast<T *>(d_bt.ptr);
   T *xt = static_cast<T *>(d_xt.ptr);
   T *dinv = static_cast<T *>(d_dinv.ptr);
@@ -145,9 +268,20 @@ int SolvePlan<T>::build() {
   memcpy(host.data() + o_inv, inv.data(), inv.size() * sizeof(TrsmOp<T>));
   if (!upd.empty()) memcpy(host.data() + o_upd, upd.data(), upd.size() * sizeof(UpdOp<T>));
   TTGC_HIP(hipMemcpy(d_ops.ptr, host.data(), total, hipMemcpyHostToDevice));
-  TTGC_HIP(hipStreamCreateWithFlags(&s_main, hipStreamNonBlocking));
-  TTGC_HIP(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
-  TTGC_HIP(hipEventCreateWithFlags(&ev_out, hipEventDisableTiming));
+  host_ops.assign(upd.begin(), upd.end());
+  // the critical path of the sweeps (next-block product, inverse apply) on 32
+  // CUs of its own, the bulk updates on the others; plain streams (and a
+  // replayed graph) if the masks are refused or TTG_CHOL_SWEEP_CUS=0
+  {
+    const char *e = getenv("TTG_CHOL_SWEEP_CUS");
+    split = make_split_streams(e ? atoi(e) : 32, &s_main, &s_rest);
+  }
+  if (!split) {
+    TTGC_HIP(hipStreamCreateWithFlags(&s_main, hipStreamNonBlocking));
+    TTGC_HIP(hipStreamCreateWithFlags(&s_rest, hipStreamNonBlocking));
+  }
+  for (hipEvent_t *e : {&ev_in, &ev_out, &ev_x, &ev_rest[0], &ev_rest[1]})
+    TTGC_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
   return TTG_CHOL_OK;
 }
 
@@ -162,22 +296,48 @@ int SolvePlan<T>::prepare(hipStream_t s) {
   return TTG_CHOL_OK;
 }
 
+// Sweep step q (block i): the update of the remaining blocks from x_i is
+// split in two.  The next block's (it feeds the next inverse apply) runs on s
+// as a k-split product whose chunks the next apply subtracts from its
+// operand as it loads it; the rest runs on s_rest as a batched update,
+// overlapping the following steps.  The next apply reads the block the rest
+// update of step q-1 wrote, so it waits for that one (events alternate
+// between two slots).
 template <typename T>
 int SolvePlan<T>::enqueue_sweeps(hipStream_t s) {
   UpdOp<T> unone{};
-  const T m1 = T(-1), p1 = T(1), z = T(0);
+  const T m1 = T(-1), p1 = T(1);
   const bool fast = NB % 128 == 0 && N % NB == 0;
-  for (int q = 0; q < NT; ++q) {
-    TTGC_HIP(launch_update<T>(s, upd_ops() + fw_app[q], 1, unone, nrhs, NB, p1, z, fast, nullptr, false));
-    if (fw_upd_cnt[q])
-      TTGC_HIP(launch_update<T>(s, upd_ops() + fw_upd_off[q], fw_upd_cnt[q], unone, nrhs, NB, m1, p1, fast, nullptr,
-                                false));
-  }
-  for (int q = 0; q < NT; ++q) {
-    TTGC_HIP(launch_update<T>(s, upd_ops() + bw_app[q], 1, unone, nrhs, NB, p1, z, fast, nullptr, true));
-    if (bw_upd_cnt[q])
-      TTGC_HIP(launch_update<T>(s, upd_ops() + bw_upd_off[q], bw_upd_cnt[q], unone, nrhs, NB, m1, p1, fast, nullptr,
-                                true));
+  T *pa = static_cast<T *>(d_part.ptr);                          // apply chunks
+  T *pn = pa + (size_t)((NB + kApB - 1) / kApB) * NB * nrhs;  // next-block update chunks
+  for (int dir = 0; dir < 2; ++dir) {
+    const bool tb = dir == 1;
+    const std::vector<int> &app = tb ? bw_app : fw_app, &off = tb ? bw_upd_off : fw_upd_off,
+                           &cnt = tb ? bw_upd_cnt : fw_upd_cnt;
+    bool pending = false;  // a rest update of the previous step is in flight on s_rest
+    int slot = 0;
+    TTGC_HIP(launch_partial<T>(s, host_ops[app[0]], tb, nullptr, 0, pa));
+    TTGC_HIP(launch_reduce<T>(s, host_ops[app[0]], pa));
+    for (int q = 0; q < NT; ++q) {
+      const int c = cnt[q];
+      if (c == 0) break;
+      // forward: next block is the first op (m = i + 1); backward: the last (m = i - 1)
+      const int next = tb ? off[q] + c - 1 : off[q], rest = tb ? off[q] : off[q] + 1;
+      if (c > 1) {
+        TTGC_HIP(hipEventRecord(ev_x, s));
+        TTGC_HIP(hipStreamWaitEvent(s_rest, ev_x, 0));
+        TTGC_HIP(launch_update<T>(s_rest, upd_ops() + rest, c - 1, unone, nrhs, NB, m1, p1, fast, nullptr, tb));
+      }
+      const UpdOp<T> &on = host_ops[next], &oa = host_ops[app[q + 1]];
+      TTGC_HIP(launch_partial<T>(s, on, tb, nullptr, 0, pn));
+      if (pending) TTGC_HIP(hipStreamWaitEvent(s, ev_rest[slot ^ 1], 0));
+      pending = c > 1;
+      if (pending) TTGC_HIP(hipEventRecord(ev_rest[slot], s_rest));
+      slot ^= 1;
+      TTGC_HIP(launch_partial<T>(s, oa, tb, pn, (on.k + kApB - 1) / kApB, pa));
+      TTGC_HIP(launch_reduce<T>(s, oa, pa));
+    }
+    if (pending) TTGC_HIP(hipStreamWaitEvent(s, ev_rest[slot ^ 1], 0));
   }
   return TTG_CHOL_OK;
 }
@@ -186,10 +346,11 @@ template <typename T>
 SolvePlan<T>::~SolvePlan() {
   if (gexec) (void)hipGraphExecDestroy(gexec);
   if (graph) (void)hipGraphDestroy(graph);
-  if (ev_in) (void)hipEventDestroy(ev_in);
-  if (ev_out) (void)hipEventDestroy(ev_out);
+  for (hipEvent_t e : {ev_in, ev_out, ev_x, ev_rest[0], ev_rest[1]})
+    if (e) (void)hipEventDestroy(e);
   if (s_main) (void)hipStreamDestroy(s_main);
-  for (DevBuf *b : {&d_ops, &d_bt, &d_xt, &d_dinv, &d_linv})
+  if (s_rest) (void)hipStreamDestroy(s_rest);
+  for (DevBuf *b : {&d_ops, &d_bt, &d_xt, &d_dinv, &d_linv, &d_part})
     if (b->ptr) (void)hipFree(b->ptr);
 }
 
@@ -197,6 +358,15 @@ SolvePlan<T>::~SolvePlan() {
 // after `user` and before whatever `user` does next.
 template <typename T>
 int SolvePlan<T>::run_sweeps(hipStream_t user) {
+  if (split) {  // CU masks live on the queues: launch directly
+    TTGC_HIP(hipEventRecord(ev_in, user));
+    TTGC_HIP(hipStreamWaitEvent(s_main, ev_in, 0));
+    int rc = enqueue_sweeps(s_main);
+    if (rc != TTG_CHOL_OK) return rc;
+    TTGC_HIP(hipEventRecord(ev_out, s_main));
+    TTGC_HIP(hipStreamWaitEvent(user, ev_out, 0));
+    return TTG_CHOL_OK;
+  }
   if (!gexec) {
     TTGC_HIP(hipStreamBeginCapture(s_main, hipStreamCaptureModeThreadLocal));
     int rc = enqueue_sweeps(s_main);

@@ -149,23 +149,11 @@ struct TiledPlanT {
   void reserve_cus() {
     const char *e = getenv("TTG_CHOL_POTRF_CUS");
     const int want = e ? atoi(e) : (sizeof(T) == 4 ? 8 : 0);
-    int dev = 0, ncu = 0;
-    if (want <= 0 || hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || want * 4 > ncu)
-      return;
-    const int words = (ncu + 31) / 32;
-    std::vector<uint32_t> mdiag(words, 0u), mrest(words, 0u);
-    for (int c = 0; c < ncu; ++c) (c < want ? mdiag : mrest)[c / 32] |= 1u << (c % 32);
     hipStream_t d = nullptr, m = nullptr;
-    if (hipExtStreamCreateWithCUMask(&d, words, mdiag.data()) != hipSuccess ||
-        hipExtStreamCreateWithCUMask(&m, words, mrest.data()) != hipSuccess) {
-      if (d) (void)hipStreamDestroy(d);
-      if (m) (void)hipStreamDestroy(m);
-      (void)hipGetLastError();
-      return;
+    if (make_split_streams(want, &d, &m)) {
+      s_diag = d;
+      s_main = m;
     }
-    s_diag = d;
-    s_main = m;
   }
 
   // Split-FP16 descriptors: panel k lives in slot k % 2 (tile m at position
