@@ -1,0 +1,47 @@
+"""The CU-split schedules of the planned solve against the plain ones: the
+diagonal POTRF on its own CUs beside the bulk update (TTG_CHOL_POTRF_CUS),
+and the triangular sweeps' critical path on its own CUs beside the far-block
+updates (TTG_CHOL_SWEEP_CUS).  Both only reorder independent work, so the
+solution must be bitwise the one of the plain two-stream / hipGraph schedule
+(a race between the streams would show as a difference).
+
+Each schedule is chosen when its plan is built, and plans are cached per
+shape and right-hand-side count, so the two runs use different (otherwise
+unused) RHS counts; a right-hand side's solution does not depend on how many
+others are solved with it.  A fixed number of refinement steps (tol 1e-300) keeps
+the two runs on the same arithmetic."""
+import numpy as np
+import pytest
+import torch
+
+from gpu_util import to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _solve(C, dA, N, NB, Bh, nrhs, prec, dev):
+    dB = torch.from_numpy(np.ascontiguousarray(Bh[:, :nrhs].T)).to(dev).t()
+    X = torch.zeros(nrhs, N, dtype=torch.float64, device=dev).t()
+    info = torch.zeros(1, dtype=torch.int32, device=dev)
+    its, _ = C.dposv_mixed(dA, N, NB, dB, X, prec=prec, tol=1e-300, max_iter=2, info=info, check=False)
+    assert int(info.item()) == 0 and its == 2
+    return X.t().cpu().numpy().T
+
+
+@pytest.mark.parametrize("prec,r1,r2", [(1, 6, 7), (0, 9, 11)])
+def test_split_schedules_match_plain(dev, oracle, monkeypatch, prec, r1, r2):
+    import ttg_amd.chol as C
+
+    N, NB = 4096, 512
+    sigma2 = sum(k * k for k in range(-127, 128)) / 255.0 / 127.0 ** 2
+    A0 = oracle.spd_tiles(N, NB, 4 * sigma2 / (1e4 - 1), 21)
+    Bh = np.random.default_rng(5).uniform(-1, 1, (N, r2))
+    dA = to_dev(A0, dev)
+    monkeypatch.setenv("TTG_CHOL_POTRF_CUS", "8")
+    monkeypatch.setenv("TTG_CHOL_SWEEP_CUS", "32")
+    Xs = _solve(C, dA, N, NB, Bh, r1, prec, dev)
+    monkeypatch.setenv("TTG_CHOL_POTRF_CUS", "0")
+    monkeypatch.setenv("TTG_CHOL_SWEEP_CUS", "0")
+    Xp = _solve(C, dA, N, NB, Bh, r2, prec, dev)
+    assert np.array_equal(Xs, Xp[:, :r1])
+    assert oracle.solve_backward_error(N, NB, A0, Xs, Bh[:, :r1]) <= 1e-12
