@@ -1,9 +1,11 @@
-"""The CU-split schedules of the planned solve against the plain ones: the
-diagonal POTRF on its own CUs beside the bulk update (TTG_CHOL_POTRF_CUS),
-and the triangular sweeps' critical path on its own CUs beside the far-block
-updates (TTG_CHOL_SWEEP_CUS).  Both only reorder independent work, so the
-solution must be bitwise the one of the plain two-stream / hipGraph schedule
-(a race between the streams would show as a difference).
+"""The schedules of the planned solve against each other: the diagonal POTRF
+on its own CUs beside the bulk update (TTG_CHOL_POTRF_CUS), the triangular
+sweeps' critical path on its own CUs beside the far-block updates
+(TTG_CHOL_SWEEP_CUS), and the paired split-FP16 schedule whose bulk updates
+apply two panels in one pass over C (TTG_CHOL_PAIRED).  All of them only
+reorder independent work or fuse two FP32 updates of a tile in their own
+order, so the solution must be bitwise the one of the plain two-stream /
+hipGraph schedule (a race between the streams would show as a difference).
 
 Each schedule is chosen when its plan is built, and plans are cached per
 shape and right-hand-side count, so the two runs use different (otherwise
@@ -45,3 +47,19 @@ def test_split_schedules_match_plain(dev, oracle, monkeypatch, prec, r1, r2):
     Xp = _solve(C, dA, N, NB, Bh, r2, prec, dev)
     assert np.array_equal(Xs, Xp[:, :r1])
     assert oracle.solve_backward_error(N, NB, A0, Xs, Bh[:, :r1]) <= 1e-12
+
+
+def test_paired_schedule_matches_unpaired(dev, oracle, monkeypatch):
+    import ttg_amd.chol as C
+
+    N, NB = 5120, 512  # NT = 10: bulk updates of several pairs
+    sigma2 = sum(k * k for k in range(-127, 128)) / 255.0 / 127.0 ** 2
+    A0 = oracle.spd_tiles(N, NB, 4 * sigma2 / (1e4 - 1), 23)
+    Bh = np.random.default_rng(6).uniform(-1, 1, (N, 13))
+    dA = to_dev(A0, dev)
+    monkeypatch.setenv("TTG_CHOL_PAIRED", "1")
+    Xp = _solve(C, dA, N, NB, Bh, 12, 1, dev)
+    monkeypatch.setenv("TTG_CHOL_PAIRED", "0")
+    Xu = _solve(C, dA, N, NB, Bh, 13, 1, dev)
+    assert np.array_equal(Xp, Xu[:, :12])
+    assert oracle.solve_backward_error(N, NB, A0, Xp, Bh[:, :12]) <= 1e-12

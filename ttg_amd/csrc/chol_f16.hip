@@ -26,7 +26,6 @@ typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
 // ---------------------------------------------------------------------------
 template <int MODE>
 __global__ __launch_bounds__(256) void split_kernel(const SplitOp *__restrict__ ops, int NB) {
-  __builtin_amdgcn_s_setprio(2);  // panel work: ahead of co-resident update waves
   const int nst = NB / 64;
   const SplitOp op = ops[blockIdx.x / nst];
   const int r0 = (blockIdx.x % nst) * 64;
@@ -104,18 +103,24 @@ struct F16Smem {
   h8_t b[NST][NP][kF16BM][kF16BK / 8];
 };
 
+// the operand parts of one product: A hi, A lo, B hi, B lo and the row factors
+struct F16Parts {
+  const uint16_t *ah, *al, *bh, *bl;
+  const float *ra, *rb;
+};
+
 template <int NP>
-__device__ __forceinline__ void f16_load(const F16UpdOp &op, int i0, int j0, int k0, int tid, h8_t (&ra)[2 * NP],
-                                         h8_t (&rb)[2 * NP]) {
+__device__ __forceinline__ void f16_load(const F16Parts &pp, long ld, int i0, int j0, int k0, int tid,
+                                         h8_t (&ra)[2 * NP], h8_t (&rb)[2 * NP]) {
   // 128 rows x 32 k per part and operand = 512 16-byte chunks; 2 per thread
   const int v = tid & 3, r = tid >> 2;  // chunk in row, row 0..63
-  const h8_t *ap[2] = {reinterpret_cast<const h8_t *>(op.ahi), reinterpret_cast<const h8_t *>(op.alo)};
-  const h8_t *bp[2] = {reinterpret_cast<const h8_t *>(op.bhi), reinterpret_cast<const h8_t *>(op.blo)};
+  const h8_t *ap[2] = {reinterpret_cast<const h8_t *>(pp.ah), reinterpret_cast<const h8_t *>(pp.al)};
+  const h8_t *bp[2] = {reinterpret_cast<const h8_t *>(pp.bh), reinterpret_cast<const h8_t *>(pp.bl)};
 #pragma unroll
   for (int p = 0; p < 2; ++p) {
     const int row = r + 64 * p;
-    const long oa = ((long)(k0 >> 5) * op.ld + (i0 + row)) * 4 + v;  // [k / 32][row][k % 32]
-    const long ob = ((long)(k0 >> 5) * op.ld + (j0 + row)) * 4 + v;
+    const long oa = ((long)(k0 >> 5) * ld + (i0 + row)) * 4 + v;  // [k / 32][row][k % 32]
+    const long ob = ((long)(k0 >> 5) * ld + (j0 + row)) * 4 + v;
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
       ra[NP * p + q] = gld(ap[q])[oa];
@@ -142,7 +147,7 @@ __device__ __forceinline__ void f16_store(F16Smem<NP, NST> &sm, int buf, int tid
 typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
 
 template <int MODE>
-__global__ __launch_bounds__(256) void update_f16x2_kernel(const F16UpdOp *__restrict__ ops, int bpo, int nbn,
+__global__ __launch_bounds__(256, 2) void update_f16x2_kernel(const F16UpdOp *__restrict__ ops, int bpo, int nbn,
                                                            const int *__restrict__ d_abort) {
   constexpr int NP = (MODE == kLowF16x2) ? 2 : 1;
   if (d_abort && *d_abort) return;
@@ -164,8 +169,16 @@ __global__ __launch_bounds__(256) void update_f16x2_kernel(const F16UpdOp *__res
   // the first slab's operands go out first, then this lane's C values (4
   // consecutive rows of one column per (i, j)), whose latency then hides
   // behind the first slabs instead of delaying them
+  // one or two products (panels), each op.k deep: slab s of the run is slab
+  // s % nk of panel s / nk
+  const F16Parts part[2] = {{op.ahi, op.alo, op.bhi, op.blo, op.ra, op.rb},
+                            {op.ahi2, op.alo2, op.bhi2, op.blo2, op.ra2, op.rb2}};
+  const int nk = op.k / kF16BK, npan = op.ahi2 ? 2 : 1;
+  auto load = [&](int s, h8_t(&ra)[2 * NP], h8_t(&rb)[2 * NP]) {
+    f16_load<NP>(part[s >= nk ? 1 : 0], op.ld, i0, j0, (s >= nk ? s - nk : s) * kF16BK, tid, ra, rb);
+  };
   h8_t ra0[2 * NP], rb0[2 * NP];
-  f16_load<NP>(op, i0, j0, 0, tid, ra0, rb0);
+  load(0, ra0, rb0);
   f4_t cv[4][4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
@@ -205,29 +218,44 @@ __global__ __launch_bounds__(256) void update_f16x2_kernel(const F16UpdOp *__res
         }
       }
   };
-  const int nk = op.k / kF16BK;
+  // cv -= acc * (row factor x column factor) of panel q: the products with
+  // powers of two are exact, so this is the one rounding of an FP32 update
+  auto fold = [&](int q) {
+    f4_t rrow[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      rrow[i] = *gld(reinterpret_cast<const f4_t *>(part[q].ra + i0 + wm * 64 + i * 16 + 4 * lk));
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float rc = gld(part[q].rb)[j0 + wn * 64 + j * 16 + lr];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        cv[i][j] = cv[i][j] - acc[i][j] * (rrow[i] * rc);
+        acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+      }
+    }
+  };
+  const int ns = npan * nk;
   f16_store<NP, 2>(sm, 0, tid, ra0, rb0);
   __syncthreads();
-  for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    const bool more = kt + 1 < nk;
-    if (more) f16_load<NP>(op, i0, j0, (kt + 1) * kF16BK, tid, ra0, rb0);
+  for (int s = 0; s < ns; ++s) {
+    const int cur = s & 1;
+    const bool more = s + 1 < ns;
+    if (more) load(s + 1, ra0, rb0);
     compute(cur);
+    if (s + 1 == nk && npan == 2) fold(0);
     if (more) f16_store<NP, 2>(sm, cur ^ 1, tid, ra0, rb0);
     __syncthreads();
   }
+  fold(npan - 1);
   const bool diag = op.lower && bi == bj;
-  f4_t rrow[4];  // row factors (powers of two: the products below are exact)
-#pragma unroll
-  for (int i = 0; i < 4; ++i) rrow[i] = *gld(reinterpret_cast<const f4_t *>(op.ra + i0 + wm * 64 + i * 16 + 4 * lk));
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int col = j0 + wn * 64 + j * 16 + lr;
-    const float rc = gld(op.rb)[col];
     auto pc = gst(op.C + (long)col * op.ldc + i0 + wm * 64 + 4 * lk);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const f4_t v = cv[i][j] - acc[i][j] * (rrow[i] * rc);
+      const f4_t v = cv[i][j];
       const int row = i0 + wm * 64 + i * 16 + 4 * lk;  // rows row .. row + 3
       if (!diag || row >= col) {
         *reinterpret_cast<__attribute__((address_space(1))) f4_t *>(pc + i * 16) = v;

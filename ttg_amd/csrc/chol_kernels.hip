@@ -283,7 +283,6 @@ template <typename T, bool TRANS>
 __global__ __launch_bounds__(256) void trsm_kernel(const TrsmOp<T> *__restrict__ ops, TrsmOp<T> one, int bpo,
                                                    const int *__restrict__ d_abort) {
   if (d_abort && *d_abort) return;
-  __builtin_amdgcn_s_setprio(2);  // panel work: ahead of co-resident update waves
   const int o = blockIdx.x / bpo, s = blockIdx.x % bpo;
   const TrsmOp<T> op = ops ? ops[o] : one;
   const int r0 = s * 64;

@@ -48,8 +48,14 @@ struct TiledPlanT {
   unsigned flags = 0;
   int lowp = 0;  // float tiles only: 0 FP32 updates, else kLowF16x2 / kLowF16 / kLowBF16
   DevBuf d_ops, d_dinv, d_info, d_pw;
-  DevBuf d_panel, d_scale, d_f16ops;  // split-FP16 panels (two slots), scales, descriptors
+  DevBuf d_panel, d_scale, d_f16ops;  // split-FP16 panels (two slots, four when paired), scales, descriptors
   std::vector<int> split_off, split_cnt;
+  // paired schedule (split-FP16 modes with lookahead): per pair p of steps
+  // (2p, 2p+1), the eager update of column 2p+1 from panel 2p, then the
+  // two-panel updates of columns 2p+2, 2p+3 (bulkA) and of the rest (bulkB)
+  bool paired = false;
+  std::vector<int> pe_off, pe_cnt, pa_off, pa_cnt, pb_off, pb_cnt;
+  hipEvent_t ev_bulk = nullptr;
   size_t o_split = 0, o_f16 = 0;
   size_t o_potrf = 0, o_trsm = 0, o_upd = 0;
   std::vector<int> trsm_off, trsm_cnt, updA_off, updA_cnt, updB_off, updB_cnt;
@@ -63,7 +69,7 @@ struct TiledPlanT {
   ~TiledPlanT() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
-    for (hipEvent_t e : {ev_in, ev_out, ev_panel, ev_col, ev_diag})
+    for (hipEvent_t e : {ev_in, ev_out, ev_panel, ev_col, ev_diag, ev_bulk})
       if (e) (void)hipEventDestroy(e);
     if (s_main) (void)hipStreamDestroy(s_main);
     if (s_look) (void)hipStreamDestroy(s_look);
@@ -133,13 +139,15 @@ struct TiledPlanT {
     if (!upd.empty()) memcpy(host.data() + o_upd, upd.data(), upd.size() * sizeof(UpdOp<T>));
     TTGC_HIP(hipMemcpy(d_ops.ptr, host.data(), total, hipMemcpyHostToDevice));
     if (lowp) {
-      int rc = build_f16(upd);
+      const char *e = getenv("TTG_CHOL_PAIRED");
+      paired = (flags & TTG_CHOL_F_LOOKAHEAD) && NT > 2 && !(e && atoi(e) == 0);
+      int rc = paired ? build_f16_paired() : build_f16(upd);
       if (rc) return rc;
     }
     if (flags & TTG_CHOL_F_LOOKAHEAD) reserve_cus();
     if (!s_main) TTGC_HIP(hipStreamCreateWithFlags(&s_main, hipStreamNonBlocking));
     TTGC_HIP(hipStreamCreateWithFlags(&s_look, hipStreamNonBlocking));
-    for (hipEvent_t *e : {&ev_in, &ev_out, &ev_panel, &ev_col, &ev_diag})
+    for (hipEvent_t *e : {&ev_in, &ev_out, &ev_panel, &ev_col, &ev_diag, &ev_bulk})
       TTGC_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
     return TTG_CHOL_OK;
   }
@@ -204,6 +212,126 @@ struct TiledPlanT {
     return TTG_CHOL_OK;
   }
 
+  // Split-FP16 descriptors of the paired schedule: panel k in slot k % 4 (a
+  // pair's two panels stay intact while the next pair's are split); the
+  // two-panel updates carry panel 2p as the first product and 2p+1 as the
+  // second, so every tile still takes its updates in the order k = 0, 1, ...
+  int build_f16_paired() {
+    const size_t per = (size_t)NB * NB;
+    const size_t slot_tiles = NT > 1 ? NT - 1 : 1;
+    TTGC_HIP(d_panel.reserve(4 * slot_tiles * 2 * per * sizeof(uint16_t)));
+    TTGC_HIP(d_scale.reserve(4 * slot_tiles * NB * sizeof(float)));
+    uint16_t *panel = static_cast<uint16_t *>(d_panel.ptr);
+    float *scale = static_cast<float *>(d_scale.ptr);
+    auto hi = [&](int k, int m) { return panel + ((size_t)(k % 4) * slot_tiles + (m - k - 1)) * 2 * per; };
+    auto sc = [&](int k, int m) { return scale + ((size_t)(k % 4) * slot_tiles + (m - k - 1)) * NB; };
+    std::vector<SplitOp> sp;
+    std::vector<F16UpdOp> fu;
+    for (int k = 0; k < NT; ++k) {
+      split_off.push_back((int)sp.size());
+      for (int m = k + 1; m < NT; ++m)
+        sp.push_back(SplitOp{reinterpret_cast<const float *>(tile(m, k)), hi(k, m), hi(k, m) + per, sc(k, m)});
+      split_cnt.push_back((int)sp.size() - split_off.back());
+    }
+    auto one = [&](int k, int m, int n) {
+      fu.push_back(F16UpdOp{hi(k, m), hi(k, m) + per, hi(k, n), hi(k, n) + per, sc(k, m), sc(k, n),
+                            reinterpret_cast<float *>(tile(m, n)), NB, NB, NB, NB, m == n ? 1 : 0, nullptr, nullptr,
+                            nullptr, nullptr, nullptr, nullptr});
+    };
+    auto two = [&](int k, int m, int n) {
+      F16UpdOp o{hi(k, m), hi(k, m) + per, hi(k, n), hi(k, n) + per, sc(k, m), sc(k, n),
+                 reinterpret_cast<float *>(tile(m, n)), NB, NB, NB, NB, m == n ? 1 : 0,
+                 hi(k + 1, m), hi(k + 1, m) + per, hi(k + 1, n), hi(k + 1, n) + per, sc(k + 1, m), sc(k + 1, n)};
+      fu.push_back(o);
+    };
+    for (int k = 0; k + 1 < NT; k += 2) {
+      pe_off.push_back((int)fu.size());
+      for (int m = k + 1; m < NT; ++m) one(k, m, k + 1);
+      pe_cnt.push_back((int)fu.size() - pe_off.back());
+      pa_off.push_back((int)fu.size());
+      for (int n = k + 2; n < std::min(k + 4, NT); ++n)
+        for (int m = n; m < NT; ++m) two(k, m, n);
+      pa_cnt.push_back((int)fu.size() - pa_off.back());
+      pb_off.push_back((int)fu.size());
+      updB_tiles(k + 2, [&](int m, int n) { two(k, m, n); });  // columns >= k + 4, 2 x 2 groups
+      pb_cnt.push_back((int)fu.size() - pb_off.back());
+    }
+    auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+    o_split = 0;
+    o_f16 = align(sp.size() * sizeof(SplitOp));
+    const size_t total = align(o_f16 + fu.size() * sizeof(F16UpdOp));
+    TTGC_HIP(d_f16ops.reserve(total));
+    std::vector<char> host(total);
+    if (!sp.empty()) memcpy(host.data() + o_split, sp.data(), sp.size() * sizeof(SplitOp));
+    if (!fu.empty()) memcpy(host.data() + o_f16, fu.data(), fu.size() * sizeof(F16UpdOp));
+    TTGC_HIP(hipMemcpy(d_f16ops.ptr, host.data(), total, hipMemcpyHostToDevice));
+    return TTG_CHOL_OK;
+  }
+
+  // The paired schedule (stream by stream):
+  //   s_diag: POTRF(2p) after bulkA(p-1); POTRF(2p+1) after the eager update
+  //   s_look: TRSM + split of 2p, the eager update of column 2p+1, TRSM +
+  //           split of 2p+1
+  //   s_main: bulkA(p) after the pair's panels, then bulkB(p)
+  // bulkB(p-1) overlaps the whole panel chain of pair p; it writes columns
+  // >= 2p+2 and reads the panels in slots (2p-2) % 4 and (2p-1) % 4, which the
+  // chain of pair p neither writes nor splits into.
+  int enqueue_paired(int *info, bool prof) {
+    const double b3 = (double)NB * NB * NB;
+    PotrfOpT<T> pnone{};
+    TrsmOp<T> tnone{};
+    TTGC_HIP(launch_prepare_t<T>(s_main, N, NB, tiles, info));
+    hipStream_t sd = s_diag ? s_diag : s_look;
+    auto dep = [&](hipEvent_t e, hipStream_t from, hipStream_t to) -> int {
+      if (from == to) return TTG_CHOL_OK;
+      TTGC_HIP(hipEventRecord(e, from));
+      TTGC_HIP(hipStreamWaitEvent(to, e, 0));
+      return TTG_CHOL_OK;
+    };
+    auto potrf = [&](int k) -> int {
+      const int slot = prof ? prof_begin(sd) : -1;
+      TTGC_HIP(launch_potrf<T>(sd, potrf_ops() + k, 1, pnone, info));
+      if (prof) prof_end(slot, sd, PROF_POTRF, b3 / 3.0);
+      return dep(ev_diag, sd, s_look);
+    };
+    auto panel = [&](int k) -> int {
+      if (!trsm_cnt[k]) return TTG_CHOL_OK;
+      const int slot = prof ? prof_begin(s_look) : -1;
+      TTGC_HIP(launch_trsm<T>(s_look, trsm_ops() + trsm_off[k], trsm_cnt[k], tnone, NB, info));
+      if (prof) prof_end(slot, s_look, PROF_TRSM, trsm_cnt[k] * b3);
+      TTGC_HIP(launch_split(s_look, split_ops() + split_off[k], split_cnt[k], NB, lowp));
+      return TTG_CHOL_OK;
+    };
+    // tile products of a launch: GEMM tiles 2 b^3, SYRK tiles b^2 (b+1), per panel
+    auto upd = [&](hipStream_t st, int off, int cnt, int nsyrk, int npan) -> int {
+      if (!cnt) return TTG_CHOL_OK;
+      const int slot = prof ? prof_begin(st) : -1;
+      TTGC_HIP(launch_update_f16x2(st, f16_ops() + off, cnt, NB, info, lowp));
+      if (prof)
+        prof_end(slot, st, PROF_UPDATE, npan * ((cnt - nsyrk) * 2.0 * b3 + nsyrk * (double)NB * NB * (NB + 1)));
+      return TTG_CHOL_OK;
+    };
+    if (int rc = dep(ev_col, s_main, sd)) return rc;
+    for (int p = 0, k = 0; k < NT; ++p, k += 2) {
+      if (p > 0) TTGC_HIP(hipStreamWaitEvent(sd, ev_bulk, 0));  // columns 2p, 2p+1 complete (bulkA(p-1))
+      if (int rc = potrf(k)) return rc;
+      if (int rc = panel(k)) return rc;
+      if (k + 1 >= NT) break;
+      if (int rc = upd(s_look, pe_off[p], pe_cnt[p], 1, 1)) return rc;
+      if (int rc = dep(ev_col, s_look, sd)) return rc;
+      if (int rc = potrf(k + 1)) return rc;
+      if (int rc = panel(k + 1)) return rc;
+      if (int rc = dep(ev_panel, s_look, s_main)) return rc;
+      if (int rc = upd(s_main, pa_off[p], pa_cnt[p], std::min(2, NT - k - 2), 2)) return rc;
+      TTGC_HIP(hipEventRecord(ev_bulk, s_main));
+      const int nb = NT - k - 4;  // columns of bulkB: its SYRK tiles
+      if (int rc = upd(s_main, pb_off[p], pb_cnt[p], nb > 0 ? nb : 0, 2)) return rc;
+    }
+    if (int rc = dep(ev_panel, s_look, s_main)) return rc;
+    if (int rc = dep(ev_diag, sd, s_main)) return rc;
+    return TTG_CHOL_OK;
+  }
+
   // Point the POTRF descriptors at `info` (host round trip only when it changes).
   int bind_info(int *info) {
     if (desc_info == info) return TTG_CHOL_OK;
@@ -218,6 +346,7 @@ struct TiledPlanT {
   // Enqueue the whole factorization on s_main (and s_look), writing *info.
   // With `prof`, every launch is bracketed by timing events.
   int enqueue(int *info, bool prof) {
+    if (paired) return enqueue_paired(info, prof);
     const bool fast = NB % 128 == 0;
     const T m1 = T(-1), p1 = T(1);
     const double b3 = (double)NB * NB * NB;
