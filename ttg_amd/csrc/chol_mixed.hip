@@ -252,7 +252,7 @@ struct MixedPlan : MixedPlanBase {
     double e = INFINITY;
     int it = 0;
     for (;; ++it) {
-      rc = residual_fp64(s, N, NB, A, nrhs, X, ldx, B, ldb, r(), N, false);
+      rc = residual_fp64(s, N, NB, A, nrhs, X, ldx, B, ldb, r(), N);
       if (rc) return rc;
       hipLaunchKernelGGL(berr_kernel, dim3(nrhs), dim3(256), 0, s, N, r(), (long)N, X, ldx, B, ldb, st(), st() + 1);
       TTGC_HIP(hipGetLastError());

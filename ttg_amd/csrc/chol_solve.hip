@@ -442,7 +442,6 @@ __global__ void residual_rows_kernel(int N, int Np, const double *__restrict__ X
 // read from HBM once), stored row-major to the tile's slots W[t][0|1][NB][16].
 // Wave w owns rows w*16..w*16+15 of every 64-row block for P and of every
 // 64-column block for Q.
-template <bool ABS>
 __global__ __launch_bounds__(256) void residual_tile_kernel(int N, int NB, int nt, const double *__restrict__ tiles,
                                                             const double *__restrict__ Xr, double *__restrict__ W) {
   const long t = blockIdx.x;
@@ -502,7 +501,7 @@ __global__ __launch_bounds__(256) void residual_tile_kernel(int N, int NB, int n
       const int e = tid + 256 * i, r = e & 63, c = e >> 6;
       const int gr = rs * kRB + r, gc = cs * kRB + c;
       const bool ok = gr < mrows && gc < ncols && (!diag || gr >= gc);
-      blk[c][r] = ok ? (ABS ? fabs(av[i]) : av[i]) : 0.0;
+      blk[c][r] = ok ? av[i] : 0.0;
     }
   };
   int ncs = 0, nrs = 0;  // the block whose elements are in av
@@ -594,7 +593,7 @@ static std::mutex g_res_mu;
 static std::map<hipStream_t, std::unique_ptr<DevBuf>> g_res_ws;
 
 int residual_fp64(hipStream_t s, int N, int NB, const double *tiles, int nrhs, const double *X, long ldx,
-                  const double *B, long ldb, double *R, long ldr, bool abs_a) {
+                  const double *B, long ldb, double *R, long ldr) {
   const int nt = (N + NB - 1) / NB;
   const long Np = (long)nt * NB;
   const long ntiles = (long)nt * (nt + 1) / 2;
@@ -614,11 +613,7 @@ int residual_fp64(hipStream_t s, int N, int NB, const double *tiles, int nrhs, c
     hipLaunchKernelGGL(residual_rows_kernel, dim3((unsigned)((Np * kRhsChunk + 255) / 256)), dim3(256), 0, s, N,
                        (int)Np, X + (long)j0 * ldx, ldx, nr, Xr);
     TTGC_HIP(hipGetLastError());
-    if (abs_a)
-      hipLaunchKernelGGL(residual_tile_kernel<true>, dim3((unsigned)ntiles), dim3(256), 0, s, N, NB, nt, tiles, Xr, W);
-    else
-      hipLaunchKernelGGL(residual_tile_kernel<false>, dim3((unsigned)ntiles), dim3(256), 0, s, N, NB, nt, tiles, Xr,
-                         W);
+    hipLaunchKernelGGL(residual_tile_kernel, dim3((unsigned)ntiles), dim3(256), 0, s, N, NB, nt, tiles, Xr, W);
     TTGC_HIP(hipGetLastError());
     const long n = (long)N * nr;
     hipLaunchKernelGGL(residual_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, N, NB, nt, W, nr,
@@ -673,7 +668,7 @@ int ttg_chol_dresidual_tiled(void *stream, int N, int NB, const double *tiles, i
     return TTG_CHOL_EINVAL;
   }
   if (N == 0 || nrhs == 0) return TTG_CHOL_OK;
-  return residual_fp64(as_stream(stream), N, NB, tiles, nrhs, X, ldx, B, ldb, R, ldr, false);
+  return residual_fp64(as_stream(stream), N, NB, tiles, nrhs, X, ldx, B, ldb, R, ldr);
 }
 
 }  // extern "C"

@@ -50,8 +50,8 @@ hipError_t launch_to_rows(hipStream_t s, int N, int Np, int nrhs, const TS *B, l
 template <typename TS, typename TD>
 hipError_t launch_from_rows(hipStream_t s, int N, int nrhs, const TS *Bt, TD *B, long ldb);
 
-// R := B - A X (abs_a: B - |A| X, for row sums of |A|)
+// R := B - A X
 int residual_fp64(hipStream_t s, int N, int NB, const double *tiles, int nrhs, const double *X, long ldx,
-                  const double *B, long ldb, double *R, long ldr, bool abs_a = false);
+                  const double *B, long ldb, double *R, long ldr);
 
 }  // namespace ttgc
