@@ -169,13 +169,16 @@ __global__ __launch_bounds__(256, 2) void update_f16x2_kernel(const F16UpdOp *__
   // the first slab's operands go out first, then this lane's C values (4
   // consecutive rows of one column per (i, j)), whose latency then hides
   // behind the first slabs instead of delaying them
-  // one or two products (panels), each op.k deep: slab s of the run is slab
-  // s % nk of panel s / nk
-  const F16Parts part[2] = {{op.ahi, op.alo, op.bhi, op.blo, op.ra, op.rb},
-                            {op.ahi2, op.alo2, op.bhi2, op.blo2, op.ra2, op.rb2}};
-  const int nk = op.k / kF16BK, npan = op.ahi2 ? 2 : 1;
+  // npan products (panels), each op.k deep: slab s of the run is slab
+  // s % nk of panel s / nk (a wave-uniform index)
+  const int nk = op.k / kF16BK, npan = op.npan > 1 ? op.npan : 1;
+  auto part = [&](int q) {
+    const long po = q * op.pstride, so = q * op.sstride;
+    return F16Parts{op.ahi + po, op.alo + po, op.bhi + po, op.blo + po, op.ra + so, op.rb + so};
+  };
   auto load = [&](int s, h8_t(&ra)[2 * NP], h8_t(&rb)[2 * NP]) {
-    f16_load<NP>(part[s >= nk ? 1 : 0], op.ld, i0, j0, (s >= nk ? s - nk : s) * kF16BK, tid, ra, rb);
+    const int q = s / nk;
+    f16_load<NP>(part(q), op.ld, i0, j0, (s - q * nk) * kF16BK, tid, ra, rb);
   };
   h8_t ra0[2 * NP], rb0[2 * NP];
   load(0, ra0, rb0);
@@ -221,13 +224,14 @@ __global__ __launch_bounds__(256, 2) void update_f16x2_kernel(const F16UpdOp *__
   // cv -= acc * (row factor x column factor) of panel q: the products with
   // powers of two are exact, so this is the one rounding of an FP32 update
   auto fold = [&](int q) {
+    const F16Parts pq = part(q);
     f4_t rrow[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      rrow[i] = *gld(reinterpret_cast<const f4_t *>(part[q].ra + i0 + wm * 64 + i * 16 + 4 * lk));
+      rrow[i] = *gld(reinterpret_cast<const f4_t *>(pq.ra + i0 + wm * 64 + i * 16 + 4 * lk));
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const float rc = gld(part[q].rb)[j0 + wn * 64 + j * 16 + lr];
+      const float rc = gld(pq.rb)[j0 + wn * 64 + j * 16 + lr];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         cv[i][j] = cv[i][j] - acc[i][j] * (rrow[i] * rc);
@@ -243,7 +247,7 @@ __global__ __launch_bounds__(256, 2) void update_f16x2_kernel(const F16UpdOp *__
     const bool more = s + 1 < ns;
     if (more) load(s + 1, ra0, rb0);
     compute(cur);
-    if (s + 1 == nk && npan == 2) fold(0);
+    if ((s + 1) % nk == 0 && s + 1 < ns) fold(s / nk);  // a panel done, more to come
     if (more) f16_store<NP, 2>(sm, cur ^ 1, tid, ra0, rb0);
     __syncthreads();
   }

@@ -19,18 +19,18 @@ struct SplitOp {
 // C (FP32 tile, column-major) -= diag(ra) (A_hi B_hi^T + A_hi B_lo^T + A_lo B_hi^T) diag(rb)
 // for n x n tiles with inner dimension k; A/B parts are k-slab-major with ld
 // rows per slab; ra / rb are the row factors of the A and B splits.  With
-// ahi2 set, a second product of the same shape (the next panel, its own
-// parts and row factors) is subtracted after the first, in the same pass
-// over C: C -= (first) then C -= (second), each rounded to FP32 as two
-// separate updates would be.
+// npan > 1, the products of npan consecutive panels are subtracted one
+// after the other in the same pass over C, each rounded to FP32 as separate
+// updates would be; panel j's parts sit pstride halves and its row factors
+// sstride floats after panel 0's.
 struct F16UpdOp {
   const uint16_t *ahi, *alo, *bhi, *blo;
   const float *ra, *rb;
   float *C;
   int n, k, ld, ldc;
   int lower;
-  const uint16_t *ahi2, *alo2, *bhi2, *blo2;  // second panel (null: one product)
-  const float *ra2, *rb2;
+  int npan;  // 0 or 1: one product
+  long pstride, sstride;
 };
 
 // low-precision trailing-update modes

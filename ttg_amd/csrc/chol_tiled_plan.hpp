@@ -50,11 +50,13 @@ struct TiledPlanT {
   DevBuf d_ops, d_dinv, d_info, d_pw;
   DevBuf d_panel, d_scale, d_f16ops;  // split-FP16 panels (two slots, four when paired), scales, descriptors
   std::vector<int> split_off, split_cnt;
-  // paired schedule (split-FP16 modes with lookahead): per pair p of steps
-  // (2p, 2p+1), the eager update of column 2p+1 from panel 2p, then the
-  // two-panel updates of columns 2p+2, 2p+3 (bulkA) and of the rest (bulkB)
+  // grouped schedule (split-FP16 modes with lookahead): the steps in groups
+  // of G; per step k of a group, the eager update of the group's later
+  // columns from panel k; per group, the G-panel updates of the next group's
+  // columns (bulkA) and of all further columns (bulkB)
   bool paired = false;
-  std::vector<int> pe_off, pe_cnt, pa_off, pa_cnt, pb_off, pb_cnt;
+  int G = 2;
+  std::vector<int> pe_off, pe_cnt, pa_off, pa_cnt, pb_off, pb_cnt;  // eager per step; bulkA, bulkB per group
   hipEvent_t ev_bulk = nullptr;
   size_t o_split = 0, o_f16 = 0;
   size_t o_potrf = 0, o_trsm = 0, o_upd = 0;
@@ -140,7 +142,9 @@ struct TiledPlanT {
     TTGC_HIP(hipMemcpy(d_ops.ptr, host.data(), total, hipMemcpyHostToDevice));
     if (lowp) {
       const char *e = getenv("TTG_CHOL_PAIRED");
-      paired = (flags & TTG_CHOL_F_LOOKAHEAD) && NT > 2 && !(e && atoi(e) == 0);
+      const char *eg = getenv("TTG_CHOL_GROUP");
+      G = eg ? std::max(2, std::min(8, atoi(eg))) : 2;
+      paired = (flags & TTG_CHOL_F_LOOKAHEAD) && NT > G && !(e && atoi(e) == 0);
       int rc = paired ? build_f16_paired() : build_f16(upd);
       if (rc) return rc;
     }
@@ -212,19 +216,25 @@ struct TiledPlanT {
     return TTG_CHOL_OK;
   }
 
-  // Split-FP16 descriptors of the paired schedule: panel k in slot k % 4 (a
-  // pair's two panels stay intact while the next pair's are split); the
-  // two-panel updates carry panel 2p as the first product and 2p+1 as the
-  // second, so every tile still takes its updates in the order k = 0, 1, ...
+  // Split-FP16 descriptors of the grouped schedule: panel k in slot k % 2G, so
+  // a group's G panels fill G consecutive slots (groups start at multiples
+  // of G) and stay intact while the next group splits into the other G.
+  // Panel k0 + j of a group then sits a fixed stride after panel k0 for
+  // every tile (the slot advances by one, the tile position by minus one),
+  // which is what a G-panel update descriptor carries.  Its products are
+  // subtracted in panel order, so every tile still takes its updates in the
+  // order k = 0, 1, ...
   int build_f16_paired() {
     const size_t per = (size_t)NB * NB;
     const size_t slot_tiles = NT > 1 ? NT - 1 : 1;
-    TTGC_HIP(d_panel.reserve(4 * slot_tiles * 2 * per * sizeof(uint16_t)));
-    TTGC_HIP(d_scale.reserve(4 * slot_tiles * NB * sizeof(float)));
+    const int S = 2 * G;
+    TTGC_HIP(d_panel.reserve(S * slot_tiles * 2 * per * sizeof(uint16_t)));
+    TTGC_HIP(d_scale.reserve(S * slot_tiles * NB * sizeof(float)));
     uint16_t *panel = static_cast<uint16_t *>(d_panel.ptr);
     float *scale = static_cast<float *>(d_scale.ptr);
-    auto hi = [&](int k, int m) { return panel + ((size_t)(k % 4) * slot_tiles + (m - k - 1)) * 2 * per; };
-    auto sc = [&](int k, int m) { return scale + ((size_t)(k % 4) * slot_tiles + (m - k - 1)) * NB; };
+    auto hi = [&](int k, int m) { return panel + ((size_t)(k % S) * slot_tiles + (m - k - 1)) * 2 * per; };
+    auto sc = [&](int k, int m) { return scale + ((size_t)(k % S) * slot_tiles + (m - k - 1)) * NB; };
+    const long pstride = (long)(slot_tiles - 1) * 2 * (long)per, sstride = (long)(slot_tiles - 1) * NB;
     std::vector<SplitOp> sp;
     std::vector<F16UpdOp> fu;
     for (int k = 0; k < NT; ++k) {
@@ -233,27 +243,27 @@ struct TiledPlanT {
         sp.push_back(SplitOp{reinterpret_cast<const float *>(tile(m, k)), hi(k, m), hi(k, m) + per, sc(k, m)});
       split_cnt.push_back((int)sp.size() - split_off.back());
     }
-    auto one = [&](int k, int m, int n) {
+    // tile (m, n) updated by panels k .. k + np - 1
+    auto op = [&](int k, int np, int m, int n) {
       fu.push_back(F16UpdOp{hi(k, m), hi(k, m) + per, hi(k, n), hi(k, n) + per, sc(k, m), sc(k, n),
-                            reinterpret_cast<float *>(tile(m, n)), NB, NB, NB, NB, m == n ? 1 : 0, nullptr, nullptr,
-                            nullptr, nullptr, nullptr, nullptr});
+                            reinterpret_cast<float *>(tile(m, n)), NB, NB, NB, NB, m == n ? 1 : 0, np,
+                            np > 1 ? pstride : 0, np > 1 ? sstride : 0});
     };
-    auto two = [&](int k, int m, int n) {
-      F16UpdOp o{hi(k, m), hi(k, m) + per, hi(k, n), hi(k, n) + per, sc(k, m), sc(k, n),
-                 reinterpret_cast<float *>(tile(m, n)), NB, NB, NB, NB, m == n ? 1 : 0,
-                 hi(k + 1, m), hi(k + 1, m) + per, hi(k + 1, n), hi(k + 1, n) + per, sc(k + 1, m), sc(k + 1, n)};
-      fu.push_back(o);
-    };
-    for (int k = 0; k + 1 < NT; k += 2) {
-      pe_off.push_back((int)fu.size());
-      for (int m = k + 1; m < NT; ++m) one(k, m, k + 1);
-      pe_cnt.push_back((int)fu.size() - pe_off.back());
+    for (int k0 = 0; k0 < NT; k0 += G) {
+      const int ge = std::min(k0 + G, NT);  // the group's steps: k0 .. ge - 1
+      for (int k = k0; k < ge; ++k) {
+        pe_off.push_back((int)fu.size());
+        for (int n = k + 1; n < ge; ++n)
+          for (int m = n; m < NT; ++m) op(k, 1, m, n);
+        pe_cnt.push_back((int)fu.size() - pe_off.back());
+      }
+      const int np = ge - k0;
       pa_off.push_back((int)fu.size());
-      for (int n = k + 2; n < std::min(k + 4, NT); ++n)
-        for (int m = n; m < NT; ++m) two(k, m, n);
+      for (int n = ge; n < std::min(ge + G, NT); ++n)
+        for (int m = n; m < NT; ++m) op(k0, np, m, n);
       pa_cnt.push_back((int)fu.size() - pa_off.back());
       pb_off.push_back((int)fu.size());
-      updB_tiles(k + 2, [&](int m, int n) { two(k, m, n); });  // columns >= k + 4, 2 x 2 groups
+      if (ge + G < NT) updB_tiles(ge + G - 2, [&](int m, int n) { op(k0, np, m, n); });  // columns >= ge + G
       pb_cnt.push_back((int)fu.size() - pb_off.back());
     }
     auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
@@ -268,14 +278,13 @@ struct TiledPlanT {
     return TTG_CHOL_OK;
   }
 
-  // The paired schedule (stream by stream):
-  //   s_diag: POTRF(2p) after bulkA(p-1); POTRF(2p+1) after the eager update
-  //   s_look: TRSM + split of 2p, the eager update of column 2p+1, TRSM +
-  //           split of 2p+1
-  //   s_main: bulkA(p) after the pair's panels, then bulkB(p)
-  // bulkB(p-1) overlaps the whole panel chain of pair p; it writes columns
-  // >= 2p+2 and reads the panels in slots (2p-2) % 4 and (2p-1) % 4, which the
-  // chain of pair p neither writes nor splits into.
+  // The grouped schedule (stream by stream), group g = steps k0 .. k0+G-1:
+  //   s_diag: POTRF(k0) after bulkA(g-1); POTRF(k) after the eager update of step k-1
+  //   s_look: per step, TRSM + split, then the eager update of the group's later columns
+  //   s_main: bulkA(g) after the group's panels, then bulkB(g)
+  // bulkB(g-1) overlaps the whole panel chain of group g; it writes columns
+  // >= k0 + G and reads the panels of group g-1, which the chain of group g
+  // neither writes nor splits into.
   int enqueue_paired(int *info, bool prof) {
     const double b3 = (double)NB * NB * NB;
     PotrfOpT<T> pnone{};
@@ -312,20 +321,22 @@ struct TiledPlanT {
       return TTG_CHOL_OK;
     };
     if (int rc = dep(ev_col, s_main, sd)) return rc;
-    for (int p = 0, k = 0; k < NT; ++p, k += 2) {
-      if (p > 0) TTGC_HIP(hipStreamWaitEvent(sd, ev_bulk, 0));  // columns 2p, 2p+1 complete (bulkA(p-1))
-      if (int rc = potrf(k)) return rc;
-      if (int rc = panel(k)) return rc;
-      if (k + 1 >= NT) break;
-      if (int rc = upd(s_look, pe_off[p], pe_cnt[p], 1, 1)) return rc;
-      if (int rc = dep(ev_col, s_look, sd)) return rc;
-      if (int rc = potrf(k + 1)) return rc;
-      if (int rc = panel(k + 1)) return rc;
+    for (int g = 0, k0 = 0; k0 < NT; ++g, k0 += G) {
+      const int ge = std::min(k0 + G, NT);
+      if (g > 0) TTGC_HIP(hipStreamWaitEvent(sd, ev_bulk, 0));  // the group's columns complete (bulkA(g-1))
+      for (int k = k0; k < ge; ++k) {
+        if (k > k0)
+          if (int rc = dep(ev_col, s_look, sd)) return rc;  // column k complete (eager update of step k-1)
+        if (int rc = potrf(k)) return rc;
+        if (int rc = panel(k)) return rc;
+        if (int rc = upd(s_look, pe_off[k], pe_cnt[k], ge - k - 1, 1)) return rc;
+      }
+      if (ge >= NT) break;
       if (int rc = dep(ev_panel, s_look, s_main)) return rc;
-      if (int rc = upd(s_main, pa_off[p], pa_cnt[p], std::min(2, NT - k - 2), 2)) return rc;
+      if (int rc = upd(s_main, pa_off[g], pa_cnt[g], std::min(G, NT - ge), ge - k0)) return rc;
       TTGC_HIP(hipEventRecord(ev_bulk, s_main));
-      const int nb = NT - k - 4;  // columns of bulkB: its SYRK tiles
-      if (int rc = upd(s_main, pb_off[p], pb_cnt[p], nb > 0 ? nb : 0, 2)) return rc;
+      const int nb = NT - ge - G;  // columns of bulkB: its SYRK tiles
+      if (int rc = upd(s_main, pb_off[g], pb_cnt[g], nb > 0 ? nb : 0, ge - k0)) return rc;
     }
     if (int rc = dep(ev_panel, s_look, s_main)) return rc;
     if (int rc = dep(ev_diag, sd, s_main)) return rc;
