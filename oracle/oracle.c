@@ -869,3 +869,228 @@ double oracle_solve_backward_error(int N, int NB, const double *A, int nrhs, con
   free(R);
   return worst;
 }
+
+/* ------------------------------------------------------------------ */
+/* POINV family: TRTRI (lower, non-unit), LAUUM (lower), POTRI         */
+/* ------------------------------------------------------------------ */
+
+/* B := alpha B L^{-1}  (Right, Lower, NoTrans, NonUnit): X L = alpha B is
+ * solved column by column from the right. */
+void oracle_dtrsm_rln(int m, int n, double alpha, const double *L, int ldl, double *B, int ldb) {
+  for (int k = n - 1; k >= 0; k--) {
+    double *bk = B + (long)k * ldb;
+    for (int i = 0; i < m; i++) bk[i] *= alpha;
+    for (int j = k + 1; j < n; j++) {
+      double f = L[j + (long)k * ldl];
+      if (f != 0.0) {
+        const double *bj = B + (long)j * ldb;
+        for (int i = 0; i < m; i++) bk[i] -= f * bj[i];
+      }
+    }
+    double t = 1.0 / L[k + (long)k * ldl];
+    for (int i = 0; i < m; i++) bk[i] *= t;
+  }
+}
+
+/* B := L^{-1} B  (Left, Lower, NoTrans, NonUnit): forward substitution. */
+void oracle_dtrsm_lln(int m, int n, const double *L, int ldl, double *B, int ldb) {
+  for (int j = 0; j < n; j++) {
+    double *b = B + (long)j * ldb;
+    for (int i = 0; i < m; i++) {
+      double s = b[i];
+      for (int l = 0; l < i; l++) s -= L[i + (long)l * ldl] * b[l];
+      b[i] = s / L[i + (long)i * ldl];
+    }
+  }
+}
+
+/* B := L^T B  (Left, Lower, Trans, NonUnit); row i of the result only reads
+ * rows >= i of B, so rows are overwritten top down. */
+void oracle_dtrmm_llt(int m, int n, const double *L, int ldl, double *B, int ldb) {
+  for (int j = 0; j < n; j++) {
+    double *b = B + (long)j * ldb;
+    for (int i = 0; i < m; i++) {
+      double s = 0.0;
+      for (int k = i; k < m; k++) s += L[k + (long)i * ldl] * b[k];
+      b[i] = s;
+    }
+  }
+}
+
+/* C := alpha A^T A + beta C, lower triangle of the n x n C; A is k x n. */
+void oracle_dsyrk_lt(int n, int k, double alpha, const double *A, int lda, double beta, double *C, int ldc) {
+  for (int j = 0; j < n; j++)
+    for (int i = j; i < n; i++) {
+      double s = 0.0;
+      for (int l = 0; l < k; l++) s += A[l + (long)i * lda] * A[l + (long)j * lda];
+      double c = (beta == 0.0) ? 0.0 : beta * C[i + (long)j * ldc];
+      C[i + (long)j * ldc] = c + alpha * s;
+    }
+}
+
+/* C := alpha op(A) B + beta C with op(A) = A (ta = 0, A m x k) or A^T (ta = 1,
+ * A k x m); B is k x n. */
+void oracle_dgemm_xn(int ta, int m, int n, int k, double alpha, const double *A, int lda, const double *B, int ldb,
+                     double beta, double *C, int ldc) {
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < m; i++) {
+      double s = 0.0;
+      for (int l = 0; l < k; l++)
+        s += (ta ? A[l + (long)i * lda] : A[i + (long)l * lda]) * B[l + (long)j * ldb];
+      double c = (beta == 0.0) ? 0.0 : beta * C[i + (long)j * ldc];
+      C[i + (long)j * ldc] = c + alpha * s;
+    }
+}
+
+/* In-place inverse of a lower triangular, non-unit n x n tile (LAPACK dtrti2,
+ * lower: columns from the right, each multiplied by the already inverted
+ * trailing block).  Returns 0, or the 1-based index of the first zero on the
+ * diagonal, in which case A is left untouched (LAPACK dtrtri's check). */
+int oracle_dtrtri_tile(int n, double *A, int lda) {
+  for (int j = 0; j < n; j++)
+    if (A[j + (long)j * lda] == 0.0) return j + 1;
+  for (int j = n - 1; j >= 0; j--) {
+    A[j + (long)j * lda] = 1.0 / A[j + (long)j * lda];
+    double ajj = -A[j + (long)j * lda];
+    /* x := T x, T = inverted block (j+1.., j+1..), x = column j below the diagonal */
+    for (int i = n - 1; i > j; i--) {
+      double s = 0.0;
+      for (int l = j + 1; l <= i; l++) s += A[i + (long)l * lda] * A[l + (long)j * lda];
+      A[i + (long)j * lda] = s;
+    }
+    for (int i = j + 1; i < n; i++) A[i + (long)j * lda] *= ajj;
+  }
+  return 0;
+}
+
+/* Lower part of L^T L in place (LAPACK dlauum, lower).  Row i of the result
+ * reads rows >= i, so each row is formed in a work row, then stored. */
+void oracle_dlauum_tile(int n, double *A, int lda) {
+  double *row = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  for (int i = 0; i < n; i++) {
+    for (int j = 0; j <= i; j++) {
+      double s = 0.0;
+      for (int k = i; k < n; k++) s += A[k + (long)i * lda] * A[k + (long)j * lda];
+      row[j] = s;
+    }
+    for (int j = 0; j <= i; j++) A[i + (long)j * lda] = row[j];
+  }
+  free(row);
+}
+
+/* L := L^{-1} over the lower tiles, in the task order of
+ * examples/potrf/trtri_L.h: at step K, TRSMR(K, M) A(M,K) := -A(M,K) L_KK^{-1}
+ * (:96-104), GEMM(K, M, N) A(M,N) += A(M,K) A(K,N) for N < K (:178-185),
+ * TRSML(K, N) A(K,N) := L_KK^{-1} A(K,N) (:49-56), TRTRI(K) (:27).  The
+ * GEMMs read A(K,N) before its TRSML and both TRSMs read the original
+ * L_KK, as in the graph.  Returns 0 or the global 1-based index of the first
+ * zero diagonal entry (nothing is changed then). */
+int oracle_dtrtri_tiled(int N, int NB, double *T) {
+  int nt = (N + NB - 1) / NB;
+  for (int K = 0; K < nt; K++) {
+    const double *d = tile_ptr(T, K, K, nt, NB);
+    for (int j = 0; j < tile_dim(N, NB, K); j++)
+      if (d[j + (long)j * NB] == 0.0) return K * NB + j + 1;
+  }
+  for (int K = 0; K < nt; K++) {
+    int dk = tile_dim(N, NB, K);
+    double *Lkk = tile_ptr(T, K, K, nt, NB);
+    for (int M = K + 1; M < nt; M++)
+      oracle_dtrsm_rln(tile_dim(N, NB, M), dk, -1.0, Lkk, NB, tile_ptr(T, M, K, nt, NB), NB);
+    for (int M = K + 1; M < nt; M++)
+      for (int Nn = 0; Nn < K; Nn++)
+        oracle_dgemm_xn(0, tile_dim(N, NB, M), tile_dim(N, NB, Nn), dk, 1.0, tile_ptr(T, M, K, nt, NB), NB,
+                        tile_ptr(T, K, Nn, nt, NB), NB, 1.0, tile_ptr(T, M, Nn, nt, NB), NB);
+    for (int Nn = 0; Nn < K; Nn++)
+      oracle_dtrsm_lln(dk, tile_dim(N, NB, Nn), Lkk, NB, tile_ptr(T, K, Nn, nt, NB), NB);
+    oracle_dtrtri_tile(dk, Lkk, NB);
+  }
+  return 0;
+}
+
+/* A := L^T L (lower tiles), in the task order of examples/potrf/lauum.h:
+ * tile (M, N), N < M, is TRMM(M, N) A(M,N) := L_MM^T A(M,N) (:72-80) followed
+ * by GEMM(K, N, M) A(M,N) += L_KM^T L_KN for K = M+1, M+2, ... (:172-180);
+ * tile (M, M) is LAUUM(M) (:37) followed by SYRK(K, M) A(M,M) += L_KM^T L_KM
+ * (:123-128).  Every task reads the original L (the dispatcher's tiles are
+ * read-only), so a tile row is formed in work tiles and stored after the
+ * whole row is done; later rows never read it. */
+void oracle_dlauum_tiled(int N, int NB, double *T) {
+  int nt = (N + NB - 1) / NB;
+  size_t per = (size_t)NB * NB;
+  double *row = (double *)malloc(sizeof(double) * per * (size_t)(nt > 0 ? nt : 1));
+  for (int M = 0; M < nt; M++) {
+    int dm = tile_dim(N, NB, M);
+    const double *Lmm = tile_ptr(T, M, M, nt, NB);
+    for (int Nn = 0; Nn <= M; Nn++) {
+      int dn = tile_dim(N, NB, Nn);
+      double *X = row + per * Nn;
+      const double *Lmn = tile_ptr(T, M, Nn, nt, NB);
+      for (int j = 0; j < dn; j++)
+        for (int i = 0; i < dm; i++) X[i + (long)j * NB] = Lmn[i + (long)j * NB];
+      if (Nn < M) {
+        oracle_dtrmm_llt(dm, dn, Lmm, NB, X, NB);
+        for (int K = M + 1; K < nt; K++)
+          oracle_dgemm_xn(1, dm, dn, tile_dim(N, NB, K), 1.0, tile_ptr(T, K, M, nt, NB), NB,
+                          tile_ptr(T, K, Nn, nt, NB), NB, 1.0, X, NB);
+      } else {
+        oracle_dlauum_tile(dm, X, NB);
+        for (int K = M + 1; K < nt; K++)
+          oracle_dsyrk_lt(dm, tile_dim(N, NB, K), 1.0, tile_ptr(T, K, M, nt, NB), NB, 1.0, X, NB);
+      }
+    }
+    for (int Nn = 0; Nn <= M; Nn++) {
+      int dn = tile_dim(N, NB, Nn);
+      double *D = tile_ptr(T, M, Nn, nt, NB);
+      const double *X = row + per * Nn;
+      for (int j = 0; j < dn; j++)
+        for (int i = (Nn == M ? j : 0); i < dm; i++) D[i + (long)j * NB] = X[i + (long)j * NB];
+    }
+  }
+  free(row);
+}
+
+/* POTRI (examples/potrf/potri.h:17-30): TRTRI then LAUUM of the factor. */
+int oracle_dpotri_tiled(int N, int NB, double *T) {
+  int info = oracle_dtrtri_tiled(N, NB, T);
+  if (info) return info;
+  oracle_dlauum_tiled(N, NB, T);
+  return 0;
+}
+
+/* check_dtrtri of examples/potrf/testing_dtrtri.cc:269-325 (lower, non-unit):
+ * ||I - L Linv||_1 / (||L||_1 ||Linv||_1 N eps), one-norms of the lower
+ * triangles; dense col-major N x N, lda = N, only lower parts read. */
+double oracle_check_dtrtri(const double *L, const double *Linv, int N) {
+  double *P = (double *)calloc((size_t)N * N, sizeof(double));
+  for (int j = 0; j < N; j++)
+    for (int i = j; i < N; i++) {
+      double s = 0.0;
+      for (int k = j; k <= i; k++) s += L[i + (long)k * N] * Linv[k + (long)j * N];
+      P[i + (long)j * N] = ((i == j) ? 1.0 : 0.0) - s;
+    }
+  double an = 0.0, bn = 0.0, rn = 0.0;
+  for (int j = 0; j < N; j++) {
+    double sa = 0.0, sb = 0.0, sr = 0.0;
+    for (int i = j; i < N; i++) {
+      sa += fabs(L[i + (long)j * N]);
+      sb += fabs(Linv[i + (long)j * N]);
+      sr += fabs(P[i + (long)j * N]);
+    }
+    if (sa > an) an = sa;
+    if (sb > bn) bn = sb;
+    if (sr > rn) rn = sr;
+  }
+  free(P);
+  double eps = 2.220446049250313e-16;
+  return (rn / an / bn) / (eps * N);
+}
+
+/* FLOPS_DTRTRI / FLOPS_DLAUUM / FLOPS_DPOTRI of lauum.h:11-25 (DPLASMA). */
+double oracle_flops_dtrtri(double n) {
+  return n * (n * (1. / 6. * n + 0.5) + 1. / 3.) + n * (n * (1. / 6. * n - 0.5) + 1. / 3.);
+}
+double oracle_flops_dpotri(double n) {
+  return n * ((2. / 3.) + n * ((1. / 3.) * n + 1.)) + n * ((1. / 6.) + n * ((1. / 3.) * n - 0.5));
+}
+double oracle_flops_dlauum(double n) { return oracle_flops_dpotri(n) - oracle_flops_dtrtri(n); }

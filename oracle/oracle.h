@@ -110,6 +110,23 @@ double oracle_solve_backward_error(int N, int NB, const double *Atiles, int nrhs
 /* ||A||_inf of the symmetric matrix held by its lower tiles. */
 double oracle_norm_inf_sym_tiled(int N, int NB, const double *Atiles);
 
+/* POINV family (examples/potrf/trtri_L.h, lauum.h, potri.h). */
+void oracle_dtrsm_rln(int m, int n, double alpha, const double *L, int ldl, double *B, int ldb);
+void oracle_dtrsm_lln(int m, int n, const double *L, int ldl, double *B, int ldb);
+void oracle_dtrmm_llt(int m, int n, const double *L, int ldl, double *B, int ldb);
+void oracle_dsyrk_lt(int n, int k, double alpha, const double *A, int lda, double beta, double *C, int ldc);
+void oracle_dgemm_xn(int ta, int m, int n, int k, double alpha, const double *A, int lda, const double *B,
+                     int ldb, double beta, double *C, int ldc);
+int oracle_dtrtri_tile(int n, double *A, int lda);
+void oracle_dlauum_tile(int n, double *A, int lda);
+int oracle_dtrtri_tiled(int N, int NB, double *tiles);
+void oracle_dlauum_tiled(int N, int NB, double *tiles);
+int oracle_dpotri_tiled(int N, int NB, double *tiles);
+double oracle_check_dtrtri(const double *L, const double *Linv, int N);
+double oracle_flops_dtrtri(double n);
+double oracle_flops_dlauum(double n);
+double oracle_flops_dpotri(double n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,20 @@ def lib():
             "oracle_dresidual_tiled": (None, [_i, _i, _p, _i, _p, _i, _p, _i, _p, _i]),
             "oracle_solve_backward_error": (_d, [_i, _i, _p, _i, _p, _i, _p, _i]),
             "oracle_norm_inf_sym_tiled": (_d, [_i, _i, _p]),
+            "oracle_dtrsm_rln": (None, [_i, _i, _d, _p, _i, _p, _i]),
+            "oracle_dtrsm_lln": (None, [_i, _i, _p, _i, _p, _i]),
+            "oracle_dtrmm_llt": (None, [_i, _i, _p, _i, _p, _i]),
+            "oracle_dsyrk_lt": (None, [_i, _i, _d, _p, _i, _d, _p, _i]),
+            "oracle_dgemm_xn": (None, [_i, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p, _i]),
+            "oracle_dtrtri_tile": (_i, [_i, _p, _i]),
+            "oracle_dlauum_tile": (None, [_i, _p, _i]),
+            "oracle_dtrtri_tiled": (_i, [_i, _i, _p]),
+            "oracle_dlauum_tiled": (None, [_i, _i, _p]),
+            "oracle_dpotri_tiled": (_i, [_i, _i, _p]),
+            "oracle_check_dtrtri": (_d, [_p, _p, _i]),
+            "oracle_flops_dtrtri": (_d, [_d]),
+            "oracle_flops_dlauum": (_d, [_d]),
+            "oracle_flops_dpotri": (_d, [_d]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -180,3 +194,25 @@ def priority(kind, nt, k0, k1=0, k2=0):
 
 
 POTRF, TRSM, SYRK, GEMM = 0, 1, 2, 3
+
+
+def trtri_tiled(N, NB, tiles):
+    """In place L := L^{-1} over lower tiles (examples/potrf/trtri_L.h order)."""
+    return lib().oracle_dtrtri_tiled(N, NB, ptr(tiles))
+
+
+def lauum_tiled(N, NB, tiles):
+    """In place A := L^T L over lower tiles (examples/potrf/lauum.h order)."""
+    lib().oracle_dlauum_tiled(N, NB, ptr(tiles))
+
+
+def potri_tiled(N, NB, tiles):
+    """In place A := (L L^T)^{-1} from the factor (examples/potrf/potri.h)."""
+    return lib().oracle_dpotri_tiled(N, NB, ptr(tiles))
+
+
+def check_dtrtri(L, Linv):
+    """check_dtrtri of examples/potrf/testing_dtrtri.cc:269-325 (dense, lower)."""
+    L = np.asfortranarray(L, dtype=np.float64)
+    Linv = np.asfortranarray(Linv, dtype=np.float64)
+    return lib().oracle_check_dtrtri(ptr(L), ptr(Linv), L.shape[0])

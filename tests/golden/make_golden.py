@@ -7,8 +7,14 @@ container, where the reference tree and scipy are present):
   lapack_tiles.npz -- FP64 tile-kernel results from LAPACK/BLAS through
                     scipy: dpotrf (lower), dtrsm (R,L,T,N), dsyrk (L,N),
                     dgemm (N,T), dpotrs, on small seeded inputs.
+  lapack_poinv.npz -- the POINV family from LAPACK/BLAS through scipy:
+                    dtrtri, dlauum, dpotri (lower, non-unit) of seeded
+                    Cholesky factors, and the tile operations of the TRTRI
+                    and LAUUM graphs: dtrsm (R,L,N,N), dtrsm (L,L,N,N),
+                    dtrmm (L,L,T,N), dsyrk (L,T), dgemm (N,N) and (T,N).
 
 Usage: python tests/golden/make_golden.py   (after `make -C oracle all ref`)
+       python tests/golden/make_golden.py poinv   (lapack_poinv.npz only)
 """
 import ctypes
 import os
@@ -98,5 +104,65 @@ def main():
     print("wrote", os.listdir(HERE))
 
 
+def poinv():
+    rng = np.random.default_rng(20261203)
+    out = {}
+
+    def factor(n):
+        B = rng.uniform(-1, 1, (n, n))
+        A = B @ B.T / n + np.eye(n)
+        L, info = sl.dpotrf(A, lower=1, clean=1)
+        assert info == 0
+        return np.tril(L)
+
+    for n in (1, 7, 45, 100, 160):
+        L = factor(n)
+        Li, info = sl.dtrtri(L, lower=1, unitdiag=0)
+        assert info == 0
+        U, info = sl.dlauum(L, lower=1)
+        assert info == 0
+        P, info = sl.dpotri(L, lower=1)
+        assert info == 0
+        out[f"L_{n}"] = L
+        out[f"trtri_{n}"] = np.tril(Li)
+        out[f"lauum_{n}"] = np.tril(U)
+        out[f"potri_{n}"] = np.tril(P)
+    for (m, n) in [(1, 1), (5, 8), (37, 45)]:
+        L = factor(n)
+        X = rng.uniform(-1, 1, (m, n))
+        out[f"trsm_rln_L_{m}_{n}"] = L
+        out[f"trsm_rln_B_{m}_{n}"] = X
+        out[f"trsm_rln_X_{m}_{n}"] = sb.dtrsm(-1.0, L, X, side=1, lower=1, trans_a=0, diag=0)
+        L2 = factor(m)
+        Y = rng.uniform(-1, 1, (m, n))
+        out[f"trsm_lln_L_{m}_{n}"] = L2
+        out[f"trsm_lln_B_{m}_{n}"] = Y
+        out[f"trsm_lln_X_{m}_{n}"] = sb.dtrsm(1.0, L2, Y, side=0, lower=1, trans_a=0, diag=0)
+        out[f"trmm_llt_X_{m}_{n}"] = sb.dtrmm(1.0, L2, Y, side=0, lower=1, trans_a=1, diag=0)
+    for (n, k) in [(1, 1), (8, 5), (45, 37)]:
+        A = rng.uniform(-1, 1, (k, n))
+        C = rng.uniform(-1, 1, (n, n))
+        out[f"syrk_lt_A_{n}_{k}"] = A
+        out[f"syrk_lt_C_{n}_{k}"] = C
+        out[f"syrk_lt_out_{n}_{k}"] = np.tril(sb.dsyrk(1.0, A, beta=1.0, c=C, lower=1, trans=1))
+    for (m, n, k) in [(1, 1, 1), (8, 5, 3), (37, 45, 29)]:
+        A = rng.uniform(-1, 1, (m, k))
+        At = rng.uniform(-1, 1, (k, m))
+        B = rng.uniform(-1, 1, (k, n))
+        C = rng.uniform(-1, 1, (m, n))
+        out[f"gemm_xn_A_{m}_{n}_{k}"] = A
+        out[f"gemm_xn_At_{m}_{n}_{k}"] = At
+        out[f"gemm_xn_B_{m}_{n}_{k}"] = B
+        out[f"gemm_xn_C_{m}_{n}_{k}"] = C
+        out[f"gemm_nn_out_{m}_{n}_{k}"] = sb.dgemm(1.0, A, B, beta=1.0, c=C)
+        out[f"gemm_tn_out_{m}_{n}_{k}"] = sb.dgemm(1.0, At, B, beta=1.0, c=C, trans_a=1)
+    np.savez_compressed(os.path.join(HERE, "lapack_poinv.npz"), **out)
+    print("wrote lapack_poinv.npz")
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["poinv"]:
+        poinv()
+    else:
+        main()
+        poinv()
