@@ -138,6 +138,35 @@ int ttg_chol_dpotrs_tiled(void *stream, int N, int NB, const double *tiles, int 
 int ttg_chol_dresidual_tiled(void *stream, int N, int NB, const double *tiles, int nrhs, const double *X,
                              int ldx, const double *B, int ldb, double *R, int ldr);
 
+/* ---------------------------------------------------------------- */
+/* POINV family (FP64, lower tile-packed layout, in place)          */
+/* ---------------------------------------------------------------- */
+
+/* tiles := L^{-1} for the lower triangular, non-unit factor held in the
+ * lower tiles (the strict upper triangle of each diagonal tile is not
+ * referenced or changed).  Replaces the TRTRI graph
+ * trtri_LOWER::make_trtri_ttg(A, Diag::NonUnit, ...) of
+ * examples/potrf/trtri_L.h:263-339, whose tasks call lapack::trtri (:27),
+ * blas::trsm Left/Right (:49-56, :96-104) and blas::gemm NN (:178-185), in
+ * that graph's step order.  *d_info receives 0 or the 1-based index of the
+ * first zero diagonal entry, in which case the tiles are left unchanged
+ * (LAPACK dtrtri); d_info may be NULL.  The padding of the last tile row is
+ * reset as in ttg_chol_dpotrf_tiled.  NB % 32 == 0, NB <= 512. */
+int ttg_chol_dtrtri_tiled(void *stream, int N, int NB, double *tiles, int *d_info);
+
+/* tiles := L^T L (lower part) for the lower triangular factor L.  Replaces
+ * lauum::make_lauum_ttg of examples/potrf/lauum.h:247-322 (lapack::lauum
+ * :37, blas::trmm Left/Lower/Trans :72-80, blas::syrk Lower/Trans :123-128,
+ * blas::gemm TN :172-180).  Uses a device copy of the tiles (allocated on
+ * the first call for a matrix and kept with its plan). */
+int ttg_chol_dlauum_tiled(void *stream, int N, int NB, double *tiles);
+
+/* tiles := A^{-1} (lower part) from the Cholesky factor of A (the output of
+ * ttg_chol_dpotrf_tiled): TRTRI then LAUUM.  Replaces
+ * potri::make_potri_ttg of examples/potrf/potri.h:17-30.  *d_info as for
+ * ttg_chol_dtrtri_tiled (LAUUM is skipped when it is nonzero). */
+int ttg_chol_dpotri_tiled(void *stream, int N, int NB, double *tiles, int *d_info);
+
 /* Working precisions of ttg_chol_dposv_mixed. */
 #define TTG_CHOL_PREC_FP64 0 /* FP64 factor (time-to-solution reference) */
 #define TTG_CHOL_PREC_FP32 1 /* FP32 TRSM + trailing updates, FP64 diagonal POTRF */

@@ -40,6 +40,10 @@ def test_version_and_error_strings():
     lambda L: L.ttg_chol_dsyrk_ln(None, -2, 8, -1.0, None, 8, 1.0, None, 8),
     lambda L: L.ttg_chol_dpotrf_tiled(None, 100, 0, None, None, 0),
     lambda L: L.ttg_chol_dpotrf_tiled(None, 100, 48, ctypes.c_void_p(16), None, 0),
+    lambda L: L.ttg_chol_dtrtri_tiled(None, 100, 0, None, None),
+    lambda L: L.ttg_chol_dtrtri_tiled(None, 100, 48, ctypes.c_void_p(16), None),
+    lambda L: L.ttg_chol_dlauum_tiled(None, 100, 1024, ctypes.c_void_p(16)),
+    lambda L: L.ttg_chol_dpotri_tiled(None, 100, 64, None, None),
 ])
 def test_bad_arguments_rejected_without_launch(call):
     L = C.lib()

@@ -1,0 +1,143 @@
+"""POINV family on the GPU (TRTRI, LAUUM, POTRI over the lower tiles; the
+graphs of examples/potrf/trtri_L.h, lauum.h, potri.h) against the oracle.
+FP64 bar: max |gpu - oracle| / max |oracle| <= 1e-12 over the lower tiles,
+plus the closed forms L L^{-1} = I and A A^{-1} = I and the reference's
+check_dtrtri bound at sizes where the oracle is too slow."""
+import numpy as np
+import pytest
+import torch
+
+from gpu_util import lower_tiles_rel_err, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+SIZES = [(160, 32), (100, 32), (1000, 128), (1536, 512), (2048, 256)]
+
+
+def factor(oracle, N, NB, seed=5):
+    A0 = oracle.spd_tiles(N, NB, 1.0, seed)
+    L = A0.copy()
+    assert oracle.potrf_tiled(N, NB, L) == 0
+    return A0, L
+
+
+def strict_upper_of_diag(oracle, N, NB, t):
+    nt = oracle.nt_of(N, NB)
+    out = []
+    for k in range(nt):
+        d = min(NB, N - k * NB)
+        out.append(np.triu(oracle.tile_view(t, N, NB, k, k)[:d, :d], 1).copy())
+    return out
+
+
+@pytest.mark.parametrize("N,NB", SIZES)
+def test_dtrtri_tiled_vs_oracle(dev, oracle, N, NB):
+    import ttg_amd.chol as C
+
+    _, L = factor(oracle, N, NB)
+    # garbage in the unreferenced strict upper part of the diagonal tiles
+    nt = oracle.nt_of(N, NB)
+    for k in range(nt):
+        v = oracle.tile_view(L, N, NB, k, k)
+        d = min(NB, N - k * NB)
+        v[:d, :d] += np.triu(np.full((d, d), 7.0), 1)
+    want = L.copy()
+    assert oracle.trtri_tiled(N, NB, want) == 0
+    t = to_dev(L, dev)
+    info = torch.full((1,), -1, dtype=torch.int32, device=dev)
+    C.dtrtri_tiled(t, N, NB, info)
+    got = to_host(t)
+    assert int(info.item()) == 0
+    assert lower_tiles_rel_err(oracle, N, NB, got, want) <= TOL
+    for a, b in zip(strict_upper_of_diag(oracle, N, NB, got), strict_upper_of_diag(oracle, N, NB, L)):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("N,NB", SIZES)
+def test_dlauum_tiled_vs_oracle(dev, oracle, N, NB):
+    import ttg_amd.chol as C
+
+    _, L = factor(oracle, N, NB, seed=9)
+    want = L.copy()
+    oracle.lauum_tiled(N, NB, want)
+    t = to_dev(L, dev)
+    C.dlauum_tiled(t, N, NB)
+    got = to_host(t)
+    assert lower_tiles_rel_err(oracle, N, NB, got, want) <= TOL
+    # a second call on fresh data reuses the plan and its work copy
+    t2 = to_dev(L, dev)
+    C.dlauum_tiled(t2, N, NB)
+    assert np.array_equal(to_host(t2), got)
+
+
+@pytest.mark.parametrize("N,NB", [(160, 32), (1000, 128), (2048, 512)])
+def test_dpotri_tiled_vs_oracle(dev, oracle, N, NB):
+    import ttg_amd.chol as C
+
+    A0, L = factor(oracle, N, NB, seed=11)
+    want = L.copy()
+    assert oracle.potri_tiled(N, NB, want) == 0
+    t = to_dev(L, dev)
+    info = torch.full((1,), -1, dtype=torch.int32, device=dev)
+    C.dpotri_tiled(t, N, NB, info)
+    got = to_host(t)
+    assert int(info.item()) == 0
+    assert lower_tiles_rel_err(oracle, N, NB, got, want) <= TOL
+    X = np.tril(oracle.tiles_to_dense(N, NB, got))
+    X = X + np.tril(X, -1).T
+    A = np.tril(oracle.tiles_to_dense(N, NB, A0))
+    A = A + np.tril(A, -1).T
+    assert np.abs(X @ A - np.eye(N)).max() < 1e-12
+
+
+def test_poinv_after_gpu_potrf_large(dev, oracle):
+    """potrf -> trtri and potrf -> potri on the GPU at N=4096, NB=512: the
+    closed forms on the GPU (FP64 products) and check_dtrtri at N=2048."""
+    import ttg_amd.chol as C
+
+    N, NB = 4096, 512
+    A0 = oracle.spd_tiles(N, NB, 1.0, 21)
+    t = to_dev(A0, dev)
+    info = torch.zeros(1, dtype=torch.int32, device=dev)
+    C.dpotrf_tiled(t, N, NB, info)
+    L = to_host(t)
+    assert int(info.item()) == 0
+    Ld = torch.from_numpy(np.tril(oracle.tiles_to_dense(N, NB, L))).to(dev)
+    t2 = to_dev(L, dev)
+    C.dtrtri_tiled(t2, N, NB, info)
+    Li = torch.from_numpy(np.tril(oracle.tiles_to_dense(N, NB, to_host(t2)))).to(dev)
+    assert int(info.item()) == 0
+    eye = torch.eye(N, dtype=torch.float64, device=dev)
+    assert (Ld @ Li - eye).abs().max().item() < 1e-12
+    t3 = to_dev(L, dev)
+    C.dpotri_tiled(t3, N, NB, info)
+    X = np.tril(oracle.tiles_to_dense(N, NB, to_host(t3)))
+    X = torch.from_numpy(X + np.tril(X, -1).T).to(dev)
+    A = np.tril(oracle.tiles_to_dense(N, NB, A0))
+    A = torch.from_numpy(A + np.tril(A, -1).T).to(dev)
+    assert (X @ A - eye).abs().max().item() < 1e-11
+    # the reference's TRTRI check at N=2048
+    N2, NB2 = 2048, 256
+    _, L2 = factor(oracle, N2, NB2, seed=3)
+    t4 = to_dev(L2, dev)
+    C.dtrtri_tiled(t4, N2, NB2, info)
+    Li2 = np.tril(oracle.tiles_to_dense(N2, NB2, to_host(t4)))
+    assert oracle.check_dtrtri(np.tril(oracle.tiles_to_dense(N2, NB2, L2)), Li2) <= 10.0
+
+
+def test_dtrtri_singular_info(dev, oracle):
+    import ttg_amd.chol as C
+
+    N, NB = 300, 128
+    _, L = factor(oracle, N, NB, seed=2)
+    v = oracle.tile_view(L, N, NB, 2, 2)
+    v[13, 13] = 0.0  # global index 2*128 + 13
+    t = to_dev(L, dev)
+    info = torch.full((1,), -1, dtype=torch.int32, device=dev)
+    C.dtrtri_tiled(t, N, NB, info)
+    assert int(info.item()) == 2 * NB + 13 + 1
+    got = to_host(t)
+    assert lower_tiles_rel_err(oracle, N, NB, got, L) == 0.0
+    C.dpotri_tiled(t, N, NB, info)
+    assert int(info.item()) == 2 * NB + 13 + 1
+    assert lower_tiles_rel_err(oracle, N, NB, to_host(t), L) == 0.0

@@ -20,6 +20,7 @@ SOURCES = [
     "chol_mixed.hip",
     "chol_f16.hip",
     "chol_batched.hip",
+    "chol_poinv.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -36,6 +36,9 @@ SIGNATURES = {
     "ttg_chol_dpotrf_tiled": (_i, [_p, _i, _i, _p, _p, _u]),
     "ttg_chol_dpotrs_tiled": (_i, [_p, _i, _i, _p, _i, _p, _i]),
     "ttg_chol_dresidual_tiled": (_i, [_p, _i, _i, _p, _i, _p, _i, _p, _i, _p, _i]),
+    "ttg_chol_dtrtri_tiled": (_i, [_p, _i, _i, _p, _p]),
+    "ttg_chol_dlauum_tiled": (_i, [_p, _i, _i, _p]),
+    "ttg_chol_dpotri_tiled": (_i, [_p, _i, _i, _p, _p]),
     "ttg_chol_plgsy_tiled": (_i, [_p, _i, _i, _d, _ull, _p]),
     "ttg_chol_dplgsy_tile": (_i, [_p, _d, _i, _i, _p, _i, _i, _i, _i, _ull]),
     "ttg_chol_spd_tiled": (_i, [_p, _i, _i, _d, _ull, _p, _p]),
@@ -207,6 +210,28 @@ def dresidual_tiled(tiles, N, NB, X, B, R, stream=None):
     nrhs = 1 if B.dim() == 1 else B.shape[1]
     _check(lib().ttg_chol_dresidual_tiled(_stream(stream), N, NB, _ptr(tiles), nrhs, _ptr(X), _ld(X), _ptr(B),
                                           _ld(B), _ptr(R), _ld(R)), "dresidual_tiled")
+
+
+def dtrtri_tiled(tiles, N, NB, info=None, stream=None):
+    """tiles := L^{-1} (lower tiles, in place); the TRTRI graph of
+    examples/potrf/trtri_L.h.  info (device int32 tensor) receives 0 or the
+    1-based index of the first zero diagonal entry."""
+    _require_cuda(tiles, info)
+    _check(lib().ttg_chol_dtrtri_tiled(_stream(stream), N, NB, _ptr(tiles), _ptr(info)), "dtrtri_tiled")
+
+
+def dlauum_tiled(tiles, N, NB, stream=None):
+    """tiles := L^T L (lower part, in place); the LAUUM graph of
+    examples/potrf/lauum.h."""
+    _require_cuda(tiles)
+    _check(lib().ttg_chol_dlauum_tiled(_stream(stream), N, NB, _ptr(tiles)), "dlauum_tiled")
+
+
+def dpotri_tiled(tiles, N, NB, info=None, stream=None):
+    """tiles := A^{-1} (lower part, in place) from the Cholesky factor of A;
+    the POTRI graph of examples/potrf/potri.h (TRTRI then LAUUM)."""
+    _require_cuda(tiles, info)
+    _check(lib().ttg_chol_dpotri_tiled(_stream(stream), N, NB, _ptr(tiles), _ptr(info)), "dpotri_tiled")
 
 
 def selftest_mfma_layout(kind, out, stream=None):
