@@ -65,6 +65,8 @@ def parse():
                    help="the batched workload instead: --batch-count FP32 SPD systems B B^T/n + I, n uniform in "
                         "[32, 1024], one right-hand side each, factor + solve in one launch (autotuned)")
     p.add_argument("--batch-count", type=int, default=4096)
+    p.add_argument("--poinv", action="store_true",
+                   help="POTRI (TRTRI + LAUUM) of an FP64 Cholesky factor, N=32768 unless --n")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="only run a plain device copy of MIB MiB three times (calibrates the HBM byte counters) "
                         "and exit")
@@ -203,6 +205,99 @@ def run_batched(a):
     print(json.dumps(out), flush=True)
 
 
+def flops_dpotri(n):
+    """FLOPS_DPOTRI of examples/potrf/potri.h:13-15 (DPLASMA)."""
+    return n * ((2. / 3.) + n * ((1. / 3.) * n + 1.)) + n * ((1. / 6.) + n * ((1. / 3.) * n - 0.5))
+
+
+def flops_dtrtri(n):
+    """FLOPS_DTRTRI of examples/potrf/trtri_L.h:13-15 (DPLASMA)."""
+    return n * (n * (1. / 6. * n + 0.5) + 1. / 3.) + n * (n * (1. / 6. * n - 0.5) + 1. / 3.)
+
+
+def cpu_baseline_poinv(n=2048, nb=256):
+    """The oracle's POTRI (TRTRI + LAUUM over tiles in the reference graphs'
+    task order), one thread, on a bounded SPD sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    t = O.spd_tiles(n, nb, 1.0, 7)
+    if O.potrf_tiled(n, nb, t) != 0:
+        raise RuntimeError("oracle poinv baseline: not SPD")
+    t0 = time.perf_counter()
+    if O.potri_tiled(n, nb, t) != 0:
+        raise RuntimeError("oracle poinv baseline: singular factor")
+    dt = time.perf_counter() - t0
+    return {"value": flops_dpotri(float(n)) / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
+            "sample": f"oracle POTRI of an SPD N={n}, NB={nb} factor, one thread, {dt:.2f} s"}
+
+
+def run_poinv(a):
+    import torch
+
+    import ttg_amd.chol as C
+
+    N, NB = a.n or 32768, a.nb
+    t = C.alloc_tiles(N, NB)
+    work = torch.empty(N * N, dtype=torch.float64, device="cuda")
+    C.spd_tiled(t, N, NB, 1.0, a.seed, work)
+    del work
+    A0 = t.clone()
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    C.dpotrf_tiled(t, N, NB, info)
+    torch.cuda.synchronize()
+    if int(info.item()) != 0:
+        raise RuntimeError("poinv bench: matrix not SPD")
+    L0 = t.clone()
+
+    def timed(fn):
+        t.copy_(L0)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
+    ts = []
+    for i in range(a.warmup + a.steps):
+        ms = timed(lambda: C.dpotri_tiled(t, N, NB, info))
+        if i >= a.warmup:
+            ts.append(ms)
+    if int(info.item()) != 0:
+        raise RuntimeError("poinv bench: singular factor")
+    # A^{-1} (A v) = v for a random v, through the tiled symmetric products
+    v = torch.rand(N, 1, dtype=torch.float64, device="cuda") * 2 - 1
+    zero = torch.zeros_like(v)
+    y, z = torch.empty_like(v), torch.empty_like(v)
+    C.dresidual_tiled(A0, N, NB, v, zero, y)   # y = -A v
+    C.dresidual_tiled(t, N, NB, y, zero, z)    # z = A^{-1} A v
+    err = ((z - v).abs().max() / v.abs().max()).item()
+    ms_trtri = timed(lambda: C.dtrtri_tiled(t, N, NB, info))
+    ms_lauum = timed(lambda: C.dlauum_tiled(t, N, NB))
+    ms = sum(ts) / len(ts)
+    F = flops_dpotri(float(N))
+    value = F / (ms * 1e-3) / 1e12
+    out = {
+        "metric": f"POTRI FP64 TFLOP/s (FLOPS_DPOTRI, N={N}, NB={NB})",
+        "value": round(value, 3), "unit": "TFLOP/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic: Cholesky factor of B B^T/N + I (B from the seeded integer generator)",
+        "config": {"workload": f"POTRI (TRTRI + LAUUM) of an FP64 factor, N={N}, NB={NB}, resident in HBM",
+                   "N": N, "NB": NB, "parallelism": "single GPU"},
+        "trtri_ms": round(ms_trtri, 3), "lauum_ms": round(ms_lauum, 3),
+        "trtri_tflops": round(flops_dtrtri(float(N)) / (ms_trtri * 1e-3) / 1e12, 3),
+        "lauum_tflops": round((F - flops_dtrtri(float(N))) / (ms_lauum * 1e-3) / 1e12, 3),
+        "inverse_check_rel_err": err,
+        "roofline": {"bound": "mfma", "achieved": round(value, 3), "peak": 78.6, "unit": "TFLOP/s",
+                     "frac": round(value / 78.6, 4), "traffic": None,
+                     "kernel": "whole POTRI (update_kernel<double> GEMMs, TRSMs, tile passes)"},
+        "cpu_baseline": None if a.no_cpu else cpu_baseline_poinv(),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def read_traffic(mode, N, NB, nrhs):
     """HBM bytes per launch of the dominant kernel from the committed PMC
     summary for this workload (profiles/pmc_<kernel>_N<N>.txt), or None."""
@@ -252,6 +347,9 @@ def main():
         return
     if a.batched:
         run_batched(a)
+        return
+    if a.poinv:
+        run_poinv(a)
         return
     import torch
 

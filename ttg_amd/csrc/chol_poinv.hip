@@ -33,6 +33,7 @@
 #include <memory>
 #include <mutex>
 #include <tuple>
+#include <utility>
 #include <vector>
 
 #include "chol_device.hpp"
@@ -133,6 +134,7 @@ struct PoinvPlan {
          o_lauum = 0;
   std::vector<int> trsmr_off, trsmr_cnt, gemm_off, gemm_cnt, trsml_off, trsml_cnt, fin_off, fin_cnt;
   int n_trans = 0, n_lauum = 0;
+  std::vector<std::pair<int, int>> lauum_mn;  // output tile of each LAUUM op, (-1, -1) for padding
   bool fast = false;
 
   double *tile(int m, int n) const { return tiles + tile_index(m, n, NT) * (long)per; }
@@ -181,9 +183,22 @@ struct PoinvPlan {
     // LAUUM: transposed copy, then one GEMM per output tile (long ones first)
     for (int n = 0; n < NT; ++n)
       for (int m = n; m < NT; ++m) trans.push_back(TileMove{tile(m, n), nullptr, m == n ? kMoveTransLower : kMoveTrans});
-    for (int m = 0; m < NT; ++m)
-      for (int n = 0; n <= m; ++n)
-        lauum.push_back(UpdOp<double>{nullptr, nullptr, tile(m, n), NB, NB, NB * (NT - m), NB, NB, NB, m == n ? 1 : 0});
+    // The update kernel gives XCD x the x-th eighth of the op list, and the
+    // depth of op (m, n) is NT - m tiles: deal the ops, deepest first, round
+    // robin into 8 equal-length groups (padded with empty ops) so that every
+    // XCD gets the same work, each in deepest-first order.
+    {
+      std::vector<std::pair<int, int>> mn;
+      for (int m = 0; m < NT; ++m)
+        for (int n = 0; n <= m; ++n) mn.emplace_back(m, n);
+      const int len = ((int)mn.size() + kXcds - 1) / kXcds;
+      lauum_mn.assign((size_t)len * kXcds, std::make_pair(-1, -1));
+      for (size_t i = 0; i < mn.size(); ++i) lauum_mn[(i % kXcds) * len + i / kXcds] = mn[i];
+      for (auto &p : lauum_mn)
+        lauum.push_back(p.first < 0 ? UpdOp<double>{nullptr, nullptr, tiles, 0, 0, 0, NB, NB, NB, 0}
+                                    : UpdOp<double>{nullptr, nullptr, tile(p.first, p.second), NB, NB,
+                                                    NB * (NT - p.first), NB, NB, NB, p.first == p.second ? 1 : 0});
+    }
     n_trans = (int)trans.size();
     n_lauum = (int)lauum.size();
     (void)ntile;
@@ -238,12 +253,12 @@ struct PoinvPlan {
     TTGC_HIP(hipMemcpy(lauum.data(), at<UpdOp<double>>(o_lauum), n_lauum * sizeof(UpdOp<double>),
                        hipMemcpyDeviceToHost));
     for (auto &t : trans) t.dst = T + (t.src - tiles);
-    int i = 0;
-    for (int m = 0; m < NT; ++m)
-      for (int n = 0; n <= m; ++n, ++i) {
-        lauum[i].A = T + tile_index(m, m, NT) * (long)per;
-        lauum[i].B = T + tile_index(m, n, NT) * (long)per;
-      }
+    for (size_t i = 0; i < lauum_mn.size(); ++i) {
+      const int m = lauum_mn[i].first, n = lauum_mn[i].second;
+      if (m < 0) continue;
+      lauum[i].A = T + tile_index(m, m, NT) * (long)per;
+      lauum[i].B = T + tile_index(m, n, NT) * (long)per;
+    }
     TTGC_HIP(hipMemcpy(const_cast<TileMove *>(at<TileMove>(o_trans)), trans.data(), n_trans * sizeof(TileMove),
                        hipMemcpyHostToDevice));
     TTGC_HIP(hipMemcpy(const_cast<UpdOp<double> *>(at<UpdOp<double>>(o_lauum)), lauum.data(),
