@@ -359,7 +359,7 @@ __global__ __launch_bounds__(256) void trsm_kernel(const TrsmOp<T> *__restrict__
 // ---------------------------------------------------------------------------
 // batched update  C := alpha A B^T + beta C
 // ---------------------------------------------------------------------------
-// Two block shapes: 128 x 128 for tile updates, 32 x 128 ("skinny") when
+// Two block shapes: 128 x 128 for tile updates, 32 x 64 ("skinny") when
 // the output has at most 32 rows (the right-hand-side panels of the solves).
 template <typename T, bool TB, bool SKINNY>
 struct UpdCfg;
@@ -373,11 +373,11 @@ struct UpdCfg<float, TB, false> {
 };
 template <bool TB>
 struct UpdCfg<double, TB, true> {
-  using Core = GemmCore<double, 32, 128, 32, 1, 4, TB>;
+  using Core = GemmCore<double, 32, 64, 32, 1, 4, TB>;
 };
 template <bool TB>
 struct UpdCfg<float, TB, true> {
-  using Core = GemmCore<float, 32, 128, 32, 1, 4, TB>;
+  using Core = GemmCore<float, 32, 64, 32, 1, 4, TB>;
 };
 
 // FAST applies to both operands for the 128 x 128 shape and to the B operand
@@ -387,7 +387,7 @@ __device__ __forceinline__ void update_body(const UpdOp<T> *__restrict__ ops, Up
                                                      T alpha, T beta, const int *__restrict__ d_abort) {
   using Core = typename UpdCfg<T, TB, SKINNY>::Core;
   constexpr int BM = SKINNY ? 32 : 128;
-  constexpr int BN = 128;
+  constexpr int BN = SKINNY ? 64 : 128;
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const int o = b / bpo, rr = b % bpo, bi = rr / nbn, bj = rr % nbn;
   const UpdOp<T> op = ops ? ops[o] : one;
@@ -525,8 +525,8 @@ hipError_t launch_trsm(hipStream_t s, const TrsmOp<T> *ops, int nops, const Trsm
 template <typename T, bool SKINNY>
 static hipError_t launch_update_shape(hipStream_t s, const UpdOp<T> *ops, int nops, const UpdOp<T> &one, int max_m,
                                       int max_n, T alpha, T beta, bool fast, const int *d_abort, bool tb) {
-  constexpr int BM = SKINNY ? 32 : 128;
-  const int nbm = (max_m + BM - 1) / BM, nbn = (max_n + 127) / 128;
+  constexpr int BM = SKINNY ? 32 : 128, BN = SKINNY ? 64 : 128;
+  const int nbm = (max_m + BM - 1) / BM, nbn = (max_n + BN - 1) / BN;
   const int bpo = nbm * nbn;
   const long grid = (long)nops * bpo;
   if (grid > 0x7fffffffL) return hipErrorInvalidConfiguration;

@@ -57,7 +57,7 @@ template <typename T>
 hipError_t launch_trsm(hipStream_t s, const TrsmOp<T> *ops, int nops, const TrsmOp<T> &one, int max_m,
                        const int *d_abort, bool trans = true);
 // fast: every op satisfies update_fast() (see below).  Outputs with at most
-// 32 rows (max_m <= 32) use a 32 x 128 block shape.
+// 32 rows (max_m <= 32) use a 32 x 64 block shape.
 // tb: the B operand's element (j, k) is read from B[k + j*ldb], i.e. the
 // update is C := alpha A B + beta C with B stored k x n column-major.
 template <typename T>
