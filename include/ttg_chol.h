@@ -59,6 +59,14 @@ const char *ttg_chol_last_error(void);
 /* Number of HIP devices visible to the library (replaces
  * ttg::device::num_devices() of ttg/ttg/device/device.h:150). */
 int ttg_chol_num_devices(void);
+/* Free every cached plan and workspace of the tiled drivers, the solves,
+ * the mixed-precision solve and the POINV drivers (their device memory,
+ * streams, events and graphs; plans are otherwise kept per matrix until the
+ * process ends).  Synchronises the device first; no other call of the
+ * library may be in flight.  The next call on a matrix builds its plan
+ * again.  No reference counterpart: the reference's buffers are owned by
+ * the PaRSEC runtime. */
+int ttg_chol_release_plans(void);
 
 /* ---------------------------------------------------------------- */
 /* per-tile operations (FP64)                                        */

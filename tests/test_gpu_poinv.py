@@ -141,3 +141,27 @@ def test_dtrtri_singular_info(dev, oracle):
     C.dpotri_tiled(t, N, NB, info)
     assert int(info.item()) == 2 * NB + 13 + 1
     assert lower_tiles_rel_err(oracle, N, NB, to_host(t), L) == 0.0
+
+
+def test_release_plans_then_rebuild(dev, oracle):
+    """Plans freed by ttg_chol_release_plans are rebuilt on the next call and
+    give the same bits (POTRI and the mixed solve)."""
+    import ttg_amd.chol as C
+
+    N, NB = 1000, 128
+    _, L = factor(oracle, N, NB, seed=13)
+    info = torch.zeros(1, dtype=torch.int32, device=dev)
+    t = to_dev(L, dev)
+    C.dpotri_tiled(t, N, NB, info)
+    first = to_host(t)
+    A = torch.from_numpy(oracle.spd_tiles(1024, 128, 1.0, 3)).to(dev)
+    B = (torch.arange(2 * 1024, dtype=torch.float64, device=dev).reshape(2, 1024) % 7 - 3).t()
+    X1 = torch.empty_like(B.t()).t()
+    C.dposv_mixed(A, 1024, 128, B, X1, prec=C.PREC_FP32, tol=1e-12, check=False)
+    C.release_plans()
+    t = to_dev(L, dev)
+    C.dpotri_tiled(t, N, NB, info)
+    assert np.array_equal(to_host(t), first)
+    X2 = torch.empty_like(B.t()).t()
+    C.dposv_mixed(A, 1024, 128, B, X2, prec=C.PREC_FP32, tol=1e-12, check=False)
+    assert torch.equal(X1, X2)

@@ -37,6 +37,7 @@ SIGNATURES = {
     "ttg_chol_dpotrs_tiled": (_i, [_p, _i, _i, _p, _i, _p, _i]),
     "ttg_chol_dresidual_tiled": (_i, [_p, _i, _i, _p, _i, _p, _i, _p, _i, _p, _i]),
     "ttg_chol_dtrtri_tiled": (_i, [_p, _i, _i, _p, _p]),
+    "ttg_chol_release_plans": (_i, []),
     "ttg_chol_dlauum_tiled": (_i, [_p, _i, _i, _p]),
     "ttg_chol_dpotri_tiled": (_i, [_p, _i, _i, _p, _p]),
     "ttg_chol_plgsy_tiled": (_i, [_p, _i, _i, _d, _ull, _p]),
@@ -210,6 +211,12 @@ def dresidual_tiled(tiles, N, NB, X, B, R, stream=None):
     nrhs = 1 if B.dim() == 1 else B.shape[1]
     _check(lib().ttg_chol_dresidual_tiled(_stream(stream), N, NB, _ptr(tiles), nrhs, _ptr(X), _ld(X), _ptr(B),
                                           _ld(B), _ptr(R), _ld(R)), "dresidual_tiled")
+
+
+def release_plans():
+    """Free every cached plan and workspace of the library (synchronises the
+    device); the next call on a matrix builds its plan again."""
+    _check(lib().ttg_chol_release_plans(), "release_plans")
 
 
 def dtrtri_tiled(tiles, N, NB, info=None, stream=None):

@@ -110,6 +110,17 @@ int ttg_chol_num_devices(void) {
   return n;
 }
 
+int ttg_chol_release_plans(void) {
+  clear_error();
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_fail(e, "release_plans: hipDeviceSynchronize");
+  release_mixed_plans();  // before the solve plans: a mixed plan's solve plan lives there
+  release_poinv_plans();
+  release_solve_plans();
+  release_tiled_plans();
+  return TTG_CHOL_OK;
+}
+
 int ttg_chol_dpotrf_tile(void *stream, int n, double *A, int lda, int *d_info, int info_offset) {
   clear_error();
   if (n < 0 || n > TTG_CHOL_MAX_TILE || lda < std::max(1, n) || (n > 0 && !A)) {

@@ -77,4 +77,12 @@ struct DevBuf {
   }
 };
 
+// Drop the cached plans and workspaces of one translation unit (device
+// memory, streams, events, graphs).  The caller has synchronised the device
+// and no other library call is in flight.
+void release_tiled_plans();
+void release_solve_plans();
+void release_mixed_plans();
+void release_poinv_plans();
+
 }  // namespace ttgc

@@ -174,6 +174,11 @@ struct MixedPlan : MixedPlanBase {
   SolvePlan<T> *splan = nullptr;
   std::vector<double> hstats;
 
+  ~MixedPlan() override {
+    for (DevBuf *b : {&W, &R, &stats, &rows, &norm_parts})
+      if (b->ptr) (void)hipFree(b->ptr);
+  }
+
   T *w() { return static_cast<T *>(W.ptr); }
   double *r() { return static_cast<double *>(R.ptr); }
   double *st() { return static_cast<double *>(stats.ptr); }
@@ -272,6 +277,11 @@ struct MixedPlan : MixedPlanBase {
 
 static std::mutex g_mixed_mu;
 static std::map<std::tuple<int, int, int, int>, std::unique_ptr<MixedPlanBase>> g_mixed;
+
+void release_mixed_plans() {
+  std::lock_guard<std::mutex> lk(g_mixed_mu);
+  g_mixed.clear();
+}
 
 }  // namespace ttgc
 

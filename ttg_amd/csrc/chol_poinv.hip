@@ -308,6 +308,11 @@ struct PoinvPlan {
 static std::mutex g_poinv_mu;
 static std::map<std::tuple<int, int, double *>, std::unique_ptr<PoinvPlan>> g_poinv;
 
+void release_poinv_plans() {
+  std::lock_guard<std::mutex> lk(g_poinv_mu);
+  g_poinv.clear();
+}
+
 static int poinv_plan(const char *who, int N, int NB, double *tiles, PoinvPlan **out) {
   if (N < 0 || NB <= 0 || NB > TTG_CHOL_MAX_TILE || (N > 0 && !tiles)) {
     set_error("%s: bad argument (N=%d NB=%d)", who, N, NB);

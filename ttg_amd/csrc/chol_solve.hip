@@ -592,6 +592,18 @@ __global__ void residual_reduce_kernel(int N, int NB, int nt, const double *__re
 static std::mutex g_res_mu;
 static std::map<hipStream_t, std::unique_ptr<DevBuf>> g_res_ws;
 
+void release_solve_plans() {
+  {
+    std::lock_guard<std::mutex> lk(g_solve_mu);
+    g_dplans.clear();
+    g_splans.clear();
+  }
+  std::lock_guard<std::mutex> lk(g_res_mu);
+  for (auto &kv : g_res_ws)
+    if (kv.second && kv.second->ptr) (void)hipFree(kv.second->ptr);
+  g_res_ws.clear();
+}
+
 int residual_fp64(hipStream_t s, int N, int NB, const double *tiles, int nrhs, const double *X, long ldx,
                   const double *B, long ldb, double *R, long ldr) {
   const int nt = (N + NB - 1) / NB;

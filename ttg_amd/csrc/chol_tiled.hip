@@ -14,6 +14,11 @@ namespace ttgc {
 static std::mutex g_plan_mu;
 static std::map<std::tuple<int, int, double *, unsigned>, std::unique_ptr<TiledPlanT<double>>> g_plans;
 
+void release_tiled_plans() {
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  g_plans.clear();
+}
+
 }  // namespace ttgc
 
 using namespace ttgc;
