@@ -165,3 +165,32 @@ def test_release_plans_then_rebuild(dev, oracle):
     X2 = torch.empty_like(B.t()).t()
     C.dposv_mixed(A, 1024, 128, B, X2, prec=C.PREC_FP32, tol=1e-12, check=False)
     assert torch.equal(X1, X2)
+
+
+def _run_poinv_driver(args):
+    import os
+    import subprocess
+
+    import ttg_amd.graph as G
+
+    exe = os.path.join(os.path.dirname(G.DRIVER), "testing_dpoinv")
+    print("testing_dpoinv", *args, flush=True)
+    r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=100)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    return r
+
+
+def test_poinv_driver_check_and_lines():
+    """bin/testing_dpoinv: PLGSY(bump=N, seed=3872), POTRF + POTRI on the
+    device, the inversion check, and the reference driver's output lines."""
+    r = _run_poinv_driver(["-N", "160", "-t", "32", "-nruns", "1"])
+    assert r.returncode == 0
+    assert "Inversion is CORRECT" in r.stdout
+    lines = [l for l in r.stdout.splitlines() if l.startswith("POINV (POTRF+POTRI)")]
+    assert len(lines) == 3 and all("TTG Execution Time (milliseconds) : " in l for l in lines)
+    r = _run_poinv_driver(["-N", "1536", "-t", "512", "-seq", "1", "-nruns", "1"])
+    assert r.returncode == 0
+    assert "Inversion is CORRECT" in r.stdout
+    kinds = [l.split("nthreads= ")[1].split()[1] for l in r.stdout.splitlines()
+             if l.startswith("POINV (POTRF+TRTRI+LAUUM)")]
+    assert kinds == ["POTRF", "TRTRI", "LAUUM", "TTG"]
