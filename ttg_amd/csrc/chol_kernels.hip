@@ -23,7 +23,8 @@ constexpr int kMaxTile = 512;                 // largest single-tile POTRF/TRSM
 // ---------------------------------------------------------------------------
 // POTRF (one workgroup per tile, FP64 arithmetic, left-looking by 32-column
 // blocks).  For block column j, the rows c0..n-1 of that column are held in
-// MFMA accumulators spread over the 4 waves (16-row blocks); they are updated
+// MFMA accumulators spread over the 8 waves (16-row blocks, two waves per
+// SIMD, so that one wave's load waits overlap the other's MFMAs); they are updated
 // by every earlier block column in one streamed GEMM (operands straight from
 // L2, two k-steps in flight), then the 32 x 32 diagonal block is factored and
 // inverted by one wave in registers and the rows below are multiplied by the
@@ -32,16 +33,17 @@ constexpr int kMaxTile = 512;                 // largest single-tile POTRF/TRSM
 // Float tiles: the finished factor columns go to an FP64 workspace (op.pw,
 // n x n) for the later columns to read, and rounded into the tile at once.
 // ---------------------------------------------------------------------------
-constexpr int kRowBlocks = kMaxTile / 16 / 4;  // 16-row blocks per wave (<= 8)
+constexpr int kPotrfWaves = 8;                          // two waves per SIMD
+constexpr int kRowBlocks = kMaxTile / 16 / kPotrfWaves;  // 16-row blocks per wave
 
 template <typename TS>
-__global__ __launch_bounds__(256) void potrf_kernel(const PotrfOpT<TS> *__restrict__ ops, PotrfOpT<TS> one,
+__global__ __launch_bounds__(kPotrfWaves * 64) void potrf_kernel(const PotrfOpT<TS> *__restrict__ ops, PotrfOpT<TS> one,
                                                     const int *__restrict__ d_abort) {
   if (d_abort && *d_abort) return;
   const PotrfOpT<TS> op = ops ? ops[blockIdx.x] : one;
   __shared__ double Ds[kPB][kPB + 1];
   __shared__ double Di[kPB][kPB + 1];
-  __shared__ double Xs[4][16][kPB + 1];  // per-wave staging of one 16 x 32 block
+  __shared__ double Xs[kPotrfWaves][16][kPB + 1];  // per-wave staging of one 16 x 32 block
   __shared__ int fail_s;
   // the diagonal tiles are the critical path of the factorization: issue
   // ahead of co-resident trailing-update waves
@@ -81,13 +83,13 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOpT<TS> *__restri
     d4_t acc[kRowBlocks][2];
     typedef __attribute__((address_space(1))) const double gdouble;
     const gdouble *Wg = (const gdouble *)W;  // tiles and workspace are global memory
-    const int tact = (nrb - wave + 3) >> 2;  // active row blocks of this wave (uniform)
+    const int tact = (nrb - wave + kPotrfWaves - 1) / kPotrfWaves;  // active row blocks of this wave (uniform)
     // initialise with MINUS the current column block (the GEMM below adds
     // L(rows, k) L(c0+cols, k) and the sign is restored after it); loads from
     // clamped addresses, masked afterwards, so that all of them are in flight
 #pragma unroll
     for (int t = 0; t < kRowBlocks; ++t) {
-      const int q = wave + 4 * t;
+      const int q = wave + kPotrfWaves * t;
       if (t < tact) {
 #pragma unroll
         for (int jj = 0; jj < 2; ++jj) {
@@ -123,7 +125,7 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOpT<TS> *__restri
       auto kloop = [&]<int TA>() {
         int oa[TA];
 #pragma unroll
-        for (int t = 0; t < TA; ++t) oa[t] = min(c0 + (wave + 4 * t) * 16 + lr, n - 1);
+        for (int t = 0; t < TA; ++t) oa[t] = min(c0 + (wave + kPotrfWaves * t) * 16 + lr, n - 1);
         // operand slots in flight: more when few row blocks leave little
         // MFMA work per k-step to cover the load latency
         constexpr int SL = TA >= 2 ? 2 : 4;
@@ -153,10 +155,6 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOpT<TS> *__restri
         }
       };
       switch (tact) {
-        case 8: kloop.template operator()<8>(); break;
-        case 7: kloop.template operator()<7>(); break;
-        case 6: kloop.template operator()<6>(); break;
-        case 5: kloop.template operator()<5>(); break;
         case 4: kloop.template operator()<4>(); break;
         case 3: kloop.template operator()<3>(); break;
         case 2: kloop.template operator()<2>(); break;
@@ -194,7 +192,7 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOpT<TS> *__restri
       break;
     }
     // L_D back to the working matrix; inverse blocks out for the TRSM
-    for (int e = tid; e < kPB * kPB; e += 256) {
+    for (int e = tid; e < kPB * kPB; e += kPotrfWaves * 64) {
       const int r = e & 31, c = e >> 5;
       if (r >= c && r < bs && c < bs) store(c0 + r, c0 + c, Ds[r][c]);
       if (op.dinv) gst(op.dinv)[jb * kPB * kPB + e] = (TS)Di[r][c];
@@ -203,7 +201,7 @@ __global__ __launch_bounds__(256) void potrf_kernel(const PotrfOpT<TS> *__restri
     // MFMA A-operand layout, one 16 x 32 block at a time per wave)
 #pragma unroll
     for (int t = 0; t < kRowBlocks; ++t) {
-      const int q = wave + 4 * t;
+      const int q = wave + kPotrfWaves * t;
       if (q >= 2 && q < nrb) {
 #pragma unroll
         for (int jj = 0; jj < 2; ++jj)
@@ -495,7 +493,7 @@ template <typename TS>
 hipError_t launch_potrf(hipStream_t s, const PotrfOpT<TS> *ops, int nops, const PotrfOpT<TS> &one,
                         const int *d_abort) {
   if (nops <= 0) return hipSuccess;
-  hipLaunchKernelGGL(potrf_kernel<TS>, dim3(nops), dim3(256), 0, s, ops, one, d_abort);
+  hipLaunchKernelGGL(potrf_kernel<TS>, dim3(nops), dim3(kPotrfWaves * 64), 0, s, ops, one, d_abort);
   return hipGetLastError();
 }
 
