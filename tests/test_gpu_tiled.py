@@ -174,3 +174,49 @@ def test_large_factor_solve_backward_error(dev):
     anorm = 1.0 + N * (1.0 / 3.0 + 0.05)  # upper bound of ||A||_inf for B B^T/N + I with |b| <= 1
     berr = (rn / (anorm * xn + bn)).max().item()
     assert berr <= 1e-15
+
+
+def test_dpotrf_tiled_full_size_sampled_residual(dev):
+    """The FP64 factorization at its full single-GPU size (N=32768, NB=512):
+    entries of L L^T - A on 48 sampled rows (every column up to the row),
+    held to the reference's check_dpotrf bound entry by entry,
+    |(L L^T - A)_ij| <= 60 N eps max|A| (testing_dpotrf.cc:315-324), with the
+    products on the GPU.  The oracle cannot factor this size in test time."""
+    import ttg_amd.chol as C
+
+    N, NB = 32768, 512
+    nt = N // NB
+    t = C.alloc_tiles(N, NB)
+    work = torch.empty(N * N, dtype=torch.float64, device=dev)
+    C.spd_tiled(t, N, NB, 1.0, 17, work)
+    del work
+    A0 = t.clone()
+    info = torch.full((1,), -3, dtype=torch.int32, device=dev)
+    C.dpotrf_tiled(t, N, NB, info)
+    torch.cuda.synchronize()
+    assert int(info.item()) == 0
+
+    def tidx(m, n):
+        return n * nt - n * (n - 1) // 2 + (m - n)
+
+    def row(tiles, i, upto):
+        """Row i of the lower triangle, columns 0..upto (tile-packed, column-major tiles)."""
+        m, r = divmod(i, NB)
+        parts = [tiles[tidx(m, n) * NB * NB:(tidx(m, n) + 1) * NB * NB].view(NB, NB)[:, r] for n in range(m + 1)]
+        return torch.cat(parts)[:upto + 1]
+
+    g = torch.Generator().manual_seed(5)
+    rows = sorted(set(int(v) for v in torch.randint(0, N, (48,), generator=g)))
+    amax = max(A0[tidx(k, k) * NB * NB:(tidx(k, k) + 1) * NB * NB].abs().max().item() for k in range(nt))
+    bound = 60 * N * np.finfo(np.float64).eps * amax
+    R = torch.zeros(len(rows), N, dtype=torch.float64, device=dev)
+    for a, i in enumerate(rows):
+        R[a, :i + 1] = row(t, i, i)
+    P = R @ R.t()  # (L L^T) on the sampled rows and columns
+    worst = 0.0
+    for a, i in enumerate(rows):
+        for b, j in enumerate(rows[:a + 1]):
+            m, n = i // NB, j // NB
+            aij = A0[tidx(m, n) * NB * NB + (i % NB) + (j % NB) * NB].item()
+            worst = max(worst, abs(P[a, b].item() - aij))
+    assert worst <= bound, (worst, bound)
