@@ -48,7 +48,7 @@ struct TiledPlanT {
   unsigned flags = 0;
   int lowp = 0;  // float tiles only: 0 FP32 updates, else kLowF16x2 / kLowF16 / kLowBF16
   DevBuf d_ops, d_dinv, d_info, d_pw;
-  DevBuf d_panel, d_scale, d_f16ops;  // split-FP16 panels (two slots, four when paired), scales, descriptors
+  DevBuf d_panel, d_scale, d_f16ops;  // split-FP16 panels (two slots, 2G when grouped), scales, descriptors
   std::vector<int> split_off, split_cnt;
   // grouped schedule (split-FP16 modes with lookahead): the steps in groups
   // of G; per step k of a group, the eager update of the group's later
