@@ -66,7 +66,7 @@ def parse():
                         "[32, 1024], one right-hand side each, factor + solve in one launch (autotuned)")
     p.add_argument("--batch-count", type=int, default=4096)
     p.add_argument("--poinv", action="store_true",
-                   help="POTRI (TRTRI + LAUUM) of an FP64 Cholesky factor, N=32768 unless --n")
+                   help="POTRI (TRTRI + LAUUM) of an FP64 Cholesky factor at the preset's N (65536) unless --n")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="only run a plain device copy of MIB MiB three times (calibrates the HBM byte counters) "
                         "and exit")
@@ -237,7 +237,7 @@ def run_poinv(a):
 
     import ttg_amd.chol as C
 
-    N, NB = a.n or 32768, a.nb
+    N, NB = a.n, a.nb
     t = C.alloc_tiles(N, NB)
     work = torch.empty(N * N, dtype=torch.float64, device="cuda")
     C.spd_tiled(t, N, NB, 1.0, a.seed, work)
