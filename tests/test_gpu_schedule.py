@@ -30,7 +30,7 @@ def _solve(C, dA, N, NB, Bh, nrhs, prec, dev):
     return X.t().cpu().numpy().T
 
 
-@pytest.mark.parametrize("prec,r1,r2", [(1, 6, 7), (0, 9, 11)])
+@pytest.mark.parametrize("prec,r1,r2", [(1, 6, 7), (0, 9, 11), (2, 3, 5)])
 def test_split_schedules_match_plain(dev, oracle, monkeypatch, prec, r1, r2):
     import ttg_amd.chol as C
 
@@ -49,17 +49,23 @@ def test_split_schedules_match_plain(dev, oracle, monkeypatch, prec, r1, r2):
     assert oracle.solve_backward_error(N, NB, A0, Xs, Bh[:, :r1]) <= 1e-12
 
 
-def test_paired_schedule_matches_unpaired(dev, oracle, monkeypatch):
+@pytest.mark.parametrize("G,r1", [(2, 12), (3, 14), (4, 16)])
+def test_grouped_schedule_matches_unpaired(dev, oracle, monkeypatch, G, r1):
+    """Split-FP16 solve: the grouped schedule (G-panel bulk updates, eager
+    updates inside a group) gives bitwise the solution of the one-step
+    schedule.  Each right-hand side is solved independently, so the first
+    r1 columns of an (r1 + 1)-column solve (a plan of its own) must match."""
     import ttg_amd.chol as C
 
-    N, NB = 5120, 512  # NT = 10: bulk updates of several pairs
+    N, NB = 5120, 512  # NT = 10: several groups, and a remainder group for G = 3, 4
     sigma2 = sum(k * k for k in range(-127, 128)) / 255.0 / 127.0 ** 2
     A0 = oracle.spd_tiles(N, NB, 4 * sigma2 / (1e4 - 1), 23)
-    Bh = np.random.default_rng(6).uniform(-1, 1, (N, 13))
+    Bh = np.random.default_rng(6).uniform(-1, 1, (N, r1 + 1))
     dA = to_dev(A0, dev)
     monkeypatch.setenv("TTG_CHOL_PAIRED", "1")
-    Xp = _solve(C, dA, N, NB, Bh, 12, 1, dev)
+    monkeypatch.setenv("TTG_CHOL_GROUP", str(G))
+    Xp = _solve(C, dA, N, NB, Bh, r1, C.PREC_F16X2, dev)
     monkeypatch.setenv("TTG_CHOL_PAIRED", "0")
-    Xu = _solve(C, dA, N, NB, Bh, 13, 1, dev)
-    assert np.array_equal(Xp, Xu[:, :12])
-    assert oracle.solve_backward_error(N, NB, A0, Xp, Bh[:, :12]) <= 1e-12
+    Xu = _solve(C, dA, N, NB, Bh, r1 + 1, C.PREC_F16X2, dev)
+    assert np.array_equal(Xp, Xu[:, :r1])
+    assert oracle.solve_backward_error(N, NB, A0, Xp, Bh[:, :r1]) <= 1e-12
