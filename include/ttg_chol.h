@@ -99,7 +99,10 @@ int ttg_chol_dgemm_nt(void *stream, int m, int n, int k, double alpha, const dou
                       const double *B, int ldb, double beta, double *C, int ldc);
 
 /* *d_result := ||x||_2 of n contiguous doubles.  Replaces hipblasDnrm2
- * (examples/potrf/potrf.h:85, DEBUG_TILES_VALUES builds). */
+ * (examples/potrf/potrf.h:85, DEBUG_TILES_VALUES builds).  Inside a launch
+ * group (below) the norm sees the values the group's earlier calls leave in
+ * x: a norm of memory that an operation already recorded in the group writes
+ * runs after the group's batched launches, any other norm runs at once. */
 int ttg_chol_dnrm2(void *stream, int n, const double *x, double *d_result);
 
 /* ---------------------------------------------------------------- */

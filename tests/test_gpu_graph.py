@@ -22,11 +22,12 @@ def expected_tasks(nt):
     return 2 * tiles + nt + 2 * (nt * (nt - 1) // 2) + nt * (nt - 1) * (nt - 2) // 6
 
 
-def run_driver(args):
+def run_driver(args, exe=None):
     import ttg_amd.graph as G
 
-    print("testing_dpotrf", *args, flush=True)
-    r = subprocess.run([G.DRIVER] + args, capture_output=True, text=True, timeout=100)
+    exe = exe or G.DRIVER
+    print(exe, *args, flush=True)
+    r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=100)
     print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
     return r
 
@@ -35,6 +36,20 @@ def test_driver_check_reference_case():
     r = run_driver(["-N", "160", "-t", "32"])
     assert r.returncode == 0
     assert "Factorization is CORRECT" in r.stdout
+
+
+@pytest.mark.parametrize("args", [["-N", "160", "-t", "32"], ["-N", "1000", "-t", "128"]])
+def test_driver_tile_norm_checks(args):
+    """The DEBUG_TILES_VALUES build: every task checks the 2-norms of its
+    input tiles against the norms their producers set and the result sink
+    checks the final tiles, so a run that ends CORRECT received every tile
+    its producer sent (a mismatch aborts with the task and tile named)."""
+    from ttg_amd.build import DEBUG_DRIVER
+
+    r = run_driver(args, DEBUG_DRIVER)
+    assert r.returncode == 0
+    assert "Factorization is CORRECT" in r.stdout
+    assert "producer sent" not in r.stderr
 
 
 def test_driver_timed_mode():

@@ -196,3 +196,38 @@ def test_dnrm2(dev):
     out = torch.zeros(1, dtype=torch.float64, device=dev)
     C.dnrm2(x, out)
     assert abs(out.item() - torch.linalg.norm(x).item()) <= 1e-12 * out.item()
+
+
+def test_dnrm2_in_group_sees_written_values(dev):
+    """Inside a launch group a norm of a tile that an operation recorded
+    earlier in the group writes is taken after the group's launches (it sees
+    the written tile); the norm of the same tile before its operation, and of
+    a read-only input, is taken at once (the per-task norms of the
+    tile-norm debug check)."""
+    import ttg_amd.chol as C
+
+    rng = np.random.default_rng(12)
+    n = 256
+    A = dev_colmajor(spd(n, rng), dev)
+    L = dev_colmajor(np.linalg.cholesky(spd(n, rng)), dev)
+    B = dev_colmajor(rng.uniform(-1, 1, (n, n)), dev)
+    A0, B0 = A.cpu().numpy().copy(), B.cpu().numpy().copy()
+    info = torch.zeros(1, dtype=torch.int32, device=dev)
+    norms = torch.zeros(5, dtype=torch.float64, device=dev)
+    with C.group():
+        C.dnrm2(A, norms[0:1])
+        C.dpotrf_tile(A, info)
+        C.dnrm2(A, norms[1:2])
+        C.dnrm2(B, norms[2:3])
+        C.dtrsm_rlt(L, B)
+        C.dnrm2(B, norms[3:4])
+        C.dnrm2(L, norms[4:5])
+    torch.cuda.synchronize()
+    got = norms.cpu().numpy()
+    A1, B1 = A.cpu().numpy(), B.cpu().numpy()
+    assert int(info.item()) == 0
+    assert not np.array_equal(A0, A1) and not np.array_equal(B0, B1)
+    want = [np.linalg.norm(A0), np.linalg.norm(A1), np.linalg.norm(B0), np.linalg.norm(B1),
+            np.linalg.norm(L.cpu().numpy())]
+    for g, w in zip(got, want):
+        assert abs(g - w) <= 1e-12 * w, (got, want)

@@ -41,6 +41,8 @@ INCLUDES = ["-I" + os.path.join(HERE, "include"), "-I" + os.path.join(os.path.di
             "-I" + EXAMPLES]
 GRAPH_LIB = os.path.join(HERE, "libttg_graph.so")
 DRIVER = os.path.join(HERE, "bin", "testing_dpotrf")
+# the same driver with the tile-norm consistency checks (DEBUG_TILES_VALUES)
+DEBUG_DRIVER = os.path.join(HERE, "bin", "testing_dpotrf_debug")
 POINV_DRIVER = os.path.join(HERE, "bin", "testing_dpoinv")
 # host-only C++ (no device code): clang++ with the HIP runtime headers
 CXX = os.environ.get("TTG_CXX", "/opt/rocm/llvm/bin/clang++")
@@ -58,8 +60,9 @@ def _run(cmd, what, verbose):
 
 
 def build_graph(verbose=False):
-    """The TTG-API runtime examples: libttg_graph.so and bin/testing_dpotrf,
-    and the POINV driver bin/testing_dpoinv (host C++ over libttg_chol.so)."""
+    """The TTG-API runtime examples: libttg_graph.so, bin/testing_dpotrf and
+    its DEBUG_TILES_VALUES build bin/testing_dpotrf_debug, and the POINV
+    driver bin/testing_dpoinv (host C++ over libttg_chol.so)."""
     os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
     link = ["-L" + HERE, "-lttg_chol", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/.."]
     _run([CXX] + HOST_FLAGS + ["-shared"] + INCLUDES + ["-o", GRAPH_LIB + ".tmp",
@@ -68,6 +71,10 @@ def build_graph(verbose=False):
     _run([CXX] + HOST_FLAGS + INCLUDES + ["-o", DRIVER + ".tmp", os.path.join(EXAMPLES, "testing_dpotrf.cc")] + link +
          HOST_LIBS, "testing_dpotrf", verbose)
     os.replace(DRIVER + ".tmp", DRIVER)
+    _run([CXX] + HOST_FLAGS + ["-DDEBUG_TILES_VALUES"] + INCLUDES +
+         ["-o", DEBUG_DRIVER + ".tmp", os.path.join(EXAMPLES, "testing_dpotrf.cc")] + link + HOST_LIBS,
+         "testing_dpotrf_debug", verbose)
+    os.replace(DEBUG_DRIVER + ".tmp", DEBUG_DRIVER)
     _run([CXX] + HOST_FLAGS + INCLUDES + ["-o", POINV_DRIVER + ".tmp", os.path.join(EXAMPLES, "testing_dpoinv.cc")] +
          link + HOST_LIBS, "testing_dpoinv", verbose)
     os.replace(POINV_DRIVER + ".tmp", POINV_DRIVER)

@@ -88,11 +88,36 @@ struct Group {
   std::vector<TrsmOp<double>> trsm;
   std::vector<PotrfOp> trsm_dinv;  // dinv jobs for the TRSMs (A = L tile)
   std::map<UpdKey, std::vector<UpdOp<double>>> upd;
+  struct Norm {
+    int n;
+    const double *x;
+    double *r;
+  };
+  std::vector<Norm> norm_after;  // norms of memory the group writes
   int last_launches = 0;
 };
 static thread_local Group g_group;
 
 static bool grouping(hipStream_t s) { return g_group.active && g_group.stream == s; }
+
+// [p, p + elements of an m x n column-major block with leading dimension ld)
+static bool overlaps(const double *x, int nx, const double *p, int m, int n, int ld) {
+  if (nx <= 0 || m <= 0 || n <= 0) return false;
+  const double *end = p + (long)(n - 1) * ld + m;
+  return x < end && p < x + nx;
+}
+
+bool group_defer_nrm2(hipStream_t s, int n, const double *x, double *r) {
+  if (!grouping(s)) return false;
+  Group &g = g_group;
+  bool written = false;
+  for (auto &o : g.potrf) written = written || overlaps(x, n, o.A, o.n, o.n, o.lda);
+  for (auto &o : g.trsm) written = written || overlaps(x, n, o.B, o.m, o.n, o.ldb);
+  for (auto &kv : g.upd)
+    for (auto &o : kv.second) written = written || overlaps(x, n, o.C, o.m, o.n, o.ldc);
+  if (written) g.norm_after.push_back(Group::Norm{n, x, r});
+  return written;
+}
 
 }  // namespace ttgc
 
@@ -211,6 +236,7 @@ int ttg_chol_group_begin(void *stream) {
   g_group.trsm.clear();
   g_group.trsm_dinv.clear();
   g_group.upd.clear();
+  g_group.norm_after.clear();
   return TTG_CHOL_OK;
 }
 
@@ -309,6 +335,7 @@ int ttg_chol_group_end(void *stream) {
       ++i;
     }
   }
+  for (auto &nm : g.norm_after) TTGC_HIP(launch_nrm2(s, nm.n, nm.x, nm.r));
   g.last_launches = launches;
   return TTG_CHOL_OK;
 }

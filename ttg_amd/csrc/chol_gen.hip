@@ -234,6 +234,11 @@ hipError_t launch_prepare(hipStream_t s, int N, int NB, double *tiles, int *d_in
   return launch_prepare_t<double>(s, N, NB, tiles, d_info);
 }
 
+hipError_t launch_nrm2(hipStream_t s, int n, const double *x, double *r) {
+  hipLaunchKernelGGL(nrm2_kernel, dim3(1), dim3(1024), 0, s, n, x, r);
+  return hipGetLastError();
+}
+
 }  // namespace ttgc
 
 using namespace ttgc;
@@ -335,8 +340,8 @@ int ttg_chol_dnrm2(void *stream, int n, const double *x, double *d_result) {
     set_error("dnrm2: bad argument (n=%d)", n);
     return TTG_CHOL_EINVAL;
   }
-  hipLaunchKernelGGL(nrm2_kernel, dim3(1), dim3(1024), 0, as_stream(stream), n, x, d_result);
-  TTGC_HIP(hipGetLastError());
+  if (group_defer_nrm2(as_stream(stream), n, x, d_result)) return TTG_CHOL_OK;
+  TTGC_HIP(launch_nrm2(as_stream(stream), n, x, d_result));
   return TTG_CHOL_OK;
 }
 

@@ -85,4 +85,12 @@ void release_solve_plans();
 void release_mixed_plans();
 void release_poinv_plans();
 
+// *r := ||x||_2 of n contiguous doubles (one workgroup).
+hipError_t launch_nrm2(hipStream_t s, int n, const double *x, double *r);
+// Inside a launch group open on stream s, a norm of memory that an operation
+// already recorded in the group writes must see the written values: it is
+// kept and launched after the group's batched kernels (returns true).  Any
+// other norm returns false and is launched at once, before them.
+bool group_defer_nrm2(hipStream_t s, int n, const double *x, double *r);
+
 }  // namespace ttgc

@@ -5,10 +5,17 @@
 // ttg::device::current_stream() instead of hipSOLVER/hipBLAS.  The runtime's
 // launch grouping turns all the tile calls of a scheduling round into a few
 // batched launches.
+// Built with DEBUG_TILES_VALUES, every task also takes the device 2-norm of
+// its input tiles, checks each against the norm the tile's producer set, and
+// sets the norm of its output tile (potrf.h:135-173, 271-318, 403-447,
+// 529-576): a task that received a tile other than the one its producer sent
+// aborts the run.
 #pragma once
 
+#include <array>
 #include <cassert>
 #include <iostream>
+#include <type_traits>
 #include <vector>
 
 #include "pmw.h"
@@ -26,6 +33,16 @@ inline void check_launch(int rc) {
   if (rc != TTG_CHOL_OK) ttg::detail::fatal(std::string("tile kernel: ") + ttg_chol_last_error());
 }
 
+#ifdef DEBUG_TILES_VALUES
+// device 2-norm of a whole tile (lda x cols values) into *d_norm, on the
+// task's stream (potrf.h:76-87)
+template <typename T>
+void device_norm(const MatrixTile<T> &t, double *d_norm) {
+  static_assert(std::is_same_v<T, double>, "tile norms are checked for double tiles only");
+  check_launch(ttg_chol_dnrm2(ttg::device::current_stream(), (int)t.size(), t.buffer().current_device_ptr(), d_norm));
+}
+#endif
+
 // POTRF(K): factor tile (K,K); broadcast it to the result and to TRSM(m,K),
 // m > K (potrf.h:90-222).
 template <typename MatrixT>
@@ -42,11 +59,26 @@ auto make_potrf(MatrixT &A, ttg::Edge<Key1, MatrixTile<typename MatrixT::element
     for (int m = K + 1; m < (int)A.rows(); ++m) keylist.push_back(Key2(m, K));
     int host_info = -32;
     auto dev_info = ttg::make_scratch(&host_info, ttg::scope::Allocate);
+#ifdef DEBUG_TILES_VALUES
+    const double expect_kk = tile_kk.norm();
+    std::array<double, 2> norms{};  // input and output of tile_kk
+    auto norms_s = ttg::make_scratch(norms.data(), ttg::scope::Allocate, norms.size());
+    co_await ttg::device::select(tile_kk.buffer(), dev_info, norms_s);
+    device_norm(tile_kk, norms_s.device_ptr() + 0);
+#else
     co_await ttg::device::select(tile_kk.buffer(), dev_info);
+#endif
     check_launch(ttg_chol_dpotrf_tile(ttg::device::current_stream(), (int)tile_kk.cols(),
                                       tile_kk.buffer().current_device_ptr(), (int)tile_kk.lda(),
                                       dev_info.device_ptr(), 0));
+#ifdef DEBUG_TILES_VALUES
+    device_norm(tile_kk, norms_s.device_ptr() + 1);
+    co_await ttg::device::wait(dev_info, norms_s);
+    expect_norm("POTRF", key, "tile_kk", expect_kk, norms[0]);
+    tile_kk.set_norm(norms[1]);
+#else
     co_await ttg::device::wait(dev_info);
+#endif
     if (host_info != 0) {
       std::cerr << "Factorization of tile " << K << " failed: " << host_info << std::endl;
       ttg::abort();
@@ -80,10 +112,26 @@ auto make_trsm(MatrixT &A, ttg::Edge<Key2, MatrixTile<typename MatrixT::element_
     keylist_col.reserve(A.rows() - M - 1);
     for (int n = K + 1; n < M; ++n) keylist_row.push_back(Key3(M, n, K));
     for (int m = M + 1; m < (int)A.rows(); ++m) keylist_col.push_back(Key3(m, M, K));
+#ifdef DEBUG_TILES_VALUES
+    const double expect_kk = tile_kk.norm(), expect_mk = tile_mk.norm();
+    std::array<double, 3> norms{};  // inputs tile_kk, tile_mk; output tile_mk
+    auto norms_s = ttg::make_scratch(norms.data(), ttg::scope::Allocate, norms.size());
+    co_await ttg::device::select(tile_kk.buffer(), tile_mk.buffer(), norms_s);
+    device_norm(tile_kk, norms_s.device_ptr() + 0);
+    device_norm(tile_mk, norms_s.device_ptr() + 1);
+#else
     co_await ttg::device::select(tile_kk.buffer(), tile_mk.buffer());
+#endif
     check_launch(ttg_chol_dtrsm_rlt(ttg::device::current_stream(), (int)tile_mk.rows(), (int)tile_mk.cols(),
                                     tile_kk.buffer().current_device_ptr(), (int)tile_kk.lda(),
                                     tile_mk.buffer().current_device_ptr(), (int)tile_mk.lda()));
+#ifdef DEBUG_TILES_VALUES
+    device_norm(tile_mk, norms_s.device_ptr() + 2);
+    co_await ttg::device::wait(norms_s);
+    expect_norm("TRSM", key, "tile_kk", expect_kk, norms[0]);
+    expect_norm("TRSM", key, "tile_mk", expect_mk, norms[1]);
+    tile_mk.set_norm(norms[2]);
+#endif
     co_await ttg::device::forward(ttg::device::broadcast<0, 1, 2, 3>(
         std::make_tuple(key, Key2(K, M), std::move(keylist_row), std::move(keylist_col)), std::move(tile_mk), out));
   };
@@ -106,10 +154,26 @@ auto make_syrk(MatrixT &A, ttg::Edge<Key2, MatrixTile<typename MatrixT::element_
               std::tuple<ttg::Out<Key1, MatrixTile<T>>, ttg::Out<Key2, MatrixTile<T>>> &out) -> ttg::device::Task {
     const int K = key[0], M = key[1];
     assert(tile_kk.rows() == tile_kk.cols() && tile_mk.rows() == tile_kk.rows());
+#ifdef DEBUG_TILES_VALUES
+    const double expect_mk = tile_mk.norm(), expect_kk = tile_kk.norm();
+    std::array<double, 3> norms{};  // inputs tile_mk, tile_kk; output tile_kk
+    auto norms_s = ttg::make_scratch(norms.data(), ttg::scope::Allocate, norms.size());
+    co_await ttg::device::select(tile_kk.buffer(), tile_mk.buffer(), norms_s);
+    device_norm(tile_mk, norms_s.device_ptr() + 0);
+    device_norm(tile_kk, norms_s.device_ptr() + 1);
+#else
     co_await ttg::device::select(tile_kk.buffer(), tile_mk.buffer());
+#endif
     check_launch(ttg_chol_dsyrk_ln(ttg::device::current_stream(), (int)tile_mk.rows(), (int)tile_mk.cols(), -1.0,
                                    tile_mk.buffer().current_device_ptr(), (int)tile_mk.lda(), 1.0,
                                    tile_kk.buffer().current_device_ptr(), (int)tile_kk.lda()));
+#ifdef DEBUG_TILES_VALUES
+    device_norm(tile_kk, norms_s.device_ptr() + 2);
+    co_await ttg::device::wait(norms_s);
+    expect_norm("SYRK", key, "tile_mk", expect_mk, norms[0]);
+    expect_norm("SYRK", key, "tile_kk", expect_kk, norms[1]);
+    tile_kk.set_norm(norms[2]);
+#endif
     if (M == K + 1)
       co_await ttg::device::send<0>(Key1(K + 1), std::move(tile_kk), out);
     else
@@ -134,11 +198,29 @@ auto make_gemm(MatrixT &A, ttg::Edge<Key3, MatrixTile<typename MatrixT::element_
               std::tuple<ttg::Out<Key2, MatrixTile<T>>, ttg::Out<Key3, MatrixTile<T>>> &out) -> ttg::device::Task {
     const int M = key[0], N = key[1], K = key[2];
     assert(M != N && M > K && N > K);
+#ifdef DEBUG_TILES_VALUES
+    const double expect_mk = tile_mk.norm(), expect_nk = tile_nk.norm(), expect_mn = tile_mn.norm();
+    std::array<double, 4> norms{};  // inputs tile_mk, tile_nk, tile_mn; output tile_mn
+    auto norms_s = ttg::make_scratch(norms.data(), ttg::scope::Allocate, norms.size());
+    co_await ttg::device::select(tile_mk.buffer(), tile_nk.buffer(), tile_mn.buffer(), norms_s);
+    device_norm(tile_mk, norms_s.device_ptr() + 0);
+    device_norm(tile_nk, norms_s.device_ptr() + 1);
+    device_norm(tile_mn, norms_s.device_ptr() + 2);
+#else
     co_await ttg::device::select(tile_mk.buffer(), tile_nk.buffer(), tile_mn.buffer());
+#endif
     check_launch(ttg_chol_dgemm_nt(ttg::device::current_stream(), (int)tile_mk.rows(), (int)tile_nk.rows(),
                                    (int)tile_nk.cols(), -1.0, tile_mk.buffer().current_device_ptr(),
                                    (int)tile_mk.lda(), tile_nk.buffer().current_device_ptr(), (int)tile_nk.lda(), 1.0,
                                    tile_mn.buffer().current_device_ptr(), (int)tile_mn.lda()));
+#ifdef DEBUG_TILES_VALUES
+    device_norm(tile_mn, norms_s.device_ptr() + 3);
+    co_await ttg::device::wait(norms_s);
+    expect_norm("GEMM", key, "tile_mk", expect_mk, norms[0]);
+    expect_norm("GEMM", key, "tile_nk", expect_nk, norms[1]);
+    expect_norm("GEMM", key, "tile_mn", expect_mn, norms[2]);
+    tile_mn.set_norm(norms[3]);
+#endif
     if (N == K + 1)
       co_await ttg::device::send<0>(Key2(M, N), std::move(tile_mn), out);
     else

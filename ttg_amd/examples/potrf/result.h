@@ -1,6 +1,8 @@
 // Result sink: a host task per final tile that writes it into the matrix
 // when it is not already there (examples/potrf/result.h:7-44).  The runtime
-// pushes the tile back from HBM before the task runs.
+// pushes the tile back from HBM before the task runs.  DEBUG_TILES_VALUES
+// builds check the tile's host norm against the norm its producer set
+// (result.h:14-17).
 #pragma once
 
 #include <algorithm>
@@ -13,6 +15,9 @@ auto make_result(MatrixT<T> &A, ttg::Edge<Key2, MatrixTile<T>> &result) {
   auto f = [&A](const Key2 &key, MatrixTile<T> &&tile, std::tuple<> &) {
     const int I = key[0], J = key[1];
     if (ttg::tracing()) ttg::print("RESULT(", key, ")");
+#ifdef DEBUG_TILES_VALUES
+    expect_norm("RESULT", key, "tile", tile.norm(), host_nrm2(tile.size(), tile.data()));
+#endif
     auto atile = A(I, J);
     if (atile.data() != tile.data()) std::copy_n(tile.data(), tile.rows() * tile.cols(), atile.data());
   };
