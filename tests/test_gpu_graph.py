@@ -93,3 +93,18 @@ def test_graph_dpotrf_vs_oracle(oracle, N, NB, gen):
     assert st["rounds"] >= 3 * nt - 2, st  # POTRF -> TRSM -> SYRK chain per column
     assert st["launches"] < st["tasks"], st  # rounds batch their tile calls
     assert lower_tiles_rel_err(oracle, N, NB, got, ref) <= 1e-12
+
+
+@pytest.mark.parametrize("flows", [0, 1, 2, 4, 8])
+def test_chain_benchmark(flows):
+    """The device-task chain benchmark (task-benchmarks/chain-ttg-dev.cc):
+    n + 1 dependent device tasks that forward F values (or, for F = 0,
+    only their keys) run to the end and report the reference's lines."""
+    from ttg_amd.build import CHAIN_BENCH
+
+    r = run_driver([str(flows), "-n", "500"], CHAIN_BENCH)
+    assert r.returncode == 0
+    assert f"Graph {flows} is connected." in r.stdout
+    assert "# of tasks = 500" in r.stdout
+    avg = float(r.stdout.strip().splitlines()[-1].split("avg")[-1])
+    assert 0.0 < avg < 10000.0

@@ -44,6 +44,8 @@ DRIVER = os.path.join(HERE, "bin", "testing_dpotrf")
 # the same driver with the tile-norm consistency checks (DEBUG_TILES_VALUES)
 DEBUG_DRIVER = os.path.join(HERE, "bin", "testing_dpotrf_debug")
 POINV_DRIVER = os.path.join(HERE, "bin", "testing_dpoinv")
+# device-task overhead benchmark (a chain of dependent device tasks)
+CHAIN_BENCH = os.path.join(HERE, "bin", "chain_ttg_dev")
 # host-only C++ (no device code): clang++ with the HIP runtime headers
 CXX = os.environ.get("TTG_CXX", "/opt/rocm/llvm/bin/clang++")
 HOST_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
@@ -61,8 +63,9 @@ def _run(cmd, what, verbose):
 
 def build_graph(verbose=False):
     """The TTG-API runtime examples: libttg_graph.so, bin/testing_dpotrf and
-    its DEBUG_TILES_VALUES build bin/testing_dpotrf_debug, and the POINV
-    driver bin/testing_dpoinv (host C++ over libttg_chol.so)."""
+    its DEBUG_TILES_VALUES build bin/testing_dpotrf_debug, the POINV driver
+    bin/testing_dpoinv and the device-task chain benchmark bin/chain_ttg_dev
+    (host C++ over libttg_chol.so)."""
     os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
     link = ["-L" + HERE, "-lttg_chol", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/.."]
     _run([CXX] + HOST_FLAGS + ["-shared"] + INCLUDES + ["-o", GRAPH_LIB + ".tmp",
@@ -78,6 +81,10 @@ def build_graph(verbose=False):
     _run([CXX] + HOST_FLAGS + INCLUDES + ["-o", POINV_DRIVER + ".tmp", os.path.join(EXAMPLES, "testing_dpoinv.cc")] +
          link + HOST_LIBS, "testing_dpoinv", verbose)
     os.replace(POINV_DRIVER + ".tmp", POINV_DRIVER)
+    _run([CXX] + HOST_FLAGS + INCLUDES + ["-o", CHAIN_BENCH + ".tmp",
+         os.path.join(HERE, "examples", "task-benchmarks", "chain_dev.cc")] + link + HOST_LIBS, "chain_ttg_dev",
+         verbose)
+    os.replace(CHAIN_BENCH + ".tmp", CHAIN_BENCH)
 
 
 def build(verbose=False):

@@ -10,7 +10,7 @@
 //                                                        (make_tt.h:488-569, edge.h, tt.h:30-116)
 //   sends           ttg::send<i>, ttg::sendk<i>, ttg::broadcast<i...>  (func.h:158+)
 //   device tasks    ttg::device::Task (C++20 coroutine), select, wait, forward,
-//                   device::send<i>, device::broadcast<i...>,
+//                   device::send<i>, device::sendk<i>, device::broadcast<i...>,
 //                   current_stream/current_device/num_devices  (device/task.h:89-696, device/device.h)
 //   data            ttg::Buffer<T>, ttg::devicescratch<T>, ttg::make_scratch,
 //                   ttg::scope::{Allocate, SyncIn}              (buffer.h, devicescratch.h, devicescope.h)
@@ -114,9 +114,15 @@ void print(const Ts &...ts) {
 
 namespace detail {
 
+// keys hash through their hash() member (MultiIndex), integral keys as themselves
 template <typename K>
 struct KeyHash {
-  std::size_t operator()(const K &k) const { return k.hash(); }
+  std::size_t operator()(const K &k) const {
+    if constexpr (std::is_integral_v<K>)
+      return static_cast<std::size_t>(k);
+    else
+      return k.hash();
+  }
 };
 
 [[noreturn]] inline void fatal(const std::string &what) {
@@ -657,6 +663,10 @@ template <std::size_t i, typename Key, typename V, typename... Outs>
   using O = std::tuple_element_t<i, std::tuple<Outs...>>;
   auto sp = ttg::detail::share<typename O::value_type>(std::forward<V>(v));
   return send_t{[k, sp, &out]() { std::get<i>(out).send_shared(k, sp); }};
+}
+template <std::size_t i, typename Key, typename... Outs>
+[[nodiscard]] send_t sendk(const Key &k, std::tuple<Outs...> &out) {
+  return send_t{[k, &out]() { std::get<i>(out).sendk(k); }};
 }
 template <std::size_t I, std::size_t... Is, typename Tup, typename V, typename... Outs>
 [[nodiscard]] send_t broadcast(Tup keylists, V &&v, std::tuple<Outs...> &out) {
