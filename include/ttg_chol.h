@@ -112,9 +112,9 @@ int ttg_chol_dnrm2(void *stream, int n, const double *x, double *d_result);
 /* Between group_begin(stream) and group_end(stream), the per-tile
  * operations above issued by the calling thread on `stream` are recorded
  * instead of launched; group_end launches them as one batched kernel per
- * operation kind (POTRF, TRSM, SYRK+GEMM), in that order.  Operations in one
- * group must be independent (no tile written by one op and read or written
- * by another).  This is how the task runtime fuses every ready device task
+ * operation kind (PLGSY tiles, POTRF, TRSM, SYRK+GEMM), in that order.
+ * Operations in one group must be independent (no tile written by one op
+ * and read or written by another).  This is how the task runtime fuses every ready device task
  * of a scheduling round into a few launches (no counterpart in the
  * reference, which issues one vendor call per task). */
 int ttg_chol_group_begin(void *stream);
@@ -217,7 +217,8 @@ int ttg_chol_plgsy_tiled(void *stream, int N, int NB, double bump, unsigned long
 /* CORE_dplgsy (examples/potrf/core_plgsy.h:63-150) for one m x n tile at
  * global offset (m0, n0) of a gM-row matrix, on the device; bit-identical to
  * the reference's host generator.  The device-task form of the PLGSY task
- * (examples/potrf/plgsy.h:12-33). */
+ * (examples/potrf/plgsy.h:12-33).  Inside a launch group the tiles of the
+ * group are generated by one batched launch. */
 int ttg_chol_dplgsy_tile(void *stream, double bump, int m, int n, double *A, int lda, int gM, int m0, int n0,
                          unsigned long long seed);
 

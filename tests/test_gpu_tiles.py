@@ -231,3 +231,40 @@ def test_dnrm2_in_group_sees_written_values(dev):
             np.linalg.norm(L.cpu().numpy())]
     for g, w in zip(got, want):
         assert abs(g - w) <= 1e-12 * w, (got, want)
+
+
+def test_plgsy_tiles_in_group_match_single_launches(dev, oracle):
+    """PLGSY tiles recorded in a launch group (one batched launch) are
+    bitwise the tiles of one launch each, and those of the oracle's
+    generator, ragged edge tiles and diagonal bumps included."""
+    import ttg_amd.chol as C
+
+    N, NB, bump, seed = 1000, 256, 1000.0, 3872
+    nt = oracle.nt_of(N, NB)
+    want = oracle.plgsy_tiles(N, NB, bump, seed)
+    keys = [(m, n) for n in range(nt) for m in range(n, nt)]
+
+    def gen(grouped):
+        tiles = {k: torch.full((NB, NB), float("nan"), dtype=torch.float64, device=dev).t() for k in keys}
+
+        def body():
+            for (m, n), t in tiles.items():
+                mb, nb = min(NB, N - m * NB), min(NB, N - n * NB)
+                C.dplgsy_tile(t[:mb, :nb], bump, N, m * NB, n * NB, seed)
+
+        if grouped:
+            with C.group():
+                body()
+            assert C.group_last_launches() == 1
+        else:
+            body()
+        torch.cuda.synchronize()
+        return {k: t.cpu().numpy() for k, t in tiles.items()}
+
+    single, batched = gen(False), gen(True)
+    for (m, n) in keys:
+        mb, nb = min(NB, N - m * NB), min(NB, N - n * NB)
+        a, b = single[(m, n)][:mb, :nb], batched[(m, n)][:mb, :nb]
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), (m, n)
+        ref = oracle.tile_view(want, N, NB, m, n)[:mb, :nb]
+        assert np.array_equal(a.view(np.uint64), ref.view(np.uint64)), (m, n)

@@ -94,6 +94,7 @@ struct Group {
     double *r;
   };
   std::vector<Norm> norm_after;  // norms of memory the group writes
+  std::vector<PlgsyOp> plgsy;
   int last_launches = 0;
 };
 static thread_local Group g_group;
@@ -111,12 +112,19 @@ bool group_defer_nrm2(hipStream_t s, int n, const double *x, double *r) {
   if (!grouping(s)) return false;
   Group &g = g_group;
   bool written = false;
+  for (auto &o : g.plgsy) written = written || overlaps(x, n, o.A, o.m, o.n, (int)o.lda);
   for (auto &o : g.potrf) written = written || overlaps(x, n, o.A, o.n, o.n, o.lda);
   for (auto &o : g.trsm) written = written || overlaps(x, n, o.B, o.m, o.n, o.ldb);
   for (auto &kv : g.upd)
     for (auto &o : kv.second) written = written || overlaps(x, n, o.C, o.m, o.n, o.ldc);
   if (written) g.norm_after.push_back(Group::Norm{n, x, r});
   return written;
+}
+
+bool group_record_plgsy(hipStream_t s, const PlgsyOp &op) {
+  if (!grouping(s)) return false;
+  g_group.plgsy.push_back(op);
+  return true;
 }
 
 }  // namespace ttgc
@@ -237,6 +245,7 @@ int ttg_chol_group_begin(void *stream) {
   g_group.trsm_dinv.clear();
   g_group.upd.clear();
   g_group.norm_after.clear();
+  g_group.plgsy.clear();
   return TTG_CHOL_OK;
 }
 
@@ -255,6 +264,8 @@ int ttg_chol_group_end(void *stream) {
   // one staging buffer holds every descriptor array of the group
   size_t off = 0;
   auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+  size_t o_plgsy = off;
+  off = align(off + g.plgsy.size() * sizeof(PlgsyOp));
   size_t o_potrf = off;
   off = align(off + g.potrf.size() * sizeof(PotrfOp));
   size_t o_dinvops = off;
@@ -287,6 +298,7 @@ int ttg_chol_group_end(void *stream) {
   }
   TTGC_HIP(w.host_reserve(dinv_off));
   char *host = w.host;
+  if (!g.plgsy.empty()) memcpy(host + o_plgsy, g.plgsy.data(), g.plgsy.size() * sizeof(PlgsyOp));
   if (!g.potrf.empty()) memcpy(host + o_potrf, g.potrf.data(), g.potrf.size() * sizeof(PotrfOp));
   if (!g.trsm_dinv.empty())
     memcpy(host + o_dinvops, g.trsm_dinv.data(), g.trsm_dinv.size() * sizeof(PotrfOp));
@@ -304,6 +316,12 @@ int ttg_chol_group_end(void *stream) {
     w.pending = true;
   }
   int launches = 0;
+  if (!g.plgsy.empty()) {
+    int max_n = 0;
+    for (auto &o : g.plgsy) max_n = std::max(max_n, o.n);
+    TTGC_HIP(launch_plgsy_batched(s, reinterpret_cast<PlgsyOp *>(base + o_plgsy), (int)g.plgsy.size(), max_n));
+    launches += ((int)g.plgsy.size() + 65534) / 65535;
+  }
   PotrfOp none{};
   if (!g.potrf.empty()) {
     TTGC_HIP(launch_potrf(s, reinterpret_cast<PotrfOp *>(base + o_potrf), (int)g.potrf.size(), none, nullptr));

@@ -101,6 +101,22 @@ __global__ __launch_bounds__(256) void plgsy_tile_kernel(double bump, int m, int
   }
 }
 
+// The same for a batch of tiles: blockIdx.y selects the tile.
+__global__ __launch_bounds__(256) void plgsy_batched_kernel(const PlgsyOp *__restrict__ ops) {
+  const PlgsyOp op = ops[blockIdx.y];
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64, lane = threadIdx.x & 63;
+  if (wave >= op.n) return;
+  const long gj = op.n0 + wave;
+  for (int ii = lane; ii < op.m; ii += 64) {
+    const long gi = op.m0 + ii;
+    const long hi = gi > gj ? gi : gj, lo = gi > gj ? gj : gi;
+    double v =
+        plgsy_value(rnd64_jump((unsigned long long)hi + (unsigned long long)lo * (unsigned long long)op.gM, op.seed));
+    if (op.m0 == op.n0 && gi == gj) v += op.bump;
+    op.A[ii + (long)wave * op.lda] = v;
+  }
+}
+
 // Integer-valued B (N x N, column-major, stored as doubles).
 __global__ __launch_bounds__(256) void spd_b_kernel(long N, unsigned long long seed, double *B) {
   const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
@@ -234,6 +250,18 @@ hipError_t launch_prepare(hipStream_t s, int N, int NB, double *tiles, int *d_in
   return launch_prepare_t<double>(s, N, NB, tiles, d_info);
 }
 
+hipError_t launch_plgsy_batched(hipStream_t s, const PlgsyOp *ops, int nops, int max_n) {
+  constexpr int kMaxY = 65535;  // grid y limit: launch in chunks of tiles
+  for (int i = 0; i < nops; i += kMaxY) {
+    const int cnt = std::min(kMaxY, nops - i);
+    hipLaunchKernelGGL(plgsy_batched_kernel, dim3((unsigned)((max_n + 3) / 4), (unsigned)cnt), dim3(256), 0, s,
+                       ops + i);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 hipError_t launch_nrm2(hipStream_t s, int n, const double *x, double *r) {
   hipLaunchKernelGGL(nrm2_kernel, dim3(1), dim3(1024), 0, s, n, x, r);
   return hipGetLastError();
@@ -269,6 +297,8 @@ int ttg_chol_dplgsy_tile(void *stream, double bump, int m, int n, double *A, int
     return TTG_CHOL_EINVAL;
   }
   if (m == 0 || n == 0) return TTG_CHOL_OK;
+  if (group_record_plgsy(as_stream(stream), PlgsyOp{bump, A, (long)lda, (long)gM, (long)m0, (long)n0, m, n, seed}))
+    return TTG_CHOL_OK;
   hipLaunchKernelGGL(plgsy_tile_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, as_stream(stream), bump, m, n, A,
                      (long)lda, (long)gM, (long)m0, (long)n0, seed);
   TTGC_HIP(hipGetLastError());

@@ -93,4 +93,19 @@ hipError_t launch_nrm2(hipStream_t s, int n, const double *x, double *r);
 // other norm returns false and is launched at once, before them.
 bool group_defer_nrm2(hipStream_t s, int n, const double *x, double *r);
 
+// One CORE_dplgsy tile: the m x n block at global (m0, n0) of the gM x gM
+// symmetric PLGSY matrix, into A (leading dimension lda).
+struct PlgsyOp {
+  double bump;
+  double *A;
+  long lda, gM, m0, n0;
+  int m, n;
+  unsigned long long seed;
+};
+// Batched PLGSY tiles (ops in device memory); max_n bounds every op's n.
+hipError_t launch_plgsy_batched(hipStream_t s, const PlgsyOp *ops, int nops, int max_n);
+// Inside a launch group open on stream s, records the tile for the group's
+// batched launch (returns true); otherwise returns false.
+bool group_record_plgsy(hipStream_t s, const PlgsyOp &op);
+
 }  // namespace ttgc

@@ -328,6 +328,9 @@ class Runtime {
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) fatal("no HIP device");
     hip_check(hipSetDevice(0), "hipSetDevice");
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    // TTG_ROUND_MIN / TTG_ROUND_MAX override the round sizes (below)
+    if (const char *e = std::getenv("TTG_ROUND_MIN")) min_batch = std::max(1, std::atoi(e));
+    if (const char *e = std::getenv("TTG_ROUND_MAX")) max_batch = std::max(1, std::atoi(e));
     inited_ = true;
   }
   void fini() {
@@ -353,7 +356,14 @@ class Runtime {
   long tasks_run = 0, rounds = 0, kernel_launches = 0;
   long max_round = 0;
   int max_inflight = 2;
+  // Round sizes: forming a round costs host time per task (~1 us) while the
+  // GPU may idle, so a round that starts an idle GPU takes at most min_batch
+  // ready tasks, and while earlier rounds run each round may take 1.5 times
+  // the last one (the host forms round r + 1 as the GPU runs round r), up to
+  // max_batch.
+  int min_batch = 512;
   int max_batch = 1 << 20;
+  int last_batch_ = 0;
 
   void make_resident(BufferBase *b, bool sync_in) {
     if (!b->dev) b->dev = pool_.get(b->bytes);
@@ -1159,10 +1169,12 @@ inline void Runtime::fence() {
   for (;;) {
     // host tasks first (they feed the device rounds), highest priority first
     std::vector<TaskBase *> dev_batch;
+    const int limit = inflight_.empty() ? min_batch
+                                        : std::min(max_batch, std::max(min_batch, last_batch_ + last_batch_ / 2));
     while (!ready_.empty()) {
       TaskBase *t = ready_.top();
       if (t->is_device()) {
-        if ((int)dev_batch.size() >= max_batch) break;
+        if ((int)dev_batch.size() >= limit) break;
         ready_.pop();
         dev_batch.push_back(t);
         continue;
@@ -1180,6 +1192,7 @@ inline void Runtime::fence() {
         complete(inflight_.front());
         inflight_.erase(inflight_.begin());
       }
+      last_batch_ = (int)dev_batch.size();
       run_device_round(dev_batch);
       continue;
     }
