@@ -108,3 +108,48 @@ def test_chain_benchmark(flows):
     assert "# of tasks = 500" in r.stdout
     avg = float(r.stdout.strip().splitlines()[-1].split("avg")[-1])
     assert 0.0 < avg < 10000.0
+
+
+CHILD = r"""
+import sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+import ttg_amd.graph as G
+A = np.load(sys.argv[2])
+st = G.dpotrf(int(sys.argv[3]), int(sys.argv[4]), A)
+np.save(sys.argv[5], A)
+print("rounds", st["rounds"], "max_round", st["max_round"], flush=True)
+"""
+
+
+def test_graph_factor_independent_of_round_sizes(oracle, tmp_path):
+    """Round sizes only regroup tasks that are ready together, and the graph
+    fixes every tile's update order, so the factor is bitwise the same with
+    one task per round (TTG_ROUND_MIN = TTG_ROUND_MAX = 1, set before the
+    runtime starts, hence in a child process) as with the default sizes."""
+    import os
+    import sys
+
+    import ttg_amd.graph as G
+
+    N, NB = 1000, 128
+    A0 = oracle.spd_tiles(N, NB, 1.0, 8)
+    np.save(tmp_path / "a0.npy", A0)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, TTG_ROUND_MIN="1", TTG_ROUND_MAX="1")
+    print("child: graph dpotrf N=1000 NB=128, one task per round", flush=True)
+    r = subprocess.run([sys.executable, "-c", CHILD, root, str(tmp_path / "a0.npy"), str(N), str(NB),
+                        str(tmp_path / "one.npy")], env=env, capture_output=True, text=True, timeout=100)
+    print(r.stdout[-500:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    got = A0.copy()
+    st = G.dpotrf(N, NB, got)
+    one = np.load(tmp_path / "one.npy")
+    nt = oracle.nt_of(N, NB)
+    assert st["max_round"] > 1 and "max_round 1" in r.stdout
+    assert int(r.stdout.split()[1]) == expected_tasks(nt) - 2 * (nt * (nt + 1) // 2)  # device tasks, one per round
+    for n in range(nt):
+        for m in range(n, nt):
+            a = oracle.tile_view(got, N, NB, m, n)
+            b = oracle.tile_view(one, N, NB, m, n)
+            assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), (m, n)
