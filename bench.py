@@ -61,6 +61,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-profile", action="store_true")
     p.add_argument("--no-fp64", action="store_true", help="skip the plain-FP64 comparison solve")
+    p.add_argument("--no-graph", action="store_true",
+                   help="skip the task-graph driver run (testing_dpotrf at the workload's N, FP64)")
     p.add_argument("--batched", action="store_true",
                    help="the batched workload instead: --batch-count FP32 SPD systems B B^T/n + I, n uniform in "
                         "[32, 1024], one right-hand side each, factor + solve in one launch (autotuned)")
@@ -312,6 +314,30 @@ def read_traffic(mode, N, NB, nrhs):
     return None
 
 
+def graph_driver(n, nb):
+    """The FP64 factorization through the task graph and the reference's own
+    driver line (ttg_amd/bin/testing_dpotrf -N n -t nb -x -n 2: device PLGSY,
+    factorization and write-backs timed, the second run reported), in a child
+    process.  Reported beside the metric, not part of it."""
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ttg_amd", "bin", "testing_dpotrf")
+    if not os.path.exists(exe):
+        return {"error": "ttg_amd/bin/testing_dpotrf is not built"}
+    args = [exe, "-N", str(n), "-t", str(nb), "-x", "-n", "2"]
+    try:
+        r = subprocess.run(args, capture_output=True, text=True, timeout=240)
+    except subprocess.TimeoutExpired:
+        return {"error": "timed out"}
+    lines = [l for l in r.stdout.splitlines() if l.startswith("TTG Execution Time (milliseconds) : ")]
+    if r.returncode != 0 or not lines:
+        return {"error": f"exit {r.returncode}"}
+    f = lines[-1].split(":")
+    ms, gf = float(f[1]), float(f[2].split()[-2])
+    return {"command": " ".join(["ttg_amd/bin/testing_dpotrf"] + args[1:]), "ms": ms, "tflops": round(gf / 1e3, 3),
+            "line": lines[-1]}
+
+
 def cpu_baseline(n, nb, nrhs):
     """The oracle (C restatement of the examples/potrf graph, priority thread
     pool on every host core) on a bounded sample: FP64 factor + solve of
@@ -476,6 +502,7 @@ def main():
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(a.cpu_n, 256, nrhs)
+    graph = None if a.no_graph else graph_driver(N, NB)
 
     out = {
         "metric": metric_name(a),
@@ -502,6 +529,7 @@ def main():
         "wall_s": round(wall, 3),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "graph_driver": graph,
     }
     print(json.dumps(out), flush=True)
 
