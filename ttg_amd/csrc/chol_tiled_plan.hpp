@@ -56,6 +56,7 @@ struct TiledPlanT {
   // columns (bulkA) and of all further columns (bulkB)
   bool paired = false;
   int G = 2;
+  int TG = 2;  // side of the square tile groups of the bulk updates (TTG_CHOL_TILE_GROUP)
   std::vector<int> pe_off, pe_cnt, pa_off, pa_cnt, pb_off, pb_cnt;  // eager per step; bulkA, bulkB per group
   hipEvent_t ev_bulk = nullptr;
   size_t o_split = 0, o_f16 = 0;
@@ -85,17 +86,19 @@ struct TiledPlanT {
   UpdOp<T> *upd_ops() { return reinterpret_cast<UpdOp<T> *>(static_cast<char *>(d_ops.ptr) + o_upd); }
   T *tile(int m, int n) { return tiles + tile_index(m, n, NT) * (long)NB * NB; }
 
-  // Tiles (m, n), k + 2 <= n <= m < NT, of updB(k) in launch order: 2 x 2
-  // groups of tiles, so that the workgroups running together on one XCD
-  // share two row panels and two column panels (measured: 6% less time per
-  // update launch than column-by-column order, from L2 hits).  The order
-  // within a launch does not change any tile's update order over k.
+  // Tiles (m, n), k + 2 <= n <= m < NT, of updB(k) in launch order: TG x TG
+  // groups of tiles (default 2), so that the workgroups running together on
+  // one XCD share TG row panels and TG column panels (measured 6% less time
+  // per update launch than column-by-column order with an earlier update
+  // kernel; with the current one, sides 1 to 4 give the same solve time
+  // within noise).  The order within a launch does not change any tile's
+  // update order over k.
   template <typename F>
   void updB_tiles(int k, F &&f) const {
-    for (int n0 = k + 2; n0 < NT; n0 += 2)
-      for (int m0 = n0; m0 < NT; m0 += 2)
-        for (int dn = 0; dn < 2; ++dn)
-          for (int dm = 0; dm < 2; ++dm) {
+    for (int n0 = k + 2; n0 < NT; n0 += TG)
+      for (int m0 = n0; m0 < NT; m0 += TG)
+        for (int dn = 0; dn < TG; ++dn)
+          for (int dm = 0; dm < TG; ++dm) {
             const int m = m0 + dm, n = n0 + dn;
             if (m < NT && n < NT && m >= n) f(m, n);
           }
@@ -103,6 +106,7 @@ struct TiledPlanT {
 
   int build() {
     NT = (N + NB - 1) / NB;
+    if (const char *et = getenv("TTG_CHOL_TILE_GROUP")) TG = std::max(1, std::min(8, atoi(et)));
     const int dstride = ((NB + 31) / 32) * 32 * 32;
     TTGC_HIP(d_dinv.reserve(sizeof(T) * dstride));
     TTGC_HIP(d_info.reserve(sizeof(int)));
