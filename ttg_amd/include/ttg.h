@@ -1111,14 +1111,20 @@ inline void Runtime::run_device_round(std::vector<TaskBase *> &batch) {
   }
   if (ttg_chol_group_end(stream_) != TTG_CHOL_OK) fatal(ttg_chol_last_error());
   kernel_launches += ttg_chol_group_last_launches();
-  if (events_.empty()) {
-    hipEvent_t e;
-    hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
-    events_.push_back(e);
+  // a round that left no work on the stream (nothing launched or copied, and
+  // everything before it done) is complete now: no event, no wait
+  const hipError_t q = hipStreamQuery(stream_);
+  if (q != hipSuccess && q != hipErrorNotReady) hip_check(q, "hipStreamQuery");
+  if (q == hipErrorNotReady) {
+    if (events_.empty()) {
+      hipEvent_t e;
+      hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+      events_.push_back(e);
+    }
+    r.done = events_.back();
+    events_.pop_back();
+    hip_check(hipEventRecord(r.done, stream_), "hipEventRecord");
   }
-  r.done = events_.back();
-  events_.pop_back();
-  hip_check(hipEventRecord(r.done, stream_), "hipEventRecord");
   ++rounds;
   max_round = std::max<long>(max_round, (long)batch.size());
   inflight_.push_back(std::move(r));
@@ -1126,8 +1132,10 @@ inline void Runtime::run_device_round(std::vector<TaskBase *> &batch) {
 
 inline void Runtime::complete(Round &r) {
   using P = device::Task::promise_type;
-  hip_check(hipEventSynchronize(r.done), "hipEventSynchronize");
-  events_.push_back(r.done);
+  if (r.done) {
+    hip_check(hipEventSynchronize(r.done), "hipEventSynchronize");
+    events_.push_back(r.done);
+  }
   for (auto &[t, hv] : r.tasks) {
     auto h = std::coroutine_handle<P>::from_address(hv.address());
     P &p = h.promise();
