@@ -27,7 +27,8 @@
 //     recorded and launched as one batched kernel per kind.  A round's sends
 //     run when its kernels have completed (the deferred-send semantics of
 //     device/task.h:205-213, 677-682).  Up to two rounds are in flight, so the
-//     host prepares round r+1 while the GPU runs round r.
+//     host prepares round r+1 while the GPU runs round r; a round that left
+//     no work on the stream completes at once.
 //   * Host tasks run on the calling thread; values they receive are pushed
 //     back to the host first (the push-out of parsec/ttg.h:3560-3562).
 //   * Values travel as shared objects: a broadcast shares one value among all
