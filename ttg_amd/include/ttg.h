@@ -329,6 +329,7 @@ class Runtime {
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) fatal("no HIP device");
     hip_check(hipSetDevice(0), "hipSetDevice");
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipStreamCreateWithFlags(&copy_, hipStreamNonBlocking), "hipStreamCreate");
     // TTG_ROUND_MIN / TTG_ROUND_MAX override the round sizes (below)
     if (const char *e = std::getenv("TTG_ROUND_MIN")) min_batch = std::max(1, std::atoi(e));
     if (const char *e = std::getenv("TTG_ROUND_MAX")) max_batch = std::max(1, std::atoi(e));
@@ -339,6 +340,7 @@ class Runtime {
     fence();
     pool_.release();
     (void)hipStreamDestroy(stream_);
+    (void)hipStreamDestroy(copy_);
     inited_ = false;
   }
   bool inited() const { return inited_; }
@@ -374,8 +376,8 @@ class Runtime {
   }
   void push_out(BufferBase *b) {  // device -> host, synchronous
     if (b->dev && b->dev_valid && !b->host_valid && b->host && b->bytes) {
-      hip_check(hipMemcpyAsync(b->host, b->dev, b->bytes, hipMemcpyDeviceToHost, stream_), "D2H");
-      hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+      hip_check(hipMemcpyAsync(b->host, b->dev, b->bytes, hipMemcpyDeviceToHost, copy_), "D2H");
+      hip_check(hipStreamSynchronize(copy_), "hipStreamSynchronize");
     }
     b->host_valid = true;
   }
@@ -398,6 +400,9 @@ class Runtime {
   bool inited_ = false;
   int nthreads_ = 1;
   hipStream_t stream_ = nullptr;
+  // push-outs to host tasks: their data comes from rounds that have
+  // completed, so the copy need not wait behind the rounds queued since
+  hipStream_t copy_ = nullptr;
   DevicePool pool_;
   std::priority_queue<TaskBase *, std::vector<TaskBase *>, TaskOrder> ready_;
   std::vector<Round> inflight_;
