@@ -39,6 +39,9 @@ def sigma2_bint():
     return sum(k * k for k in range(-127, 128)) / 255.0 / (127.0 * 127.0)
 
 
+SIGMA2_UNIFORM = 1.0 / 3.0  # variance of B uniform on [-1, 1)
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -57,7 +60,15 @@ def parse():
     p.add_argument("--max-iter", type=int, default=30)
     p.add_argument("--tol", type=float, default=1e-12)
     p.add_argument("--seed", type=int, default=2024)
-    p.add_argument("--cpu-n", type=int, default=16384, help="order of the bounded CPU-baseline sample")
+    p.add_argument("--b-dist", choices=["uniform", "int"], default="uniform",
+                   help="B of A = B B^T/N + alpha I: FP64 uniform on [-1,1) (the configurations' construction) or "
+                        "k/127 with k a uniform integer in [-127,127] (exact products, the test suite's)")
+    p.add_argument("--cpu-n", type=int, default=4096, help="order of the CPU baseline (N=4096, NB=256: cfg1)")
+    p.add_argument("--cpu-nb", type=int, default=256)
+    p.add_argument("--cpu-plgsy-n", type=int, default=16384,
+                   help="order of the secondary CPU sample on the reference's PLGSY matrix (0: skip)")
+    p.add_argument("--no-check", action="store_true",
+                   help="skip the factor backward errors (||A - L L^T|| of the mixed and FP64 factors)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-profile", action="store_true")
     p.add_argument("--no-fp64", action="store_true", help="skip the plain-FP64 comparison solve")
@@ -338,37 +349,79 @@ def graph_driver(n, nb):
             "line": lines[-1]}
 
 
-def cpu_baseline(n, nb, nrhs):
-    """The oracle (C restatement of the examples/potrf graph, priority thread
-    pool on every host core) on a bounded sample: FP64 factor + solve of
-    PLGSY(n, bump=n, seed=3872)."""
+def cpu_baseline(n, nb, nrhs, plgsy_n, seed, niter=3):
+    """The oracle (C restatement of the examples/potrf graph: the same keys,
+    priorities and tile updates, on a priority thread pool over every host
+    core, tile kernels in plain C) on cfg1: N=4096, NB=256, A = B B^T/N + I
+    with B FP64 uniform on [-1, 1), 16 right-hand sides, niter runs, the
+    median reported.  Per run: generation of A (the reference driver times
+    its generator with the factorization, testing_dpotrf.cc:145-161), factor,
+    solve.  Plus a secondary sample on the reference's own PLGSY matrix."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import statistics
+
     import numpy as np
     import oracle as O
 
     cores = len(os.sched_getaffinity(0))
-    tg = time.perf_counter()
-    tiles = O.plgsy_tiles(n, nb, n, 3872)
-    t_gen = time.perf_counter() - tg
-    rng = np.random.default_rng(1)
+    F = flops_potrf(n)
+    rng = np.random.default_rng(seed)
     B = np.asfortranarray(rng.uniform(-1, 1, (n, nrhs)))
-    t0 = time.perf_counter()
-    info = O.lib().oracle_dpotrf_tiled(n, nb, O.ptr(tiles), cores)
-    t_fac = time.perf_counter() - t0
-    O.lib().oracle_dpotrs_tiled(n, nb, O.ptr(tiles), nrhs, O.ptr(B), n)
-    dt = time.perf_counter() - t0
-    if info != 0:
-        raise RuntimeError(f"oracle factorization failed, info={info}")
-    work = flops_potrf(n) + 2.0 * n * n * nrhs
-    return {"value": work / dt / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "port",
-            "sample": f"oracle FP64 tiled factor + {nrhs}-RHS solve of PLGSY N={n} NB={nb} "
-                      f"(bump=N, seed=3872) on {cores} threads, {dt:.2f} s",
-            # the reference driver's own convention (testing_dpotrf.cc:145-161) times
-            # generation and factorization together; factor-only is the GF/s it prints
-            # without generation
-            "factor_only_s": round(t_fac, 3), "generation_s": round(t_gen, 3),
-            "factor_gflops_with_generation": round(flops_potrf(n) / (t_fac + t_gen) / 1e9, 2),
-            "factor_gflops": round(flops_potrf(n) / t_fac / 1e9, 2)}
+    runs, lines = [], []
+    for _ in range(niter):
+        tg = time.perf_counter()
+        tiles = O.spd_uniform_tiles(n, nb, 1.0, seed)
+        t0 = time.perf_counter()
+        info = O.lib().oracle_dpotrf_tiled(n, nb, O.ptr(tiles), cores)
+        t1 = time.perf_counter()
+        X = B.copy(order="F")
+        O.lib().oracle_dpotrs_tiled(n, nb, O.ptr(tiles), nrhs, O.ptr(X), n)
+        t2 = time.perf_counter()
+        if info != 0:
+            raise RuntimeError(f"oracle factorization failed, info={info}")
+        runs.append((t0 - tg, t1 - t0, t2 - t1))
+        ms = (t1 - tg) * 1e3
+        # testing_dpotrf.cc:160-161 prints this line per iteration
+        lines.append(f"TTG Execution Time (milliseconds) : {ms:g} : Flops {F:g} {F / 1e9 / (ms / 1e3):g} GF/s")
+    gen = statistics.median(r[0] for r in runs)
+    fac = statistics.median(r[1] for r in runs)
+    sol = statistics.median(r[1] + r[2] for r in runs)
+    work = F + 2.0 * n * n * nrhs
+    out = {"value": work / sol / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "port",
+           "sample": f"cfg1: oracle FP64 tiled factor + {nrhs}-RHS solve of A = B B^T/N + I, N={n}, NB={nb}, "
+                     f"B FP64 uniform[-1,1) (seed {seed}), {cores} threads, median of {niter} runs",
+           "backend": "oracle C tile kernels (register-blocked C loops, gcc -O3 -march=x86-64-v3) on a "
+                      "priority-ordered pthread pool; no vendor BLAS",
+           "niter": niter, "N": n, "NB": nb, "nrhs": nrhs,
+           "factor_ms_median": round(fac * 1e3, 3), "factor_gflops": round(F / fac / 1e9, 2),
+           "generation_ms_median": round(gen * 1e3, 3),
+           "factor_gflops_with_generation": round(F / (fac + gen) / 1e9, 2),
+           "reference_lines": lines,
+           "backward_error": O.solve_backward_error(n, nb, O.spd_uniform_tiles(n, nb, 1.0, seed), X, B)}
+    for l in lines:
+        print(l, file=sys.stderr, flush=True)
+    if plgsy_n:
+        tg = time.perf_counter()
+        t = O.plgsy_tiles(plgsy_n, nb, plgsy_n, 3872)
+        t0 = time.perf_counter()
+        info = O.lib().oracle_dpotrf_tiled(plgsy_n, nb, O.ptr(t), cores)
+        t1 = time.perf_counter()
+        if info != 0:
+            raise RuntimeError(f"oracle PLGSY factorization failed, info={info}")
+        Fp = flops_potrf(plgsy_n)
+        out["plgsy_sample"] = {
+            "workload": f"PLGSY N={plgsy_n} NB={nb} (bump=N, seed=3872, the reference driver's matrix), factor",
+            "factor_s": round(t1 - t0, 3), "generation_s": round(t0 - tg, 3),
+            "factor_gflops": round(Fp / (t1 - t0) / 1e9, 2),
+            "factor_gflops_with_generation": round(Fp / (t1 - tg) / 1e9, 2)}
+    return out
+
+
+def factor_check(C, A0, L, N, NB):
+    """||A - L L^T||_inf / (||A||_inf N eps) of testing_dpotrf.cc:277-335,
+    on the device in tiles (L L^T formed in FP64)."""
+    a, r, q = C.dcheck_dpotrf_tiled(A0, L, N, NB)
+    return {"anorm_inf": a, "residual_inf": r, "rel": r / a, "scaled": q, "passes_reference_bound_60": q <= 60.0}
 
 
 def main():
@@ -396,12 +449,12 @@ def main():
     stream = torch.cuda.current_stream()
 
     # ---- inputs, resident in HBM before the timed region
-    s2 = sigma2_bint()
+    s2 = SIGMA2_UNIFORM if a.b_dist == "uniform" else sigma2_bint()
     lam_max = 4.0 * s2  # Marchenko-Pastur edge of B B^T / N for square B
     alpha = lam_max / (a.kappa - 1.0)
     A0 = C.alloc_tiles(N, NB)
     work = torch.empty(N * N, dtype=torch.float64, device=dev)
-    C.spd_tiled(A0, N, NB, alpha, a.seed, work)
+    (C.spd_uniform_tiled if a.b_dist == "uniform" else C.spd_tiled)(A0, N, NB, alpha, a.seed, work)
     del work
     torch.cuda.empty_cache()
     g = torch.Generator(device="cpu").manual_seed(a.seed)
@@ -409,17 +462,6 @@ def main():
     X = torch.empty_like(B.t()).t()
     info = torch.zeros(1, dtype=torch.int32, device=dev)
     R = torch.empty_like(X)
-
-    # ||A||_inf from the residual of X = 1 against |A| is overkill; bound it by
-    # its definition on the device instead: row sums of |A| via one residual
-    # with B = 0, X = 1 on the absolute values.
-    Aabs = A0.abs()
-    ones = torch.ones(N, 1, dtype=torch.float64, device=dev)
-    zeros = torch.zeros(N, 1, dtype=torch.float64, device=dev)
-    rs = torch.empty(N, 1, dtype=torch.float64, device=dev)
-    C.dresidual_tiled(Aabs, N, NB, ones, zeros, rs)
-    anorm = float((-rs).max().item())
-    del Aabs
 
     prec = {"f16x2": C.PREC_F16X2, "fp32": C.PREC_FP32, "fp64": C.PREC_FP64, "f16": C.PREC_F16, "bf16": C.PREC_BF16}
     iters = {}
@@ -431,11 +473,7 @@ def main():
         return step
 
     def backward_error():
-        C.dresidual_tiled(A0, N, NB, X, B, R)
-        rn = R.abs().max(dim=0).values
-        xn = X.abs().max(dim=0).values
-        bn = B.abs().max(dim=0).values
-        return float((rn / (anorm * xn + bn)).max().item())
+        return C.backward_error(A0, N, NB, X, B)
 
     def timed(mode, steps, warmup):
         step = make_step(mode)
@@ -460,10 +498,30 @@ def main():
 
     ms_step, wall, berr = timed(a.mode, a.steps, a.warmup)
     it_main = iters[a.mode][0]
+    checks = {}
+    L64 = None
+    if not a.no_check:
+        L64 = C.alloc_tiles(N, NB)
+        C.dposv_mixed_factor(L64, N, NB, nrhs, prec[a.mode])
+        checks[a.mode] = factor_check(C, A0, L64, N, NB)
+    X_main = X.clone()
     other = "fp64" if a.mode != "fp64" and not a.no_fp64 else None
     ms_fp64 = berr_fp64 = None
+    x_diff = factor_diff = None
     if other:
         ms_fp64, _, berr_fp64 = timed(other, 1, 1)
+        # the refined solution against the plain-FP64 one, per right-hand side
+        x_diff = float(((X_main - X).abs().amax(dim=0) / X.abs().amax(dim=0)).max().item())
+        if L64 is not None:
+            L64d = C.alloc_tiles(N, NB)
+            C.dposv_mixed_factor(L64d, N, NB, nrhs, prec[other])
+            checks[other] = factor_check(C, A0, L64d, N, NB)
+            # the strict upper parts of the diagonal tiles hold the same input
+            # values in both factors and add nothing to the difference
+            factor_diff = float(((L64 - L64d).abs().max() / L64d.abs().max()).item())
+            del L64d
+    del L64
+    torch.cuda.empty_cache()
     F = flops_potrf(N) + 2.0 * N * N * nrhs
     value = F / (ms_step * 1e-3) / 1e12
 
@@ -483,13 +541,18 @@ def main():
         ms_u, n_u, fl_u = C.profile_read(C.PROF_UPDATE)
         ms_t, n_t, fl_t = C.profile_read(C.PROF_TRSM)
         ms_p, n_p, fl_p = C.profile_read(C.PROF_POTRF)
-        peak = {"f16x2": FP16_PEAK / 3.0, "fp32": FP32_PEAK, "fp64": FP64_PEAK, "f16": FP16_PEAK,
+        # the published dense peak of the unit the kernel runs on; the split
+        # FP16 kernel issues three MFMA products per credited flop, so its
+        # MFMA pipe utilisation is 3x its fraction of the FP16 peak
+        peak = {"f16x2": FP16_PEAK, "fp32": FP32_PEAK, "fp64": FP64_PEAK, "f16": FP16_PEAK,
                 "bf16": FP16_PEAK}[a.mode]
+        products = 3.0 if a.mode == "f16x2" else 1.0
         ach = fl_u / (ms_u * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": round(ach, 3), "peak": round(peak / 1e12, 1), "unit": "TFLOP/s",
                 "frac": round(ach / (peak / 1e12), 4), "traffic": read_traffic(a.mode, N, NB, nrhs),
+                "mfma_pipe_util": round(products * ach / (peak / 1e12), 4),
                 "kernel": {"f16x2": "update_f16x2_kernel (batched SYRK+GEMM, 3 FP16 MFMA products per output; "
-                                    "peak = FP16 dense / 3, flops counted once)",
+                                    "achieved counts each output flop once against the FP16 dense peak)",
                            "fp32": "update_kernel<float> (batched SYRK+GEMM trailing update)",
                            "fp64": "update_kernel<double> (batched SYRK+GEMM trailing update)",
                            "f16": "update_f16x2_kernel<FP16, one product> (batched SYRK+GEMM)",
@@ -501,7 +564,7 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(a.cpu_n, 256, nrhs)
+        cpu = cpu_baseline(a.cpu_n, a.cpu_nb, nrhs, a.cpu_plgsy_n, a.seed)
     graph = None if a.no_graph else graph_driver(N, NB)
 
     out = {
@@ -517,12 +580,20 @@ def main():
         "vs_baseline": None,
         "dtype": {"f16x2": "f16x2+f32+f64", "fp32": "f32+f64", "fp64": "f64", "f16": "f16+f32+f64",
                   "bf16": "bf16+f32+f64"}[a.mode],
-        "data": "synthetic: A = B B^T/N + alpha I, B(i,j) = k/127 with k uniform integer in [-127,127] "
-                f"(seeded LCG), alpha for kappa ~ {a.kappa:g}; {nrhs} RHS uniform[-1,1]",
+        "data": ("synthetic: A = B B^T/N + alpha I, B FP64 uniform[-1,1) (seeded LCG) "
+                 if a.b_dist == "uniform" else
+                 "synthetic: A = B B^T/N + alpha I, B(i,j) = k/127 with k uniform integer in [-127,127] (seeded LCG) ")
+                + f"alpha = (4 sigma^2)/(kappa-1) for kappa ~ {a.kappa:g}; {nrhs} RHS uniform[-1,1]",
         "config": {"workload": f"SPD solve N={N} NB={NB} nrhs={nrhs} kappa~{a.kappa:g} (tiled POTRF graph)",
                    "N": N, "NB": NB, "nrhs": nrhs, "mode": a.mode, "parallelism": "single GPU"},
         "backward_error": berr,
         "refinement_steps": it_main,
+        "solve_tflops_frac": {"fp64_peak": round(value / (FP64_PEAK / 1e12), 4),
+                              "fp32_peak": round(value / (FP32_PEAK / 1e12), 4),
+                              "fp16_dense_peak": round(value / (FP16_PEAK / 1e12), 4)},
+        "factor_backward_error": checks or None,
+        "x_rel_diff_vs_fp64": x_diff,
+        "factor_rel_diff_vs_fp64": factor_diff,
         "fp64_ms_per_solve": None if ms_fp64 is None else round(ms_fp64, 3),
         "fp64_backward_error": berr_fp64,
         "speedup_vs_fp64": None if ms_fp64 is None else round(ms_fp64 / ms_step, 3),

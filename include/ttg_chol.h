@@ -205,6 +205,30 @@ int ttg_chol_dpotri_tiled(void *stream, int N, int NB, double *tiles, int *d_inf
 int ttg_chol_dposv_mixed(void *stream, int N, int NB, const double *A, int nrhs, const double *B, int ldb, double *X,
                          int ldx, int prec, double tol, int max_iter, int *d_info, int *iters, double *berr);
 
+/* The working factor of the last ttg_chol_dposv_mixed call with the same
+ * (N, NB, nrhs, prec), widened to FP64, into L (device, num_tiles(N, NB) *
+ * NB * NB doubles, the lower tile-packed layout): L in the lower part, the
+ * input's values in the strict upper part of the diagonal tiles (POTRF does
+ * not touch them), the padding of the edge tiles as the factorization set
+ * it.  TTG_CHOL_ESTATE if no such call has run since the plans were built.
+ * (No counterpart in the reference: it exposes the factor that the
+ * mixed-precision solve keeps internally, so that its backward error can
+ * be stated.) */
+int ttg_chol_dposv_mixed_factor(void *stream, int N, int NB, int nrhs, int prec, double *L);
+
+/* The factorization check of examples/potrf/testing_dpotrf.cc:277-335 in
+ * tiles on the device: d_out[0] = ||A0||_inf, d_out[1] = ||L L^T - A0||_inf,
+ * d_out[2] = d_out[1] / (d_out[0] N eps), eps = DBL_EPSILON (the reference
+ * passes when d_out[2] <= 60).  A0 and L are FP64 lower tiles (device): L is
+ * read from the lower triangle of each diagonal tile only (the trmm of
+ * :298-299); the residual is mirrored from its lower part (:302-310);
+ * ||A0||_inf is taken over the lower tiles as stored, diagonal tiles whole,
+ * as getLAPACKMatrix (pmw.h:122-139) would hold them (:312).  L L^T is formed
+ * in FP64 on MFMA in one batched launch (N^3/3 flops).  Uses library-owned
+ * workspaces of 2 num_tiles(N, NB) NB^2 doubles and waits for its kernels
+ * before it returns.  NB <= TTG_CHOL_MAX_TILE. */
+int ttg_chol_dcheck_dpotrf_tiled(void *stream, int N, int NB, const double *A0, const double *L, double *d_out);
+
 /* ---------------------------------------------------------------- */
 /* generators                                                        */
 /* ---------------------------------------------------------------- */
@@ -222,13 +246,23 @@ int ttg_chol_plgsy_tiled(void *stream, int N, int NB, double bump, unsigned long
 int ttg_chol_dplgsy_tile(void *stream, double bump, int m, int n, double *A, int lda, int gM, int m0, int n0,
                          unsigned long long seed);
 
-/* A = B B^T / N + alpha I into the lower tiles, with B(i,j) = k/127, k an
- * integer in [-127, 127] drawn from the PLASMA LCG stream position i + j*N.
- * B B^T is formed exactly (integer data in FP64 MFMA), so A is bit-identical
- * to the CPU construction.  `work` must hold N*N doubles (the integer B).
- * (No counterpart in the reference: the benchmark's SPD construction.) */
+/* A = B B^T / N + alpha I into the lower tiles (diagonal tiles whole), with
+ * B(i,j) = k/127, k an integer in [-127, 127] drawn from the PLASMA LCG
+ * stream position i + j*N.  B B^T is formed exactly (integer data in FP64
+ * MFMA), so A is bit-identical to the CPU construction.  `work` must hold
+ * N*N doubles (the integer B).
+ * (No counterpart in the reference: the test suite's SPD construction.) */
 int ttg_chol_spd_tiled(void *stream, int N, int NB, double alpha, unsigned long long seed, double *tiles,
                        double *work);
+
+/* The same construction with B FP64 uniform on [-1, 1): B(i,j) =
+ * 2 (ran >> 11) 2^-53 - 1 from LCG stream position i + j*N (bit-identical
+ * to the CPU construction), A = B B^T / N + alpha I formed on FP64 MFMA
+ * (rounded: equal to a CPU product up to summation order).  Diagonal tiles
+ * are written whole.  `work` holds N*N doubles.  (No counterpart in the
+ * reference: the benchmark configurations' "B fp64 uniform[-1,1]".) */
+int ttg_chol_spd_uniform_tiled(void *stream, int N, int NB, double alpha, unsigned long long seed, double *tiles,
+                               double *work);
 
 /* ---------------------------------------------------------------- */
 /* diagnostics                                                       */

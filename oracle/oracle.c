@@ -167,9 +167,17 @@ static void gemm_nt_acc(int m, int n, int k, double alpha, const double *restric
                         const double *restrict B, int ldb, double *restrict C, int ldc);
 static int host_cores(void);
 
+/* B(i,j) of the uniform construction: 2 u - 1 with u = (ran >> 11) 2^-53,
+ * ran the LCG state at stream position i + j*N.  Both steps are exact, so B
+ * takes every value k 2^-52 - 1, k < 2^53, of [-1, 1). */
+double oracle_spd_uval(long i, long j, long N, unsigned long long seed) {
+  unsigned long long ran = oracle_rnd64_jump((unsigned long long)(i + j * N), seed);
+  return 2.0 * ((double)(ran >> 11) * 0x1p-53) - 1.0;
+}
+
 typedef struct {
   int N, NB, nt, nthreads, tid;
-  double alpha;
+  double alpha, c;
   const double *B;
   double *tiles;
 } spd_arg;
@@ -177,7 +185,7 @@ typedef struct {
 static void *spd_worker(void *p) {
   spd_arg *a = (spd_arg *)p;
   const int N = a->N, NB = a->NB, nt = a->nt;
-  const double c = 1.0 / (127.0 * 127.0 * (double)N);
+  const double c = a->c;
   long idx = 0;
   for (int n = 0; n < nt; n++)
     for (int m = n; m < nt; m++, idx++) {
@@ -186,8 +194,9 @@ static void *spd_worker(void *p) {
       int mb = N - m * NB < NB ? N - m * NB : NB, nb = N - n * NB < NB ? N - n * NB : NB;
       for (int j = 0; j < nb; j++)
         for (int i = 0; i < mb; i++) t[i + (long)j * NB] = 0.0;
-      /* integer-valued operands: every product and partial sum is an exact
-       * integer below 2^53, so the blocked kernel's order does not matter */
+      /* integer-valued operands (oracle_spd_tiled): every product and
+       * partial sum is an exact integer below 2^53, so the blocked kernel's
+       * order does not matter; uniform operands round in its k order */
       gemm_nt_acc(mb, nb, N, 1.0, a->B + (long)m * NB, N, a->B + (long)n * NB, N, t, NB);
       for (int j = 0; j < nb; j++)
         for (int i = 0; i < mb; i++) {
@@ -197,6 +206,20 @@ static void *spd_worker(void *p) {
         }
     }
   return NULL;
+}
+
+/* tiles := c B B^T + alpha I (lower tiles), on every host core */
+static void spd_run(int N, int NB, int nt, double alpha, double c, const double *B, double *tiles) {
+  int nth = host_cores();
+  pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nth);
+  spd_arg *args = (spd_arg *)malloc(sizeof(spd_arg) * nth);
+  for (int t = 0; t < nth; t++) {
+    args[t] = (spd_arg){N, NB, nt, nth, t, alpha, c, B, tiles};
+    pthread_create(&th[t], NULL, spd_worker, &args[t]);
+  }
+  for (int t = 0; t < nth; t++) pthread_join(th[t], NULL);
+  free(th);
+  free(args);
 }
 
 void oracle_spd_tiled(int N, int NB, double alpha, unsigned long long seed, double *tiles) {
@@ -210,16 +233,21 @@ void oracle_spd_tiled(int N, int NB, double alpha, unsigned long long seed, doub
       ran = ORACLE_RND64_A * ran + ORACLE_RND64_C;
     }
   }
-  int nth = host_cores();
-  pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nth);
-  spd_arg *args = (spd_arg *)malloc(sizeof(spd_arg) * nth);
-  for (int t = 0; t < nth; t++) {
-    args[t] = (spd_arg){N, NB, nt, nth, t, alpha, B, tiles};
-    pthread_create(&th[t], NULL, spd_worker, &args[t]);
+  spd_run(N, NB, nt, alpha, 1.0 / (127.0 * 127.0 * (double)N), B, tiles);
+  free(B);
+}
+
+void oracle_spd_uniform_tiled(int N, int NB, double alpha, unsigned long long seed, double *tiles) {
+  int nt = (N + NB - 1) / NB;
+  double *B = (double *)malloc(sizeof(double) * (size_t)N * N);
+  for (long j = 0; j < N; j++) {
+    unsigned long long ran = oracle_rnd64_jump((unsigned long long)(j * N), seed);
+    for (long i = 0; i < N; i++) {
+      B[i + j * N] = 2.0 * ((double)(ran >> 11) * 0x1p-53) - 1.0;
+      ran = ORACLE_RND64_A * ran + ORACLE_RND64_C;
+    }
   }
-  for (int t = 0; t < nth; t++) pthread_join(th[t], NULL);
-  free(th);
-  free(args);
+  spd_run(N, NB, nt, alpha, 1.0 / (double)N, B, tiles);
   free(B);
 }
 
@@ -702,12 +730,15 @@ static void *chk_worker(void *p) {
       const double *a0 = tile_cptr(c->A0, m, n, nt, NB);
       for (int j = 0; j < nb; j++)
         for (int i = 0; i < mb; i++) {
-          if (m == n && j > i) continue;
-          double r = fabs(R[i + j * NB] - a0[i + (long)j * NB]);
-          double a = (m == n && j > i) ? 0.0 : fabs(a0[i + (long)j * NB]);
           long gi = (long)m * NB + i, gj = (long)n * NB + j;
+          /* ||A0||_inf is lange(Inf) of getLAPACKMatrix(A0) (pmw.h:122-139,
+           * testing_dpotrf.cc:312): the lower tiles as stored, diagonal tiles
+           * whole (their upper part included), no mirror. */
+          c->rows_a[gi] += fabs(a0[i + (long)j * NB]);
+          if (m == n && j > i) continue;
+          /* the residual is mirrored (testing_dpotrf.cc:302-310) */
+          double r = fabs(R[i + j * NB] - a0[i + (long)j * NB]);
           c->rows_r[gi] += r;
-          c->rows_a[gi] += a;
           if (gi != gj) c->rows_r[gj] += r; /* symmetric counterpart */
         }
     }
@@ -738,8 +769,6 @@ double oracle_check_dpotrf_tiled(int N, int NB, const double *Ltiles, const doub
     free(args[t].rows_r);
     free(args[t].rows_a);
   }
-  /* ||A0||_inf uses A0 as stored by getLAPACKMatrix: lower part only
-   * (testing_dpotrf.cc:200,312). */
   for (int i = 0; i < N; i++) {
     if (rr[i] > Rn || isnan(rr[i])) Rn = rr[i];
     if (ra[i] > An) An = ra[i];

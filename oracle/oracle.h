@@ -56,6 +56,13 @@ void oracle_dense_to_tiles(int N, int NB, const double *A, double *tiles);
  * Pinned: integer arithmetic identity (tests compare to int64 sums). */
 int oracle_spd_bint(long i, long j, long N, unsigned long long seed);
 void oracle_spd_tiled(int N, int NB, double alpha, unsigned long long seed, double *tiles);
+/* The same construction with B FP64 uniform on [-1, 1) (the benchmark
+ * configurations' "B fp64 uniform[-1,1]"): B(i,j) = 2 (ran >> 11) 2^-53 - 1
+ * from LCG stream position i + j*N (exact), A = B B^T / N + alpha I summed in
+ * FP64 in k order (rounded; not bit-comparable with other summation orders).
+ * Pinned: B values against a Python LCG; A against numpy's B B^T / N. */
+double oracle_spd_uval(long i, long j, long N, unsigned long long seed);
+void oracle_spd_uniform_tiled(int N, int NB, double alpha, unsigned long long seed, double *tiles);
 
 /* Tile kernels, FP64, column-major (textbook definitions; pinned against
  * scipy LAPACK/BLAS fixtures and brute force in tests). */
@@ -92,9 +99,16 @@ void oracle_last_task_counts(long counts[4]);
 
 /* check_dpotrf of examples/potrf/testing_dpotrf.cc:277-335:
  * ||L*L^T - A0||_inf / (||A0||_inf * N * eps).  A holds L in its lower part
- * (dense, col-major, lda = N); A0 the original lower part. */
+ * (dense, col-major, lda = N; its upper part is not read); A0 is the
+ * original matrix as getLAPACKMatrix returns it (pmw.h:122-139: the lower
+ * tiles, diagonal tiles whole, zeros above).  The residual is mirrored from
+ * its lower part (:302-310); ||A0||_inf is lange(Inf) of A0 as given (:312).
+ * Pinned: planted symmetric perturbations of an exactly representable
+ * L L^T give ||E||_inf / (||A0||_inf N eps) to rounding (tests). */
 double oracle_check_dpotrf(const double *A, const double *A0, int N);
-/* Same quantity computed tile by tile (no dense N*N work arrays). */
+/* Same quantity computed tile by tile (no dense N*N work arrays), with
+ * ||A0||_inf over the lower tiles as stored (diagonal tiles whole), as
+ * getLAPACKMatrix would hold them.  Pinned: equal to the dense form. */
 double oracle_check_dpotrf_tiled(int N, int NB, const double *Ltiles, const double *A0tiles,
                                  int nthreads);
 
@@ -104,10 +118,14 @@ void oracle_dpotrs_tiled(int N, int NB, const double *Ltiles, int nrhs, double *
 void oracle_dresidual_tiled(int N, int NB, const double *Atiles, int nrhs, const double *X,
                             int ldx, const double *B, int ldb, double *R, int ldr);
 /* Normwise backward error of a solve, max over columns:
- * ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf). */
+ * ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf), with ||A||_inf of the
+ * full symmetric matrix.  Pinned: a planted error d e_j on an exact integer
+ * solve gives d max_i |A_ij| / (||A||_inf ||x||_inf + ||b||_inf) (tests). */
 double oracle_solve_backward_error(int N, int NB, const double *Atiles, int nrhs,
                                    const double *X, int ldx, const double *B, int ldb);
-/* ||A||_inf of the symmetric matrix held by its lower tiles. */
+/* ||A||_inf of the symmetric matrix held by its lower tiles (upper part of
+ * the diagonal tiles not read).  Pinned: numpy's norm of the mirrored dense
+ * matrix, exactly on integer data. */
 double oracle_norm_inf_sym_tiled(int N, int NB, const double *Atiles);
 
 /* POINV family (examples/potrf/trtri_L.h, lauum.h, potri.h). */

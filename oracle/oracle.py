@@ -37,6 +37,8 @@ def lib():
             "oracle_dense_to_tiles": (None, [_i, _i, _p, _p]),
             "oracle_spd_bint": (_i, [_l, _l, _l, _ull]),
             "oracle_spd_tiled": (None, [_i, _i, _d, _ull, _p]),
+            "oracle_spd_uval": (_d, [_l, _l, _l, _ull]),
+            "oracle_spd_uniform_tiled": (None, [_i, _i, _d, _ull, _p]),
             "oracle_dpotrf_tile": (_i, [_i, _p, _i]),
             "oracle_dtrsm_rlt": (None, [_i, _i, _p, _i, _p, _i]),
             "oracle_dsyrk_ln": (None, [_i, _i, _d, _p, _i, _d, _p, _i]),
@@ -122,6 +124,16 @@ def spd_tiles(N, NB, alpha, seed):
     t = alloc_tiles(N, NB)
     lib().oracle_spd_tiled(N, NB, float(alpha), seed, ptr(t))
     return t
+
+
+def spd_uniform_tiles(N, NB, alpha, seed):
+    t = alloc_tiles(N, NB)
+    lib().oracle_spd_uniform_tiled(N, NB, float(alpha), seed, ptr(t))
+    return t
+
+
+def norm_inf_sym_tiled(N, NB, tiles):
+    return lib().oracle_norm_inf_sym_tiled(N, NB, ptr(tiles))
 
 
 def tiles_to_dense(N, NB, tiles):

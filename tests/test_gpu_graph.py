@@ -78,11 +78,13 @@ def test_graph_plgsy_bit_exact(oracle, N, NB):
 
 
 @pytest.mark.parametrize("N,NB,gen", [(160, 32, "plgsy"), (700, 64, "plgsy"), (1000, 128, "spd"),
-                                      (2048, 256, "spd")])
+                                      (2048, 256, "spd"), (4096, 256, "uniform")])
 def test_graph_dpotrf_vs_oracle(oracle, N, NB, gen):
+    """(4096, 256, uniform) is cfg1: A = B B^T/N + I with B FP64 uniform."""
     import ttg_amd.graph as G
 
-    A0 = oracle.plgsy_tiles(N, NB, N, 3872) if gen == "plgsy" else oracle.spd_tiles(N, NB, 1.0, 5)
+    A0 = {"plgsy": lambda: oracle.plgsy_tiles(N, NB, N, 3872), "spd": lambda: oracle.spd_tiles(N, NB, 1.0, 5),
+          "uniform": lambda: oracle.spd_uniform_tiles(N, NB, 1.0, 2024)}[gen]()
     ref = A0.copy()
     assert oracle.potrf_tiled(N, NB, ref) == 0
     got = A0.copy()

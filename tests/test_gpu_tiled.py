@@ -44,17 +44,48 @@ def test_spd_bit_exact(dev, oracle, N, NB, alpha):
             mb, nb = min(NB, N - m * NB), min(NB, N - n * NB)
             a = oracle.tile_view(got, N, NB, m, n)[:mb, :nb]
             b = oracle.tile_view(want, N, NB, m, n)[:mb, :nb]
-            if m == n:
-                a, b = np.tril(a), np.tril(b)
+            # diagonal tiles are written whole (both triangles), exactly
             assert np.array_equal(a, b), (m, n)
 
 
-@pytest.mark.parametrize("N,NB,gen", [(160, 32, "plgsy"), (1000, 128, "spd"), (2048, 256, "spd"),
-                                      (4096, 512, "spd"), (1536, 512, "plgsy")])
-def test_dpotrf_tiled_vs_oracle(dev, oracle, N, NB, gen):
+@pytest.mark.parametrize("N,NB,alpha", [(256, 64, 1.0), (300, 128, 1e-3), (2048, 512, 1e-4)])
+def test_spd_uniform_vs_oracle(dev, oracle, N, NB, alpha):
+    """B FP64 uniform: B is bit-identical, so the product differs from the
+    oracle's only by summation order (a few ulps of the largest entry)."""
     import ttg_amd.chol as C
 
-    A0 = oracle.plgsy_tiles(N, NB, N, 3872) if gen == "plgsy" else oracle.spd_tiles(N, NB, 1.0, 5)
+    want = oracle.spd_uniform_tiles(N, NB, alpha, 2024)
+    t = torch.zeros(want.size, dtype=torch.float64, device=dev)
+    work = torch.empty(N * N, dtype=torch.float64, device=dev)
+    C.spd_uniform_tiled(t, N, NB, alpha, 2024, work)
+    got = to_host(t)
+    Bh = work.cpu().numpy().reshape(N, N).T  # column-major B
+    assert np.array_equal(Bh[:5, :3], np.array([[oracle.lib().oracle_spd_uval(i, j, N, 2024) for j in range(3)]
+                                                for i in range(5)]))
+    nt = oracle.nt_of(N, NB)
+    scale = 0.0
+    err = 0.0
+    for n in range(nt):
+        for m in range(n, nt):
+            mb, nb = min(NB, N - m * NB), min(NB, N - n * NB)
+            a = oracle.tile_view(got, N, NB, m, n)[:mb, :nb]
+            b = oracle.tile_view(want, N, NB, m, n)[:mb, :nb]
+            err = max(err, np.abs(a - b).max())
+            scale = max(scale, np.abs(b).max())
+    # N products of magnitude <= 1 summed in another order, over N: rounding
+    # of order sqrt(N) eps of the largest entry (a wrong operand, scale or
+    # shift is off by far more)
+    assert err <= 16 * np.finfo(np.float64).eps * scale * np.sqrt(N), (err, scale)
+
+
+@pytest.mark.parametrize("N,NB,gen", [(160, 32, "plgsy"), (1000, 128, "spd"), (2048, 256, "spd"),
+                                      (4096, 512, "spd"), (1536, 512, "plgsy"), (4096, 256, "uniform")])
+def test_dpotrf_tiled_vs_oracle(dev, oracle, N, NB, gen):
+    """(4096, 256, uniform) is cfg1: A = B B^T/N + I with B FP64 uniform."""
+    import ttg_amd.chol as C
+
+    A0 = {"plgsy": lambda: oracle.plgsy_tiles(N, NB, N, 3872), "spd": lambda: oracle.spd_tiles(N, NB, 1.0, 5),
+          "uniform": lambda: oracle.spd_uniform_tiles(N, NB, 1.0, 2024)}[gen]()
     ref = A0.copy()
     assert oracle.potrf_tiled(N, NB, ref) == 0
     t = to_dev(A0, dev)

@@ -182,6 +182,8 @@ def test_check_dpotrf_reference_case(oracle):
     r_dense = oracle.check_dpotrf_dense(oracle.tiles_to_dense(N, NB, L), oracle.tiles_to_dense(N, NB, A0))
     r_tiled = oracle.check_dpotrf_tiled(N, NB, L, A0)
     assert r_dense <= 60 and r_tiled <= 60
+    # the two forms sum the rounding noise of L L^T in different orders; their
+    # exact agreement is pinned on planted exact data (test_oracle_checks.py)
     assert r_tiled == pytest.approx(r_dense, rel=0.5, abs=0.5)
 
 

@@ -43,6 +43,9 @@ SIGNATURES = {
     "ttg_chol_plgsy_tiled": (_i, [_p, _i, _i, _d, _ull, _p]),
     "ttg_chol_dplgsy_tile": (_i, [_p, _d, _i, _i, _p, _i, _i, _i, _i, _ull]),
     "ttg_chol_spd_tiled": (_i, [_p, _i, _i, _d, _ull, _p, _p]),
+    "ttg_chol_spd_uniform_tiled": (_i, [_p, _i, _i, _d, _ull, _p, _p]),
+    "ttg_chol_dposv_mixed_factor": (_i, [_p, _i, _i, _i, _i, _p]),
+    "ttg_chol_dcheck_dpotrf_tiled": (_i, [_p, _i, _i, _p, _p, _p]),
     "ttg_chol_selftest_mfma_layout": (_i, [_p, _i, _p]),
     "ttg_chol_profile_read": (_i, [_i, _p, _p, _p]),
     "ttg_chol_profile_enable": (_i, [_i]),
@@ -96,9 +99,109 @@ def _stream(stream):
 
 
 def _require_cuda(*ts):
+    """Every tensor on the GPU, and on the one device the call runs on."""
+    import torch
+
+    devs = set()
     for t in ts:
-        if t is not None and not t.is_cuda:
+        if t is None:
+            continue
+        if not t.is_cuda:
             raise CholError("ttg_amd.chol works on device tensors only (no CPU path)")
+        devs.add(t.device.index)
+    if len(devs) > 1:
+        raise CholError(f"tensors on different devices {sorted(devs)}")
+    if devs and devs.pop() != torch.cuda.current_device():
+        raise CholError("tensors are not on the current device (the stream's device)")
+
+
+# ---- argument checks: the C ABI takes raw pointers and cannot see sizes, so
+# a short or mistyped tensor is rejected here, before any kernel could read
+# or write past its allocation.  They run before the device check, so they
+# hold for any tensor.
+
+
+def _dtype_name(t):
+    return str(t.dtype).replace("torch.", "")
+
+
+def _need_dtype(t, want, name):
+    import torch
+
+    want_t = {"f64": torch.float64, "f32": torch.float32, "i32": torch.int32}[want]
+    if t.dtype != want_t:
+        raise CholError(f"{name} must be {str(want_t).replace('torch.', '')}, got {_dtype_name(t)}")
+
+
+def _span(t):
+    """Elements from the first to one past the last element a strided view reaches."""
+    if t.numel() == 0:
+        return 0
+    return 1 + sum((d - 1) * st for d, st in zip(t.shape, t.stride()))
+
+
+def _need_storage(t, elems, name):
+    """t's storage holds `elems` elements from t's first element."""
+    avail = t.untyped_storage().nbytes() // t.element_size() - t.storage_offset()
+    if avail < elems:
+        raise CholError(f"{name} holds {avail} elements from its first one, needs {elems}")
+
+
+def _check_tiles(t, N, NB, name="tiles", dtype="f64"):
+    if not isinstance(N, int) or not isinstance(NB, int) or N < 0 or NB <= 0:
+        raise CholError(f"bad sizes N={N!r}, NB={NB!r}")
+    _need_dtype(t, dtype, name)
+    if not t.is_contiguous():
+        raise CholError(f"{name} must be contiguous (lower tile-packed layout)")
+    need = num_tiles(N, NB) * NB * NB
+    if t.numel() < need:
+        raise CholError(f"{name} has {t.numel()} elements, the lower tile-packed layout of N={N}, NB={NB} "
+                        f"needs {need}")
+
+
+def _check_cols(X, N, name, nrhs=None, dtype="f64"):
+    """A column-major N x nrhs matrix (or a vector of N); returns nrhs."""
+    _need_dtype(X, dtype, name)
+    if X.dim() == 1:
+        cols = 1
+        if X.shape[0] != N or (N > 1 and X.stride(0) != 1):
+            raise CholError(f"{name} must be a contiguous vector of {N} elements, got shape {tuple(X.shape)}")
+    elif X.dim() == 2:
+        cols = X.shape[1]
+        if X.shape[0] != N:
+            raise CholError(f"{name} must have N={N} rows, got shape {tuple(X.shape)}")
+        if N > 1 and X.stride(0) != 1:
+            raise CholError(f"{name} must be column-major (stride(0) == 1)")
+        if cols > 1 and X.stride(1) < max(1, N):
+            raise CholError(f"{name}: leading dimension {X.stride(1)} < N={N}")
+    else:
+        raise CholError(f"{name} must be 1-D or 2-D, got {X.dim()}-D")
+    if nrhs is not None and cols != nrhs:
+        raise CholError(f"{name} has {cols} columns, expected {nrhs}")
+    _need_storage(X, _span(X), name)
+    return cols
+
+
+def _check_tile(A, name, rows=None, cols=None):
+    """A column-major 2-D float64 tile."""
+    _need_dtype(A, "f64", name)
+    if A.dim() != 2:
+        raise CholError(f"{name} must be 2-D, got {A.dim()}-D")
+    if rows is not None and A.shape[0] != rows or cols is not None and A.shape[1] != cols:
+        raise CholError(f"{name} has shape {tuple(A.shape)}, expected ({rows}, {cols})")
+    if A.shape[0] > 1 and A.stride(0) != 1:
+        raise CholError(f"{name} must be column-major (stride(0) == 1)")
+    if A.shape[1] > 1 and A.stride(1) < max(1, A.shape[0]):
+        raise CholError(f"{name}: leading dimension {A.stride(1)} < rows {A.shape[0]}")
+    _need_storage(A, _span(A), name)
+
+
+def _check_info(info, count=1, name="info"):
+    if info is None:
+        return
+    _need_dtype(info, "i32", name)
+    if info.numel() < count:
+        raise CholError(f"{name} must hold at least {count} int32 entries, got {info.numel()}")
 
 
 def _ld(t):
@@ -121,34 +224,48 @@ def colmajor(t):
 
 
 def dpotrf_tile(A, info=None, info_offset=0, stream=None):
+    n = A.shape[0] if A.dim() == 2 else -1
+    _check_tile(A, "A", n, n)
+    _check_info(info)
     _require_cuda(A, info)
-    n = A.shape[0]
     _check(lib().ttg_chol_dpotrf_tile(_stream(stream), n, _ptr(A), _ld(A), _ptr(info), info_offset),
            "dpotrf_tile")
 
 
 def dtrsm_rlt(L, B, stream=None):
-    _require_cuda(L, B)
+    _check_tile(B, "B")
     m, n = B.shape
+    _check_tile(L, "L", n, n)
+    _require_cuda(L, B)
     _check(lib().ttg_chol_dtrsm_rlt(_stream(stream), m, n, _ptr(L), _ld(L), _ptr(B), _ld(B)), "dtrsm_rlt")
 
 
 def dsyrk_ln(A, C, alpha=-1.0, beta=1.0, stream=None):
-    _require_cuda(A, C)
+    _check_tile(A, "A")
     n, k = A.shape
+    _check_tile(C, "C", n, n)
+    _require_cuda(A, C)
     _check(lib().ttg_chol_dsyrk_ln(_stream(stream), n, k, alpha, _ptr(A), _ld(A), beta, _ptr(C), _ld(C)),
            "dsyrk_ln")
 
 
 def dgemm_nt(A, B, C, alpha=-1.0, beta=1.0, stream=None):
-    _require_cuda(A, B, C)
+    _check_tile(A, "A")
     m, k = A.shape
+    _check_tile(B, "B", None, k)
     n = B.shape[0]
+    _check_tile(C, "C", m, n)
+    _require_cuda(A, B, C)
     _check(lib().ttg_chol_dgemm_nt(_stream(stream), m, n, k, alpha, _ptr(A), _ld(A), _ptr(B), _ld(B), beta,
                                    _ptr(C), _ld(C)), "dgemm_nt")
 
 
 def dnrm2(x, out, stream=None):
+    _need_dtype(x, "f64", "x")
+    _need_dtype(out, "f64", "out")
+    dense = x.is_contiguous() or (x.dim() == 2 and x.t().is_contiguous())  # row- or column-major, no gaps
+    if not dense or out.numel() < 1:
+        raise CholError("dnrm2: x must be dense (contiguous row- or column-major) and out hold one double")
     _require_cuda(x, out)
     _check(lib().ttg_chol_dnrm2(_stream(stream), x.numel(), _ptr(x), _ptr(out)), "dnrm2")
 
@@ -196,21 +313,78 @@ def alloc_tiles(N, NB, dtype=None, device="cuda"):
 
 
 def dpotrf_tiled(tiles, N, NB, info=None, flags=F_DEFAULT, stream=None):
+    _check_tiles(tiles, N, NB)
+    _check_info(info)
     _require_cuda(tiles, info)
     _check(lib().ttg_chol_dpotrf_tiled(_stream(stream), N, NB, _ptr(tiles), _ptr(info), flags), "dpotrf_tiled")
 
 
 def dpotrs_tiled(tiles, N, NB, B, stream=None):
+    _check_tiles(tiles, N, NB)
+    nrhs = _check_cols(B, N, "B")
     _require_cuda(tiles, B)
-    nrhs = 1 if B.dim() == 1 else B.shape[1]
     _check(lib().ttg_chol_dpotrs_tiled(_stream(stream), N, NB, _ptr(tiles), nrhs, _ptr(B), _ld(B)), "dpotrs_tiled")
 
 
 def dresidual_tiled(tiles, N, NB, X, B, R, stream=None):
+    _check_tiles(tiles, N, NB)
+    nrhs = _check_cols(B, N, "B")
+    _check_cols(X, N, "X", nrhs)
+    _check_cols(R, N, "R", nrhs)
     _require_cuda(tiles, X, B, R)
-    nrhs = 1 if B.dim() == 1 else B.shape[1]
     _check(lib().ttg_chol_dresidual_tiled(_stream(stream), N, NB, _ptr(tiles), nrhs, _ptr(X), _ld(X), _ptr(B),
                                           _ld(B), _ptr(R), _ld(R)), "dresidual_tiled")
+
+
+def backward_error(tiles, N, NB, X, B, stream=None):
+    """max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf) of a
+    solve, A symmetric from its lower FP64 tiles, on the device: the residual
+    by ttg_chol_dresidual_tiled, ||A||_inf as the largest row sum of |A| (the
+    same residual kernel applied to |A| and a vector of ones)."""
+    import torch
+
+    nrhs = _check_cols(B, N, "B")
+    _check_cols(X, N, "X", nrhs)
+    _check_tiles(tiles, N, NB)
+    _require_cuda(tiles, X, B)
+    dev = tiles.device
+    ones = torch.ones(N, 1, dtype=torch.float64, device=dev)
+    zeros = torch.zeros(N, 1, dtype=torch.float64, device=dev)
+    rs = torch.empty(N, 1, dtype=torch.float64, device=dev)
+    dresidual_tiled(tiles.abs(), N, NB, ones, zeros, rs, stream)  # rs = -|A| 1
+    anorm = (-rs).max()
+    X2 = X.reshape(N, 1) if X.dim() == 1 else X
+    B2 = B.reshape(N, 1) if B.dim() == 1 else B
+    R = torch.empty(nrhs, N, dtype=torch.float64, device=dev).t()
+    dresidual_tiled(tiles, N, NB, X2, B2, R, stream)
+    rn = R.abs().amax(dim=0)
+    e = rn / (anorm * X2.abs().amax(dim=0) + B2.abs().amax(dim=0))
+    if torch.isnan(rn).any():
+        return float("nan")
+    return float(e.max().item())
+
+
+def dcheck_dpotrf_tiled(A0, L, N, NB, stream=None):
+    """The factorization check of testing_dpotrf.cc:277-335 on the device:
+    (||A0||_inf, ||L L^T - A0||_inf, ||L L^T - A0||_inf / (||A0||_inf N eps))."""
+    import torch
+
+    _check_tiles(A0, N, NB, "A0")
+    _check_tiles(L, N, NB, "L")
+    _require_cuda(A0, L)
+    out = torch.empty(3, dtype=torch.float64, device=A0.device)
+    _check(lib().ttg_chol_dcheck_dpotrf_tiled(_stream(stream), N, NB, _ptr(A0), _ptr(L), _ptr(out)),
+           "dcheck_dpotrf_tiled")
+    a, r, q = (float(v) for v in out.cpu())
+    return a, r, q
+
+
+def dposv_mixed_factor(L, N, NB, nrhs, prec, stream=None):
+    """The working factor of the last dposv_mixed call with these arguments,
+    widened to FP64, into the tiles L."""
+    _check_tiles(L, N, NB, "L")
+    _require_cuda(L)
+    _check(lib().ttg_chol_dposv_mixed_factor(_stream(stream), N, NB, nrhs, prec, _ptr(L)), "dposv_mixed_factor")
 
 
 def release_plans():
@@ -223,6 +397,8 @@ def dtrtri_tiled(tiles, N, NB, info=None, stream=None):
     """tiles := L^{-1} (lower tiles, in place); the TRTRI graph of
     examples/potrf/trtri_L.h.  info (device int32 tensor) receives 0 or the
     1-based index of the first zero diagonal entry."""
+    _check_tiles(tiles, N, NB)
+    _check_info(info)
     _require_cuda(tiles, info)
     _check(lib().ttg_chol_dtrtri_tiled(_stream(stream), N, NB, _ptr(tiles), _ptr(info)), "dtrtri_tiled")
 
@@ -230,6 +406,7 @@ def dtrtri_tiled(tiles, N, NB, info=None, stream=None):
 def dlauum_tiled(tiles, N, NB, stream=None):
     """tiles := L^T L (lower part, in place); the LAUUM graph of
     examples/potrf/lauum.h."""
+    _check_tiles(tiles, N, NB)
     _require_cuda(tiles)
     _check(lib().ttg_chol_dlauum_tiled(_stream(stream), N, NB, _ptr(tiles)), "dlauum_tiled")
 
@@ -237,11 +414,17 @@ def dlauum_tiled(tiles, N, NB, stream=None):
 def dpotri_tiled(tiles, N, NB, info=None, stream=None):
     """tiles := A^{-1} (lower part, in place) from the Cholesky factor of A;
     the POTRI graph of examples/potrf/potri.h (TRTRI then LAUUM)."""
+    _check_tiles(tiles, N, NB)
+    _check_info(info)
     _require_cuda(tiles, info)
     _check(lib().ttg_chol_dpotri_tiled(_stream(stream), N, NB, _ptr(tiles), _ptr(info)), "dpotri_tiled")
 
 
 def selftest_mfma_layout(kind, out, stream=None):
+    need = {0: 64 * 64 * 4, 1: 64 * 64 * 4, 2: 512 * 64 * 4, 3: 512 * 64 * 16}.get(kind)
+    _need_dtype(out, "f64", "out")
+    if need is None or out.numel() < need or not out.is_contiguous():
+        raise CholError(f"selftest_mfma_layout: kind {kind} needs a contiguous float64 out of {need} elements")
     _require_cuda(out)
     _check(lib().ttg_chol_selftest_mfma_layout(_stream(stream), kind, _ptr(out)), "selftest_mfma_layout")
 
@@ -259,10 +442,13 @@ def dposv_mixed(A, N, NB, B, X, prec=PREC_FP32, tol=1e-12, max_iter=30, info=Non
     """Mixed-precision solve with FP64 refinement; returns (iterations, backward error)."""
     import torch
 
-    _require_cuda(A, B, X)
+    _check_tiles(A, N, NB, "A")
+    nrhs = _check_cols(B, N, "B")
+    _check_cols(X, N, "X", nrhs)
+    _check_info(info)
+    _require_cuda(A, B, X, info)
     if info is None:
         info = torch.zeros(1, dtype=torch.int32, device=A.device)
-    nrhs = 1 if B.dim() == 1 else B.shape[1]
     it = ctypes.c_int()
     be = ctypes.c_double()
     rc = lib().ttg_chol_dposv_mixed(_stream(stream), N, NB, _ptr(A), nrhs, _ptr(B), _ld(B), _ptr(X), _ld(X), prec,
@@ -279,6 +465,7 @@ def profile_enable(on):
 
 def dplgsy_tile(A, bump, gM, m0, n0, seed, stream=None):
     """One CORE_dplgsy tile (column-major device tensor A) at global offset (m0, n0)."""
+    _check_tile(A, "A")
     _require_cuda(A)
     m, n = A.shape
     _check(lib().ttg_chol_dplgsy_tile(_stream(stream), float(bump), m, n, _ptr(A), _ld(A), gM, m0, n0, seed),
@@ -286,14 +473,32 @@ def dplgsy_tile(A, bump, gM, m0, n0, seed, stream=None):
 
 
 def plgsy_tiled(tiles, N, NB, bump, seed, stream=None):
+    _check_tiles(tiles, N, NB)
     _require_cuda(tiles)
     _check(lib().ttg_chol_plgsy_tiled(_stream(stream), N, NB, float(bump), seed, _ptr(tiles)), "plgsy_tiled")
 
 
+def _check_work(work, N):
+    _need_dtype(work, "f64", "work")
+    if not work.is_contiguous() or work.numel() < N * N:
+        raise CholError(f"work must be a contiguous float64 tensor of at least N*N = {N * N} elements")
+
+
 def spd_tiled(tiles, N, NB, alpha, seed, work, stream=None):
+    _check_tiles(tiles, N, NB)
+    _check_work(work, N)
     _require_cuda(tiles, work)
     _check(lib().ttg_chol_spd_tiled(_stream(stream), N, NB, float(alpha), seed, _ptr(tiles), _ptr(work)),
            "spd_tiled")
+
+
+def spd_uniform_tiled(tiles, N, NB, alpha, seed, work, stream=None):
+    """A = B B^T / N + alpha I with B FP64 uniform on [-1, 1) (seeded LCG)."""
+    _check_tiles(tiles, N, NB)
+    _check_work(work, N)
+    _require_cuda(tiles, work)
+    _check(lib().ttg_chol_spd_uniform_tiled(_stream(stream), N, NB, float(alpha), seed, _ptr(tiles), _ptr(work)),
+           "spd_uniform_tiled")
 
 
 class Batch:
@@ -303,6 +508,13 @@ class Batch:
     def __init__(self, mats, rhs):
         if len(mats) != len(rhs):
             raise ValueError("one right-hand side per matrix")
+        for i, (m, v) in enumerate(zip(mats, rhs)):
+            _need_dtype(m, "f32", f"matrix {i}")
+            _need_dtype(v, "f32", f"right-hand side {i}")
+            if m.dim() != 2 or m.shape[0] != m.shape[1] or not m.is_contiguous() or m.numel() == 0:
+                raise CholError(f"matrix {i} must be a non-empty contiguous square tensor, got {tuple(m.shape)}")
+            if v.numel() != m.shape[0] or not v.is_contiguous():
+                raise CholError(f"right-hand side {i} must be {m.shape[0]} contiguous floats")
         self.count = len(mats)
         n = [int(m.shape[0]) for m in mats]
         self.n = (ctypes.c_int * self.count)(*n)
@@ -314,6 +526,12 @@ class Batch:
     def packed(cls, sizes, A, b):
         """Matrices packed back to back in one float32 tensor A (each n_i^2,
         column-major, lda = n_i) and right-hand sides in b (each n_i)."""
+        _need_dtype(A, "f32", "A")
+        _need_dtype(b, "f32", "b")
+        if not A.is_contiguous() or not b.is_contiguous():
+            raise CholError("packed batch buffers must be contiguous")
+        if any(int(s) < 1 for s in sizes):
+            raise CholError("every matrix of a batch needs n >= 1")
         self = cls.__new__(cls)
         self.count = len(sizes)
         self.n = (ctypes.c_int * self.count)(*sizes)
@@ -326,7 +544,7 @@ class Batch:
             offa += s * s
             offb += s
         if offa > A.numel() or offb > b.numel():
-            raise ValueError("packed batch larger than its buffers")
+            raise CholError("packed batch larger than its buffers")
         self.A = (ctypes.c_void_p * self.count)(*pa)
         self.b = (ctypes.c_void_p * self.count)(*pb)
         return self
@@ -335,6 +553,9 @@ class Batch:
 def sposv_batched(batch, info, cfg=-1, stream=None):
     """Factor + solve every system of `batch` in one launch (info: int32 device
     tensor of batch.count entries)."""
+    _check_info(info, batch.count)
+    if info is None:
+        raise CholError("sposv_batched needs an info tensor of batch.count int32 entries")
     _require_cuda(info)
     _check(lib().ttg_chol_sposv_batched(_stream(stream), batch.count, batch.n, batch.A, batch.lda, batch.b,
                                         _ptr(info), cfg), "sposv_batched")

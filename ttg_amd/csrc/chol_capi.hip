@@ -151,6 +151,7 @@ int ttg_chol_release_plans(void) {
   release_poinv_plans();
   release_solve_plans();
   release_tiled_plans();
+  release_check_work();
   return TTG_CHOL_OK;
 }
 

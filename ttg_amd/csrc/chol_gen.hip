@@ -1,7 +1,7 @@
 // Matrix generators on the device (tile-packed lower layout):
 //   PLGSY (examples/potrf/core_plgsy.h:63-150), bit-identical to the
 //   reference, and the SPD construction A = B B^T / N + alpha I with integer
-//   B, formed exactly on FP64 MFMA.  Also the small utility kernels of the
+//   B (formed exactly on FP64 MFMA) or FP64 uniform B (rounded).  Also the small utility kernels of the
 //   tiled drivers (edge-tile padding, info reset, diagonal shift, nrm2).
 #include <hip/hip_runtime.h>
 
@@ -117,7 +117,14 @@ __global__ __launch_bounds__(256) void plgsy_batched_kernel(const PlgsyOp *__res
   }
 }
 
-// Integer-valued B (N x N, column-major, stored as doubles).
+// B for the SPD construction (N x N, column-major, doubles).  UNIFORM: FP64
+// uniform on [-1, 1), 2 (ran >> 11) 2^-53 - 1 (every step exact); otherwise
+// the integer k/127 construction's k.
+__device__ __forceinline__ double spd_uval(unsigned long long ran) {
+  return 2.0 * ((double)(ran >> 11) * 0x1p-53) - 1.0;
+}
+
+template <bool UNIFORM>
 __global__ __launch_bounds__(256) void spd_b_kernel(long N, unsigned long long seed, double *B) {
   const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
   const int lane = threadIdx.x & 63;
@@ -136,7 +143,7 @@ __global__ __launch_bounds__(256) void spd_b_kernel(long N, unsigned long long s
     c64 = c;
   }
   for (long i = lane; i < N; i += 64) {
-    B[i + j * N] = spd_bint(ran);
+    B[i + j * N] = UNIFORM ? spd_uval(ran) : spd_bint(ran);
     ran = a64 * ran + c64;
   }
 }
@@ -267,6 +274,47 @@ hipError_t launch_nrm2(hipStream_t s, int n, const double *x, double *r) {
   return hipGetLastError();
 }
 
+// A = c B B^T + alpha I into the lower tiles (diagonal tiles whole, both
+// triangles), c = 1/(127^2 N) for the integer B (every product and sum exact,
+// so any order gives the same bits) or 1/N for the uniform B.
+static int spd_generate(hipStream_t s, int N, int NB, double alpha, unsigned long long seed, double *tiles,
+                        double *work, bool uniform) {
+  if (N == 0) return TTG_CHOL_OK;
+  const int nt = (N + NB - 1) / NB;
+  if (uniform)
+    hipLaunchKernelGGL(spd_b_kernel<true>, dim3((unsigned)(((long)N + 3) / 4)), dim3(256), 0, s, (long)N, seed, work);
+  else
+    hipLaunchKernelGGL(spd_b_kernel<false>, dim3((unsigned)(((long)N + 3) / 4)), dim3(256), 0, s, (long)N, seed,
+                       work);
+  TTGC_HIP(hipGetLastError());
+  std::vector<UpdOp<double>> fast, slow;
+  for (int n = 0; n < nt; ++n)
+    for (int m = n; m < nt; ++m) {
+      const int mb = std::min(NB, N - m * NB), nb = std::min(NB, N - n * NB);
+      UpdOp<double> op{work + (long)m * NB, work + (long)n * NB, tiles + tile_index(m, n, nt) * (long)NB * NB,
+                       mb, nb, N, N, N, NB, 0};
+      const bool f = update_fast(op, NB);
+      (f ? fast : slow).push_back(op);
+    }
+  const double c = uniform ? 1.0 / (double)N : 1.0 / (127.0 * 127.0 * (double)N);
+  DevBuf ops;
+  const size_t bytes = sizeof(UpdOp<double>) * (fast.size() + slow.size());
+  TTGC_HIP(ops.reserve(bytes));
+  auto *d = static_cast<UpdOp<double> *>(ops.ptr);
+  if (!fast.empty()) TTGC_HIP(hipMemcpy(d, fast.data(), sizeof(UpdOp<double>) * fast.size(), hipMemcpyHostToDevice));
+  if (!slow.empty())
+    TTGC_HIP(hipMemcpy(d + fast.size(), slow.data(), sizeof(UpdOp<double>) * slow.size(), hipMemcpyHostToDevice));
+  TTGC_HIP(launch_gen_gemm(s, d, (int)fast.size(), NB, NB, c, 0.0, true));
+  TTGC_HIP(launch_gen_gemm(s, d + fast.size(), (int)slow.size(), NB, NB, c, 0.0, false));
+  hipLaunchKernelGGL(add_diag_kernel, dim3((unsigned)(((long)N + 255) / 256)), dim3(256), 0, s, N, NB, nt, alpha,
+                     tiles);
+  TTGC_HIP(hipGetLastError());
+  // the descriptor buffer is freed when this call returns: wait for its users
+  TTGC_HIP(hipStreamSynchronize(s));
+  (void)hipFree(ops.ptr);
+  return TTG_CHOL_OK;
+}
+
 }  // namespace ttgc
 
 using namespace ttgc;
@@ -312,38 +360,17 @@ int ttg_chol_spd_tiled(void *stream, int N, int NB, double alpha, unsigned long 
     set_error("spd_tiled: bad argument (N=%d NB=%d)", N, NB);
     return TTG_CHOL_EINVAL;
   }
-  if (N == 0) return TTG_CHOL_OK;
-  hipStream_t s = as_stream(stream);
-  const int nt = (N + NB - 1) / NB;
-  hipLaunchKernelGGL(spd_b_kernel, dim3((unsigned)(((long)N + 3) / 4)), dim3(256), 0, s, (long)N, seed, work);
-  TTGC_HIP(hipGetLastError());
-  // lower tiles of B B^T, exact; scaled once by 1/(127^2 N) in the epilogue
-  std::vector<UpdOp<double>> fast, slow;
-  for (int n = 0; n < nt; ++n)
-    for (int m = n; m < nt; ++m) {
-      const int mb = std::min(NB, N - m * NB), nb = std::min(NB, N - n * NB);
-      UpdOp<double> op{work + (long)m * NB, work + (long)n * NB, tiles + tile_index(m, n, nt) * (long)NB * NB,
-                       mb, nb, N, N, N, NB, m == n ? 1 : 0};
-      const bool f = update_fast(op, NB);
-      (f ? fast : slow).push_back(op);
-    }
-  const double c = 1.0 / (127.0 * 127.0 * (double)N);
-  DevBuf ops;
-  const size_t bytes = sizeof(UpdOp<double>) * (fast.size() + slow.size());
-  TTGC_HIP(ops.reserve(bytes));
-  auto *d = static_cast<UpdOp<double> *>(ops.ptr);
-  if (!fast.empty()) TTGC_HIP(hipMemcpy(d, fast.data(), sizeof(UpdOp<double>) * fast.size(), hipMemcpyHostToDevice));
-  if (!slow.empty())
-    TTGC_HIP(hipMemcpy(d + fast.size(), slow.data(), sizeof(UpdOp<double>) * slow.size(), hipMemcpyHostToDevice));
-  TTGC_HIP(launch_gen_gemm(s, d, (int)fast.size(), NB, NB, c, 0.0, true));
-  TTGC_HIP(launch_gen_gemm(s, d + fast.size(), (int)slow.size(), NB, NB, c, 0.0, false));
-  hipLaunchKernelGGL(add_diag_kernel, dim3((unsigned)(((long)N + 255) / 256)), dim3(256), 0, s, N, NB, nt, alpha,
-                     tiles);
-  TTGC_HIP(hipGetLastError());
-  // the descriptor buffer is freed when this call returns: wait for its users
-  TTGC_HIP(hipStreamSynchronize(s));
-  (void)hipFree(ops.ptr);
-  return TTG_CHOL_OK;
+  return spd_generate(as_stream(stream), N, NB, alpha, seed, tiles, work, false);
+}
+
+int ttg_chol_spd_uniform_tiled(void *stream, int N, int NB, double alpha, unsigned long long seed, double *tiles,
+                               double *work) {
+  clear_error();
+  if (N < 0 || NB <= 0 || (N > 0 && (!tiles || !work))) {
+    set_error("spd_uniform_tiled: bad argument (N=%d NB=%d)", N, NB);
+    return TTG_CHOL_EINVAL;
+  }
+  return spd_generate(as_stream(stream), N, NB, alpha, seed, tiles, work, true);
 }
 
 int ttg_chol_selftest_mfma_layout(void *stream, int kind, double *out) {

@@ -84,6 +84,11 @@ void release_tiled_plans();
 void release_solve_plans();
 void release_mixed_plans();
 void release_poinv_plans();
+void release_check_work();
+
+// dst[i] := (double)src[i] for i < n (FP32 or FP64 tiles to FP64).
+template <typename T>
+hipError_t launch_widen(hipStream_t s, long n, const T *src, double *dst);
 
 // *r := ||x||_2 of n contiguous doubles (one workgroup).
 hipError_t launch_nrm2(hipStream_t s, int n, const double *x, double *r);

@@ -222,6 +222,15 @@ struct MixedPlan : MixedPlanBase {
     return TTG_CHOL_OK;
   }
 
+  // The working factor as FP64 lower tiles (the tile layout, padding as
+  // the factorization prepared it).
+  int factor(hipStream_t s, double *L) const {
+    const int nt = (N + NB - 1) / NB;
+    const long n = (long)nt * (nt + 1) / 2 * NB * NB;
+    TTGC_HIP(launch_widen<T>(s, n, static_cast<const T *>(W.ptr), L));
+    return TTG_CHOL_OK;
+  }
+
   int run(hipStream_t s, const double *A, const double *B, long ldb, double *X, long ldx, double tol, int max_iter,
           int *d_info, int *iters, double *berr) {
     const int nt = (N + NB - 1) / NB;
@@ -342,4 +351,22 @@ extern "C" int ttg_chol_dposv_mixed(void *stream, int N, int NB, const double *A
     }
   }
   return static_cast<MixedPlan<float> *>(p.get())->run(s, A, B, ldb, X, ldx, tol, max_iter, d_info, iters, berr);
+}
+
+extern "C" int ttg_chol_dposv_mixed_factor(void *stream, int N, int NB, int nrhs, int prec, double *L) {
+  clear_error();
+  if (!L) {
+    set_error("dposv_mixed_factor: L is NULL");
+    return TTG_CHOL_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(g_mixed_mu);
+  auto it = g_mixed.find(std::make_tuple(N, NB, nrhs, prec));
+  if (it == g_mixed.end() || !it->second) {
+    set_error("dposv_mixed_factor: no dposv_mixed call with N=%d NB=%d nrhs=%d prec=%d since the plans were built",
+              N, NB, nrhs, prec);
+    return TTG_CHOL_ESTATE;
+  }
+  if (prec == TTG_CHOL_PREC_FP64)
+    return static_cast<MixedPlan<double> *>(it->second.get())->factor(as_stream(stream), L);
+  return static_cast<MixedPlan<float> *>(it->second.get())->factor(as_stream(stream), L);
 }

@@ -58,15 +58,13 @@ class TiledMatrix {
   int Q() const { return 1; }
   T *data() const { return data_; }
 
-  // dense column-major copy with a zero upper part (pmw.h:122-139)
+  // dense column-major copy of the lower tiles as stored, diagonal tiles
+  // whole and zeros above them (pmw.h:122-139)
   std::vector<T> getLAPACKMatrix() const {
     std::vector<T> A(N_ * N_, T(0));
     for (std::size_t j = 0; j < N_; ++j)
-      for (std::size_t i = j; i < N_; ++i) {
-        const std::size_t m = i / NB_, n = j / NB_;
-        if (n > m) continue;
-        A[i + j * N_] = tile_ptr(m, n)[(i % NB_) + (j % NB_) * NB_];
-      }
+      for (std::size_t i = (j / NB_) * NB_; i < N_; ++i)
+        A[i + j * N_] = tile_ptr(i / NB_, j / NB_)[(i % NB_) + (j % NB_) * NB_];
     return A;
   }
 

@@ -35,7 +35,9 @@ static bool has_opt(int argc, char **argv, const char *name) {
 }
 
 // ||L L^T - A0||_inf / (||A0||_inf N eps): L in the lower part of A (dense,
-// column-major), A0 the original lower part (testing_dpotrf.cc:277-335).
+// column-major; its upper part is not read), A0 the original matrix as
+// getLAPACKMatrix returns it; the residual is mirrored from its lower part
+// and ||A0||_inf is taken of A0 as given (testing_dpotrf.cc:277-335).
 static int check_dpotrf(const std::vector<double> &A, const std::vector<double> &A0, int N) {
   std::vector<double> R((size_t)N * N);
   for (int j = 0; j < N; ++j)

@@ -34,9 +34,17 @@ def lib():
     return _lib
 
 
-def _tiles(t):
+def _tiles(t, N, NB):
+    """The host tile array of the lower tile-packed layout of (N, NB): the
+    library writes every tile, so a short array is refused here."""
     if not (isinstance(t, np.ndarray) and t.dtype == np.float64 and t.flags.c_contiguous):
         raise TypeError("tiles must be a contiguous float64 numpy array")
+    if not isinstance(N, int) or not isinstance(NB, int) or N < 0 or NB <= 0:
+        raise ValueError(f"bad sizes N={N!r}, NB={NB!r}")
+    nt = (N + NB - 1) // NB
+    need = nt * (nt + 1) // 2 * NB * NB
+    if t.size < need:
+        raise ValueError(f"tiles has {t.size} elements, the lower tile-packed layout of N={N}, NB={NB} needs {need}")
     return t.ctypes.data
 
 
@@ -44,13 +52,13 @@ def dpotrf(N, NB, tiles):
     """Factor host tiles in place with the task graph; returns the runtime's
     statistics (tasks, rounds, kernel launches, largest round)."""
     stats = (ctypes.c_long * 4)()
-    rc = lib().ttg_graph_dpotrf(N, NB, _tiles(tiles), ctypes.addressof(stats))
+    rc = lib().ttg_graph_dpotrf(N, NB, _tiles(tiles, N, NB), ctypes.addressof(stats))
     if rc != 0:
         raise ValueError(f"ttg_graph_dpotrf(N={N}, NB={NB}) rejected its arguments")
     return {"tasks": stats[0], "rounds": stats[1], "launches": stats[2], "max_round": stats[3]}
 
 
 def dplgsy(N, NB, bump, seed, tiles):
-    rc = lib().ttg_graph_dplgsy(N, NB, float(bump), seed, _tiles(tiles))
+    rc = lib().ttg_graph_dplgsy(N, NB, float(bump), seed, _tiles(tiles, N, NB))
     if rc != 0:
         raise ValueError(f"ttg_graph_dplgsy(N={N}, NB={NB}) rejected its arguments")
