@@ -86,6 +86,20 @@ int ttg_chol_dpotrf_tile(void *stream, int n, double *A, int lda, int *d_info, i
  * (examples/potrf/potrf.h:299-304). */
 int ttg_chol_dtrsm_rlt(void *stream, int m, int n, const double *L, int ldl, double *B, int ldb);
 
+/* ttg_chol_dtrsm_rlt without any allocation: the same kernels and bitwise
+ * the same result, with the inverses of L's 32 x 32 diagonal blocks staged in
+ * a fixed 128 KiB device array of the library (one per GPU, held until the
+ * process ends) instead of a per-stream workspace that the first call on a
+ * stream allocates.  n <= TTG_CHOL_MAX_TILE (larger n: TTG_CHOL_EINVAL); no
+ * hipMalloc, hipFree or host allocation on any call; refused
+ * (TTG_CHOL_ESTATE) while a launch group is open on the stream; the stream
+ * must belong to the current device.  All calls on one GPU share its array:
+ * calls on one stream are ordered, calls that overlap on different streams of
+ * one GPU race.  No counterpart in the reference: for callers that cannot
+ * allocate (code under HIP graph capture, serving loops, device-task bodies
+ * run where allocation is forbidden). */
+int ttg_chol_dtrsm_rlt_noalloc(void *stream, int m, int n, const double *L, int ldl, double *B, int ldb);
+
 /* C := alpha A A^T + beta C on the lower triangle of the n x n tile C, A n x k.
  * Replaces hipblasDsyrk(h, LOWER, N, n, k, &alpha, A, lda, &beta, C, ldc)
  * (examples/potrf/potrf.h:429-434; the graph uses alpha = -1, beta = 1). */

@@ -27,6 +27,7 @@ SIGNATURES = {
     "ttg_chol_num_devices": (_i, []),
     "ttg_chol_dpotrf_tile": (_i, [_p, _i, _p, _i, _p, _i]),
     "ttg_chol_dtrsm_rlt": (_i, [_p, _i, _i, _p, _i, _p, _i]),
+    "ttg_chol_dtrsm_rlt_noalloc": (_i, [_p, _i, _i, _p, _i, _p, _i]),
     "ttg_chol_dsyrk_ln": (_i, [_p, _i, _i, _d, _p, _i, _d, _p, _i]),
     "ttg_chol_dgemm_nt": (_i, [_p, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p, _i]),
     "ttg_chol_dnrm2": (_i, [_p, _i, _p, _p]),

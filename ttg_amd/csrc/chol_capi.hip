@@ -75,6 +75,41 @@ static StreamWork &stream_work(hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
+// scratch of the allocation-free TRSM (ttg_chol_dtrsm_rlt_noalloc): the
+// inverses of the 32 x 32 diagonal blocks of one L of order <= 512, 128 KiB.
+// Every GPU the library runs on holds its own copy until the process ends.
+// Host code never uses the array's name as an address: the device address of
+// each GPU's copy comes from hipGetSymbolAddress, taken once per device with
+// that device current.
+// ---------------------------------------------------------------------------
+constexpr int kScratchDoubles = (TTG_CHOL_MAX_TILE / 32) * kDinvBlock;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kMaxDevices] = {};
+
+static int scratch_ptr(int dev, double **out) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("dtrsm_rlt_noalloc: device %d beyond the %d the library tracks", dev, kMaxDevices);
+    return TTG_CHOL_ENOTSUP;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "dtrsm_rlt_noalloc: hipGetSymbolAddress(scratch)");
+    if (!p) {
+      set_error("dtrsm_rlt_noalloc: hipGetSymbolAddress returned no address");
+      return TTG_CHOL_EHIP;
+    }
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return TTG_CHOL_OK;
+}
+
+// ---------------------------------------------------------------------------
 // launch grouping state (per calling thread)
 // ---------------------------------------------------------------------------
 struct UpdKey {
@@ -195,6 +230,46 @@ int ttg_chol_dtrsm_rlt(void *stream, int m, int n, const double *L, int ldl, dou
   TTGC_HIP(w.dinv.reserve(sizeof(double) * dinv_doubles(n)));
   dj.dinv = static_cast<double *>(w.dinv.ptr);
   op.dinv = dj.dinv;
+  TTGC_HIP(launch_dinv<double>(s, nullptr, 1, dj));
+  TTGC_HIP(launch_trsm<double>(s, nullptr, 1, op, m, nullptr));
+  return TTG_CHOL_OK;
+}
+
+int ttg_chol_dtrsm_rlt_noalloc(void *stream, int m, int n, const double *L, int ldl, double *B, int ldb) {
+  clear_error();
+  if (n > TTG_CHOL_MAX_TILE) {
+    set_error("dtrsm_rlt_noalloc: n=%d exceeds %d, the order its fixed scratch (%d doubles) serves", n,
+              TTG_CHOL_MAX_TILE, kScratchDoubles);
+    return TTG_CHOL_EINVAL;
+  }
+  if (m < 0 || n < 0 || ldl < std::max(1, n) || ldb < std::max(1, m) || (m > 0 && n > 0 && (!L || !B))) {
+    set_error("dtrsm_rlt_noalloc: bad argument (m=%d n=%d ldl=%d ldb=%d)", m, n, ldl, ldb);
+    return TTG_CHOL_EINVAL;
+  }
+  if (m == 0 || n == 0) return TTG_CHOL_OK;
+  hipStream_t s = as_stream(stream);
+  if (grouping(s)) {
+    // a group stages its descriptors in library-allocated memory
+    set_error("dtrsm_rlt_noalloc: a launch group is open on this stream; use ttg_chol_dtrsm_rlt there");
+    return TTG_CHOL_ESTATE;
+  }
+  int dev = 0;
+  TTGC_HIP(hipGetDevice(&dev));
+  if (s) {
+    int sdev = 0;
+    TTGC_HIP(hipStreamGetDevice(s, &sdev));
+    if (sdev != dev) {
+      set_error("dtrsm_rlt_noalloc: the stream belongs to device %d, the current device is %d", sdev, dev);
+      return TTG_CHOL_EINVAL;
+    }
+  }
+  double *dinv = nullptr;
+  int rc = scratch_ptr(dev, &dinv);
+  if (rc != TTG_CHOL_OK) return rc;
+  // dinv_kernel writes every block inverse the TRSM reads (padding blocks as
+  // identity), so the scratch needs no clearing between calls
+  PotrfOp dj{const_cast<double *>(L), dinv, nullptr, n, ldl, 0, 0};
+  TrsmOp<double> op{L, B, dinv, m, n, ldl, ldb};
   TTGC_HIP(launch_dinv<double>(s, nullptr, 1, dj));
   TTGC_HIP(launch_trsm<double>(s, nullptr, 1, op, m, nullptr));
   return TTG_CHOL_OK;
