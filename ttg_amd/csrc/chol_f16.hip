@@ -86,8 +86,15 @@ __global__ __launch_bounds__(256) void split_kernel(const SplitOp *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// update: 128 x 128 output block per workgroup, 2 x 2 waves of 64 x 64,
-// k slabs of 32 staged in LDS (double buffered)
+// update: 128 x 128 output block per workgroup, 2 x 2 waves of 64 x 64, two
+// workgroups per CU.  The 32-k slabs of the A and B panel rows go from HBM
+// straight into LDS with gfx950's LDS-DMA loads (global_load_lds_dwordx4:
+// each lane's 16 bytes land at the wave's LDS base + 16 lane), so no VGPRs
+// stage them and no ds_write runs; two LDS stages, one barrier per slab.
+// The compiler does not track LDS-DMA completion, so every wait is explicit:
+// before the barrier of slab s each wave waits (vmcnt) for its own loads of
+// slab s, and the barrier then publishes all of them and retires every
+// wave's reads of the stage refilled next.
 // ---------------------------------------------------------------------------
 constexpr int kF16BM = 128, kF16BK = 32;  // LDS rows of 32 halves = 4 chunks of 16 bytes
 
@@ -97,51 +104,21 @@ constexpr int kF16BM = 128, kF16BK = 32;  // LDS rows of 32 halves = 4 chunks of
 // 128-byte bank window (rows alternate halves, the XOR walks the chunks).
 __device__ __forceinline__ int swz(int row, int chunk) { return chunk ^ ((row >> 1) & 3); }
 
-template <int NP, int NST = 2>  // parts per operand: 2 (hi, lo) or 1 (hi); LDS stages
-struct F16Smem {
-  h8_t a[NST][NP][kF16BM][kF16BK / 8];  // [buf][part][row][chunk]
-  h8_t b[NST][NP][kF16BM][kF16BK / 8];
-};
-
-// the operand parts of one product: A hi, A lo, B hi, B lo and the row factors
-struct F16Parts {
-  const uint16_t *ah, *al, *bh, *bl;
-  const float *ra, *rb;
-};
-
-template <int NP>
-__device__ __forceinline__ void f16_load(const F16Parts &pp, long ld, int i0, int j0, int k0, int tid,
-                                         h8_t (&ra)[2 * NP], h8_t (&rb)[2 * NP]) {
-  // 128 rows x 32 k per part and operand = 512 16-byte chunks; 2 per thread
-  const int v = tid & 3, r = tid >> 2;  // chunk in row, row 0..63
-  const h8_t *ap[2] = {reinterpret_cast<const h8_t *>(pp.ah), reinterpret_cast<const h8_t *>(pp.al)};
-  const h8_t *bp[2] = {reinterpret_cast<const h8_t *>(pp.bh), reinterpret_cast<const h8_t *>(pp.bl)};
-#pragma unroll
-  for (int p = 0; p < 2; ++p) {
-    const int row = r + 64 * p;
-    const long oa = ((long)(k0 >> 5) * ld + (i0 + row)) * 4 + v;  // [k / 32][row][k % 32]
-    const long ob = ((long)(k0 >> 5) * ld + (j0 + row)) * 4 + v;
-#pragma unroll
-    for (int q = 0; q < NP; ++q) {
-      ra[NP * p + q] = gld(ap[q])[oa];
-      rb[NP * p + q] = gld(bp[q])[ob];
-    }
-  }
+// s_waitcnt vmcnt(N), expcnt and lgkmcnt not waited for (gfx9 encoding:
+// vmcnt bits [3:0] and [15:14], expcnt [6:4], lgkmcnt [11:8])
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
 }
 
-template <int NP, int NST>
-__device__ __forceinline__ void f16_store(F16Smem<NP, NST> &sm, int buf, int tid, const h8_t (&ra)[2 * NP],
-                                          const h8_t (&rb)[2 * NP]) {
-  const int v = tid & 3, r = tid >> 2;
-#pragma unroll
-  for (int p = 0; p < 2; ++p) {
-    const int row = r + 64 * p, c = swz(row, v);
-#pragma unroll
-    for (int q = 0; q < NP; ++q) {
-      sm.a[buf][q][row][c] = ra[NP * p + q];
-      sm.b[buf][q][row][c] = rb[NP * p + q];
-    }
-  }
+// s_barrier without the fence of __syncthreads (which would wait for every
+// load in flight); the empty asm keeps the compiler from moving memory
+// operations across it
+__device__ __forceinline__ void raw_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
 }
 
 typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
@@ -149,39 +126,61 @@ typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
 template <int MODE>
 __global__ __launch_bounds__(256, 2) void update_f16x2_kernel(const F16UpdOp *__restrict__ ops, int bpo, int nbn,
                                                            const int *__restrict__ d_abort) {
-  constexpr int NP = (MODE == kLowF16x2) ? 2 : 1;
+  constexpr int NP = (MODE == kLowF16x2) ? 2 : 1;  // parts per operand: hi, lo
+  constexpr int BM = kF16BM, BN = kF16BM;
+  constexpr int A_CH = NP * BM * 4, ST_CH = A_CH + NP * BN * 4;  // 16-byte chunks per stage
+  constexpr int NSEG = ST_CH / 64;                                 // 1 KB (16-row) segments per stage
+  constexpr int L = NSEG / 4;                                      // LDS-DMA loads per wave per slab
+  constexpr int NCV = 16;                                          // C loads per lane
   if (d_abort && *d_abort) return;
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const int o = b / bpo, rr = b % bpo, bi = rr / nbn, bj = rr % nbn;
   const F16UpdOp op = ops[o];
   if (op.lower && bi < bj) return;
-  const int i0 = bi * kF16BM, j0 = bj * kF16BM;
+  const int i0 = bi * BM, j0 = bj * BN;
   if (i0 >= op.n || j0 >= op.n) return;
-  __shared__ __align__(16) F16Smem<NP, 2> sm;
+  __shared__ __align__(16) h8_t lds[2][ST_CH];  // [stage][A parts | B parts][row][chunk]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int lr = lane & 15, lk = lane >> 4;
+  // npan products (panels), each op.k deep: slab s of the run is slab
+  // s % nk of panel s / nk (a wave-uniform index)
+  const int nk = op.k / kF16BK, npan = op.npan > 1 ? op.npan : 1;
+  const int ns = npan * nk;
+
+  // slab s into stage st: wave w loads segments w, w + 4, ...; lane l of a
+  // segment fills physical chunk l & 3 of the segment's row l >> 2 with the
+  // logical chunk the swizzle puts there (a permutation within each 64-byte
+  // row, so the wave still reads whole 1 KB blocks of the k-slab-major panel)
+  auto issue = [&](int s, int st) {
+    const int q = s / nk, ks = s - q * nk;
+    const long po = q * op.pstride;
+    const int rseg = lane >> 2, p = lane & 3;
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+      const int g = wave + 4 * i;
+      const bool isa = g < NP * BM / 16;
+      const int h = isa ? g : g - NP * BM / 16;
+      const int part = h / (BM / 16), sg = h % (BM / 16);
+      const int row = sg * 16 + rseg;
+      const uint16_t *base = isa ? (part ? op.alo : op.ahi) : (part ? op.blo : op.bhi);
+      const h8_t *src = reinterpret_cast<const h8_t *>(base + po) + ((long)ks * op.ld + (isa ? i0 : j0) + row) * 4 +
+                        (p ^ ((row >> 1) & 3));
+      const int off = (isa ? 0 : A_CH) + part * BM * 4 + sg * 64;
+      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                       (__attribute__((address_space(3))) void *)&lds[st][off], 16, 0, 0);
+    }
+  };
+
   f4_t acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
-  // the first slab's operands go out first, then this lane's C values (4
+  // the first slab's loads go out first, then this lane's C values (4
   // consecutive rows of one column per (i, j)), whose latency then hides
   // behind the first slabs instead of delaying them
-  // npan products (panels), each op.k deep: slab s of the run is slab
-  // s % nk of panel s / nk (a wave-uniform index)
-  const int nk = op.k / kF16BK, npan = op.npan > 1 ? op.npan : 1;
-  auto part = [&](int q) {
-    const long po = q * op.pstride, so = q * op.sstride;
-    return F16Parts{op.ahi + po, op.alo + po, op.bhi + po, op.blo + po, op.ra + so, op.rb + so};
-  };
-  auto load = [&](int s, h8_t(&ra)[2 * NP], h8_t(&rb)[2 * NP]) {
-    const int q = s / nk;
-    f16_load<NP>(part(q), op.ld, i0, j0, (s - q * nk) * kF16BK, tid, ra, rb);
-  };
-  h8_t ra0[2 * NP], rb0[2 * NP];
-  load(0, ra0, rb0);
+  issue(0, 0);
   f4_t cv[4][4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
@@ -190,19 +189,19 @@ __global__ __launch_bounds__(256, 2) void update_f16x2_kernel(const F16UpdOp *__
     for (int i = 0; i < 4; ++i) cv[i][j] = *gld(reinterpret_cast<const f4_t *>(pc + i * 16));
   }
 
-  auto compute = [&](int cur) {
+  auto compute = [&](int st) {
     h8_t ah[NP][4], bh[NP][4];
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int row = wm * 64 + i * 16 + lr;
-        ah[q][i] = sm.a[cur][q][row][swz(row, lk)];
+        ah[q][i] = lds[st][q * BM * 4 + row * 4 + swz(row, lk)];
       }
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int row = wn * 64 + j * 16 + lr;
-        bh[q][j] = sm.b[cur][q][row][swz(row, lk)];
+        bh[q][j] = lds[st][A_CH + q * BN * 4 + row * 4 + swz(row, lk)];
       }
     }
 #pragma unroll
@@ -224,14 +223,14 @@ __global__ __launch_bounds__(256, 2) void update_f16x2_kernel(const F16UpdOp *__
   // cv -= acc * (row factor x column factor) of panel q: the products with
   // powers of two are exact, so this is the one rounding of an FP32 update
   auto fold = [&](int q) {
-    const F16Parts pq = part(q);
+    const long so = q * op.sstride;
     f4_t rrow[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      rrow[i] = *gld(reinterpret_cast<const f4_t *>(pq.ra + i0 + wm * 64 + i * 16 + 4 * lk));
+      rrow[i] = *gld(reinterpret_cast<const f4_t *>(op.ra + so + i0 + wm * 64 + i * 16 + 4 * lk));
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const float rc = gld(pq.rb)[j0 + wn * 64 + j * 16 + lr];
+      const float rc = gld(op.rb + so)[j0 + wn * 64 + j * 16 + lr];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         cv[i][j] = cv[i][j] - acc[i][j] * (rrow[i] * rc);
@@ -239,17 +238,18 @@ __global__ __launch_bounds__(256, 2) void update_f16x2_kernel(const F16UpdOp *__
       }
     }
   };
-  const int ns = npan * nk;
-  f16_store<NP, 2>(sm, 0, tid, ra0, rb0);
-  __syncthreads();
   for (int s = 0; s < ns; ++s) {
-    const int cur = s & 1;
-    const bool more = s + 1 < ns;
-    if (more) load(s + 1, ra0, rb0);
-    compute(cur);
+    // this wave's loads of slab s are done once at most the C loads issued
+    // after them (slab 0) are outstanding; the barrier makes every wave's
+    // visible and retires all reads of stage (s + 1) & 1 (slab s - 1)
+    if (s == 0)
+      wait_vmcnt<NCV>();
+    else
+      wait_vmcnt<0>();
+    raw_barrier();
+    if (s + 1 < ns) issue(s + 1, (s + 1) & 1);
+    compute(s & 1);
     if ((s + 1) % nk == 0 && s + 1 < ns) fold(s / nk);  // a panel done, more to come
-    if (more) f16_store<NP, 2>(sm, cur ^ 1, tid, ra0, rb0);
-    __syncthreads();
   }
   fold(npan - 1);
   const bool diag = op.lower && bi == bj;
@@ -271,7 +271,6 @@ __global__ __launch_bounds__(256, 2) void update_f16x2_kernel(const F16UpdOp *__
     }
   }
 }
-
 
 hipError_t launch_split(hipStream_t s, const SplitOp *ops, int nops, int NB, int mode) {
   if (nops <= 0) return hipSuccess;
