@@ -1,7 +1,6 @@
 """The schedules of the planned solve against each other: the diagonal POTRF
-on its own CUs beside the bulk update (TTG_CHOL_POTRF_CUS), the triangular
-sweeps' critical path on its own CUs beside the far-block updates
-(TTG_CHOL_SWEEP_CUS), and the paired split-FP16 schedule whose bulk updates
+on its own CUs beside the bulk update (TTG_CHOL_POTRF_CUS), and the paired
+split-FP16 schedule whose bulk updates
 apply two panels in one pass over C (TTG_CHOL_PAIRED).  All of them only
 reorder independent work or fuse two FP32 updates of a tile in their own
 order, so the solution must be bitwise the one of the plain two-stream /
@@ -40,10 +39,8 @@ def test_split_schedules_match_plain(dev, oracle, monkeypatch, prec, r1, r2):
     Bh = np.random.default_rng(5).uniform(-1, 1, (N, r2))
     dA = to_dev(A0, dev)
     monkeypatch.setenv("TTG_CHOL_POTRF_CUS", "8")
-    monkeypatch.setenv("TTG_CHOL_SWEEP_CUS", "32")
     Xs = _solve(C, dA, N, NB, Bh, r1, prec, dev)
     monkeypatch.setenv("TTG_CHOL_POTRF_CUS", "0")
-    monkeypatch.setenv("TTG_CHOL_SWEEP_CUS", "0")
     Xp = _solve(C, dA, N, NB, Bh, r2, prec, dev)
     assert np.array_equal(Xs, Xp[:, :r1])
     assert oracle.solve_backward_error(N, NB, A0, Xs, Bh[:, :r1]) <= 1e-12

@@ -91,22 +91,19 @@ __global__ void identity_tiles_kernel(int N, int NB, T *__restrict__ linv) {
 }
 
 // ---------------------------------------------------------------------------
-// The two GEMMs on the critical path of a sweep step, split over k:
+// The inverse apply of a sweep step, split over k:
 //   P[y][c][r] = sum_{kk in chunk y} A(r, kk) B(c, kk),   A (nr x k), B (n x k),
 // B(c, kk) = B[c + kk ldb] (TB: B[kk + c ldb]).  Workgroup (x, y) takes the
 // 64 columns x and the 64-k chunk y, so a 512 x 512 tile is streamed by 64
-// workgroups with all their loads in flight at once.  With SUB, the A operand
-// is first reduced from an earlier product:  A(r, kk) - sum_q Q[q][kk][r]
-// (the next-block update folded into the inverse apply that consumes it).
-// apply_reduce_kernel adds the chunks in order (deterministic).
+// workgroups with all their loads in flight at once.  apply_reduce_kernel
+// adds the chunks in order (deterministic).
 // ---------------------------------------------------------------------------
 constexpr int kApB = 64;
 constexpr int kMaxChunks = 8;  // k chunks of a product: NB <= 512
 
-template <typename T, bool TB, bool SUB>
+template <typename T, bool TB>
 __global__ __launch_bounds__(256) void apply_partial_kernel(int nr, int n, int k, const T *__restrict__ A, int lda,
-                                                            const T *__restrict__ B, int ldb, const T *__restrict__ Q,
-                                                            int nq, T *__restrict__ P) {
+                                                            const T *__restrict__ B, int ldb, T *__restrict__ P) {
   const int c0 = blockIdx.x * kApB, k0 = blockIdx.y * kApB;
   __shared__ T bs[kApB][kApB + 1];  // bs[kk][cc] = B(c0 + cc, k0 + kk)
   __shared__ T as[kApB][17];        // as[kk][r] = A(r0 + r, k0 + kk), 16 right-hand sides at a time
@@ -131,17 +128,7 @@ __global__ __launch_bounds__(256) void apply_partial_kernel(int nr, int n, int k
     for (int i = 0; i < 4; ++i) {
       const int e = tid + 256 * i, r = e & 15, kk = e >> 4;
       const bool ok = r0 + r < nr && k0 + kk < k;
-      T v = ok ? A[(r0 + r) + (long)(k0 + kk) * lda] : T(0);
-      if constexpr (SUB) {
-        // all chunks loaded before the first subtraction (nq <= 8), then
-        // subtracted in chunk order
-        T qv[kMaxChunks];
-#pragma unroll
-        for (int q = 0; q < kMaxChunks; ++q) qv[q] = (ok && q < nq) ? Q[((long)q * k + k0 + kk) * nr + r0 + r] : T(0);
-#pragma unroll
-        for (int q = 0; q < kMaxChunks; ++q) v -= qv[q];
-      }
-      av[i] = v;
+      av[i] = ok ? A[(r0 + r) + (long)(k0 + kk) * lda] : T(0);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -180,26 +167,16 @@ __global__ void apply_reduce_kernel(int nr, int n, int nkc, const T *__restrict_
   C[(long)c * ldc + r] = s;
 }
 
-// P := the k-split product of op (A B^T); with Q, A is op.A - sum of the nq
-// chunks of Q first.
+// P := the k-split product of op (A B^T).
 template <typename T>
-static hipError_t launch_partial(hipStream_t s, const UpdOp<T> &op, bool tb, const T *Q, int nq, T *P) {
+static hipError_t launch_partial(hipStream_t s, const UpdOp<T> &op, bool tb, T *P) {
   const dim3 g((op.n + kApB - 1) / kApB, (op.k + kApB - 1) / kApB);
-  if (tb) {
-    if (Q)
-      hipLaunchKernelGGL((apply_partial_kernel<T, true, true>), g, dim3(256), 0, s, op.m, op.n, op.k, op.A, op.lda,
-                         op.B, op.ldb, Q, nq, P);
-    else
-      hipLaunchKernelGGL((apply_partial_kernel<T, true, false>), g, dim3(256), 0, s, op.m, op.n, op.k, op.A, op.lda,
-                         op.B, op.ldb, Q, nq, P);
-  } else {
-    if (Q)
-      hipLaunchKernelGGL((apply_partial_kernel<T, false, true>), g, dim3(256), 0, s, op.m, op.n, op.k, op.A, op.lda,
-                         op.B, op.ldb, Q, nq, P);
-    else
-      hipLaunchKernelGGL((apply_partial_kernel<T, false, false>), g, dim3(256), 0, s, op.m, op.n, op.k, op.A, op.lda,
-                         op.B, op.ldb, Q, nq, P);
-  }
+  if (tb)
+    hipLaunchKernelGGL((apply_partial_kernel<T, true>), g, dim3(256), 0, s, op.m, op.n, op.k, op.A, op.lda, op.B,
+                       op.ldb, P);
+  else
+    hipLaunchKernelGGL((apply_partial_kernel<T, false>), g, dim3(256), 0, s, op.m, op.n, op.k, op.A, op.lda, op.B,
+                       op.ldb, P);
   return hipGetLastError();
 }
 
@@ -220,7 +197,8 @@ int SolvePlan<T>::build() {
   TTGC_HIP(d_xt.reserve(sizeof(T) * (size_t)Np * nrhs));
   TTGC_HIP(d_dinv.reserve(sizeof(T) * (size_t)dstride * NT));
   TTGC_HIP(d_linv.reserve(sizeof(T) * (size_t)NB * NB * NT));
-  TTGC_HIP(d_part.reserve(sizeof(T) * 2 * (size_t)((NB + kApB - 1) / kApB) * NB * nrhs));
+  // the inverse apply's k chunks
+  TTGC_HIP(d_part.reserve(sizeof(T) * (size_t)((NB + kApB - 1) / kApB) * NB * nrhs));
   T *bt = static_cast<T *>(d_bt.ptr);
   T *xt = static_cast<T *>(d_xt.ptr);
   T *dinv = static_cast<T *>(d_dinv.ptr);
@@ -269,19 +247,7 @@ int SolvePlan<T>::build() {
   if (!upd.empty()) memcpy(host.data() + o_upd, upd.data(), upd.size() * sizeof(UpdOp<T>));
   TTGC_HIP(hipMemcpy(d_ops.ptr, host.data(), total, hipMemcpyHostToDevice));
   host_ops.assign(upd.begin(), upd.end());
-  // the critical path of the sweeps (next-block product, inverse apply) on 32
-  // CUs of its own, the bulk updates on the others; plain streams (and a
-  // replayed graph) if the masks are refused or TTG_CHOL_SWEEP_CUS=0
-  {
-    const char *e = getenv("TTG_CHOL_SWEEP_CUS");
-    split = make_split_streams(e ? atoi(e) : 32, &s_main, &s_rest);
-  }
-  if (!split) {
-    TTGC_HIP(hipStreamCreateWithFlags(&s_main, hipStreamNonBlocking));
-    TTGC_HIP(hipStreamCreateWithFlags(&s_rest, hipStreamNonBlocking));
-  }
-  for (hipEvent_t *e : {&ev_in, &ev_out, &ev_x, &ev_rest[0], &ev_rest[1]})
-    TTGC_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+  TTGC_HIP(hipStreamCreateWithFlags(&s_main, hipStreamNonBlocking));
   return TTG_CHOL_OK;
 }
 
@@ -296,48 +262,33 @@ int SolvePlan<T>::prepare(hipStream_t s) {
   return TTG_CHOL_OK;
 }
 
-// Sweep step q (block i): the update of the remaining blocks from x_i is
-// split in two.  The next block's (it feeds the next inverse apply) runs on s
-// as a k-split product whose chunks the next apply subtracts from its
-// operand as it loads it; the rest runs on s_rest as a batched update,
-// overlapping the following steps.  The next apply reads the block the rest
-// update of step q-1 wrote, so it waits for that one (events alternate
-// between two slots).
+// Sweep step q solves block i (x_i known) and prepares block i' = i +- 1,
+// three launches on one stream: (1) the updates of every remaining block
+// from x_i, one batched launch (HBM-bound early on: the whole column of L
+// tiles is read), (2) the inverse apply of block i' as a k-split product
+// (every load in flight at once), (3) its chunk reduction, giving x_i'.  The
+// sweeps are replayed as one graph on one stream: kernels then follow each
+// other with no gap, while a dependency across queues costs ~20 us on this
+// GPU and a directly launched step is bound by the host's launch calls.
 template <typename T>
 int SolvePlan<T>::enqueue_sweeps(hipStream_t s) {
   UpdOp<T> unone{};
   const T m1 = T(-1), p1 = T(1);
   const bool fast = NB % 128 == 0 && N % NB == 0;
-  T *pa = static_cast<T *>(d_part.ptr);                          // apply chunks
-  T *pn = pa + (size_t)((NB + kApB - 1) / kApB) * NB * nrhs;  // next-block update chunks
+  T *pa = static_cast<T *>(d_part.ptr);  // the apply's chunks
   for (int dir = 0; dir < 2; ++dir) {
     const bool tb = dir == 1;
     const std::vector<int> &app = tb ? bw_app : fw_app, &off = tb ? bw_upd_off : fw_upd_off,
                            &cnt = tb ? bw_upd_cnt : fw_upd_cnt;
-    bool pending = false;  // a rest update of the previous step is in flight on s_rest
-    int slot = 0;
-    TTGC_HIP(launch_partial<T>(s, host_ops[app[0]], tb, nullptr, 0, pa));
+    TTGC_HIP(launch_partial<T>(s, host_ops[app[0]], tb, pa));
     TTGC_HIP(launch_reduce<T>(s, host_ops[app[0]], pa));
     for (int q = 0; q < NT; ++q) {
       const int c = cnt[q];
       if (c == 0) break;
-      // forward: next block is the first op (m = i + 1); backward: the last (m = i - 1)
-      const int next = tb ? off[q] + c - 1 : off[q], rest = tb ? off[q] : off[q] + 1;
-      if (c > 1) {
-        TTGC_HIP(hipEventRecord(ev_x, s));
-        TTGC_HIP(hipStreamWaitEvent(s_rest, ev_x, 0));
-        TTGC_HIP(launch_update<T>(s_rest, upd_ops() + rest, c - 1, unone, nrhs, NB, m1, p1, fast, nullptr, tb));
-      }
-      const UpdOp<T> &on = host_ops[next], &oa = host_ops[app[q + 1]];
-      TTGC_HIP(launch_partial<T>(s, on, tb, nullptr, 0, pn));
-      if (pending) TTGC_HIP(hipStreamWaitEvent(s, ev_rest[slot ^ 1], 0));
-      pending = c > 1;
-      if (pending) TTGC_HIP(hipEventRecord(ev_rest[slot], s_rest));
-      slot ^= 1;
-      TTGC_HIP(launch_partial<T>(s, oa, tb, pn, (on.k + kApB - 1) / kApB, pa));
-      TTGC_HIP(launch_reduce<T>(s, oa, pa));
+      TTGC_HIP(launch_update<T>(s, upd_ops() + off[q], c, unone, nrhs, NB, m1, p1, fast, nullptr, tb));
+      TTGC_HIP(launch_partial<T>(s, host_ops[app[q + 1]], tb, pa));
+      TTGC_HIP(launch_reduce<T>(s, host_ops[app[q + 1]], pa));
     }
-    if (pending) TTGC_HIP(hipStreamWaitEvent(s, ev_rest[slot ^ 1], 0));
   }
   return TTG_CHOL_OK;
 }
@@ -346,27 +297,15 @@ template <typename T>
 SolvePlan<T>::~SolvePlan() {
   if (gexec) (void)hipGraphExecDestroy(gexec);
   if (graph) (void)hipGraphDestroy(graph);
-  for (hipEvent_t e : {ev_in, ev_out, ev_x, ev_rest[0], ev_rest[1]})
-    if (e) (void)hipEventDestroy(e);
   if (s_main) (void)hipStreamDestroy(s_main);
-  if (s_rest) (void)hipStreamDestroy(s_rest);
   for (DevBuf *b : {&d_ops, &d_bt, &d_xt, &d_dinv, &d_linv, &d_part})
     if (b->ptr) (void)hipFree(b->ptr);
 }
 
-// Run the two sweeps on Bt (already filled) as a replayed graph, ordered
-// after `user` and before whatever `user` does next.
+// Run the two sweeps on Bt (already filled) as a replayed graph (captured on
+// the first run), ordered after `user` and before whatever `user` does next.
 template <typename T>
 int SolvePlan<T>::run_sweeps(hipStream_t user) {
-  if (split) {  // CU masks live on the queues: launch directly
-    TTGC_HIP(hipEventRecord(ev_in, user));
-    TTGC_HIP(hipStreamWaitEvent(s_main, ev_in, 0));
-    int rc = enqueue_sweeps(s_main);
-    if (rc != TTG_CHOL_OK) return rc;
-    TTGC_HIP(hipEventRecord(ev_out, s_main));
-    TTGC_HIP(hipStreamWaitEvent(user, ev_out, 0));
-    return TTG_CHOL_OK;
-  }
   if (!gexec) {
     TTGC_HIP(hipStreamBeginCapture(s_main, hipStreamCaptureModeThreadLocal));
     int rc = enqueue_sweeps(s_main);

@@ -21,13 +21,11 @@ template <typename T>
 struct SolvePlan {
   int N = 0, NB = 0, NT = 0, Np = 0, nrhs = 0, dstride = 0;
   const T *tiles = nullptr;
-  DevBuf d_ops, d_bt, d_xt, d_dinv, d_linv, d_part;  // d_part: split-k partials of one inverse apply
+  DevBuf d_ops, d_bt, d_xt, d_dinv, d_linv, d_part;  // d_part: k-split chunks of one inverse apply
   size_t o_dj = 0, o_inv = 0, o_upd = 0;
   std::vector<int> fw_app, fw_upd_off, fw_upd_cnt, bw_app, bw_upd_off, bw_upd_cnt;
   std::vector<UpdOp<T>> host_ops;  // host copy of the sweep descriptors (the applies read theirs from it)
-  hipStream_t s_main = nullptr, s_rest = nullptr;  // s_rest: the far part of each sweep update
-  bool split = false;                               // s_main / s_rest on disjoint CU masks
-  hipEvent_t ev_in = nullptr, ev_out = nullptr, ev_x = nullptr, ev_rest[2] = {nullptr, nullptr};
+  hipStream_t s_main = nullptr;  // the stream the sweep graph is captured on
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
 
