@@ -18,10 +18,12 @@ extern "C" {
 /* Factor the tiles in place with the task graph: the body of
  * examples/potrf/testing_dpotrf.cc:227-262 (load -> make_potrf_ttg -> result
  * writer, then fence).  stats (optional, 4 longs): tasks run, scheduling
- * rounds, batched kernel launches, largest round.  Returns 0, -1 on a bad
- * argument; a tile that is not positive definite aborts the process, as the
- * reference's POTRF task does (potrf.h:160-164). */
-int ttg_graph_dpotrf(int N, int NB, double *tiles, long *stats);
+ * rounds, batched kernel launches, largest round.  *info (optional): 0, or
+ * LAPACK's info (1-based column of the first non-positive pivot) when a tile
+ * is not positive definite; the graph then stops cleanly after the rounds in
+ * flight, leaving the tiles partly factored (the reference's POTRF task ends
+ * the process instead, potrf.h:183-187).  Returns 0, -1 on a bad argument. */
+int ttg_graph_dpotrf(int N, int NB, double *tiles, long *stats, int *info);
 
 /* PLGSY(N, bump, seed) into the tiles with one device task per tile (the
  * PLGSY template task of examples/potrf/plgsy.h:7-50; values bit-identical

@@ -80,8 +80,8 @@ def test_graph_rejects_bad_arguments_without_gpu():
     import ttg_amd.graph as G
 
     t = np.zeros(4, dtype=np.float64)
-    assert G.lib().ttg_graph_dpotrf(0, 32, t.ctypes.data, None) == -1
-    assert G.lib().ttg_graph_dpotrf(100, 1024, t.ctypes.data, None) == -1
+    assert G.lib().ttg_graph_dpotrf(0, 32, t.ctypes.data, None, None) == -1
+    assert G.lib().ttg_graph_dpotrf(100, 1024, t.ctypes.data, None, None) == -1
     assert G.lib().ttg_graph_dplgsy(100, 0, 1.0, 1, t.ctypes.data) == -1
 
 

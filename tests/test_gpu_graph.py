@@ -97,6 +97,34 @@ def test_graph_dpotrf_vs_oracle(oracle, N, NB, gen):
     assert lower_tiles_rel_err(oracle, N, NB, got, ref) <= 1e-12
 
 
+@pytest.mark.parametrize("N,NB,col", [(700, 64, 300), (1000, 128, 1), (1000, 128, 129), (1000, 128, 1000)])
+def test_graph_stops_with_lapack_info(oracle, N, NB, col):
+    """A matrix whose leading minor of order `col` is not positive definite
+    (a_{col,col} planted negative, 1-based): the graph stops cleanly with
+    LAPACK's info = col, the oracle's tiled factor reports the same, and the
+    next factorization through the same runtime runs in full."""
+    import ttg_amd.graph as G
+
+    A0 = oracle.spd_tiles(N, NB, 1.0, 5)
+    t, i = (col - 1) // NB, (col - 1) % NB
+    oracle.tile_view(A0, N, NB, t, t)[i, i] = -1.0
+    ref = A0.copy()
+    want = oracle.potrf_tiled(N, NB, ref)
+    assert want == col
+    got = A0.copy()
+    print(f"graph dpotrf N={N} NB={NB}, planted pivot at {col}", flush=True)
+    st = G.dpotrf(N, NB, got)
+    assert st["info"] == want, st
+    assert st["tasks"] < expected_tasks(oracle.nt_of(N, NB)), st
+    # the runtime is ready for the next graph
+    good = oracle.spd_tiles(N, NB, 1.0, 6)
+    ref = good.copy()
+    assert oracle.potrf_tiled(N, NB, ref) == 0
+    st = G.dpotrf(N, NB, good)
+    assert st["info"] == 0 and st["tasks"] == expected_tasks(oracle.nt_of(N, NB)), st
+    assert lower_tiles_rel_err(oracle, N, NB, good, ref) <= 1e-12
+
+
 @pytest.mark.parametrize("flows", [0, 1, 2, 4, 8])
 def test_chain_benchmark(flows):
     """The device-task chain benchmark (task-benchmarks/chain-ttg-dev.cc):

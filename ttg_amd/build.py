@@ -47,6 +47,7 @@ DEBUG_DRIVER = os.path.join(HERE, "bin", "testing_dpotrf_debug")
 POINV_DRIVER = os.path.join(HERE, "bin", "testing_dpoinv")
 # device-task overhead benchmark (a chain of dependent device tasks)
 CHAIN_BENCH = os.path.join(HERE, "bin", "chain_ttg_dev")
+RUNTIME_CHECKS = os.path.join(HERE, "bin", "runtime_checks")
 # host-only C++ (no device code): clang++ with the HIP runtime headers
 CXX = os.environ.get("TTG_CXX", "/opt/rocm/llvm/bin/clang++")
 HOST_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
@@ -86,6 +87,9 @@ def build_graph(verbose=False):
          os.path.join(HERE, "examples", "task-benchmarks", "chain_dev.cc")] + link + HOST_LIBS, "chain_ttg_dev",
          verbose)
     os.replace(CHAIN_BENCH + ".tmp", CHAIN_BENCH)
+    _run([CXX] + HOST_FLAGS + INCLUDES + ["-o", RUNTIME_CHECKS + ".tmp",
+         os.path.join(HERE, "examples", "runtime", "runtime_checks.cc")] + link + HOST_LIBS, "runtime_checks", verbose)
+    os.replace(RUNTIME_CHECKS + ".tmp", RUNTIME_CHECKS)
 
 
 def build(verbose=False):

@@ -14,10 +14,14 @@ extern "C" {
 
 // Factor the N x N SPD matrix held in `tiles` (host, lower tile-packed) in
 // place with the TTG graph; stats (optional, 4 longs): tasks run, scheduling
-// rounds, kernel launches, largest round.  Returns 0, or -1 on a bad argument.
-int ttg_graph_dpotrf(int N, int NB, double *tiles, long *stats) {
+// rounds, kernel launches, largest round.  *info (optional) receives 0, or
+// LAPACK's info of a tile that is not positive definite: the graph stops
+// there and the tiles are left partly factored.  Returns 0, or -1 on a bad
+// argument.
+int ttg_graph_dpotrf(int N, int NB, double *tiles, long *stats, int *info) {
   if (N <= 0 || NB <= 0 || NB > TTG_CHOL_MAX_TILE || !tiles) return -1;
   ttg::initialize(1, nullptr, 1);
+  ttg::clear_cancel();
   auto &rt = ttg::detail::Runtime::get();
   const long t0 = rt.tasks_run, r0 = rt.rounds, k0 = rt.kernel_launches;
   rt.max_round = 0;
@@ -38,6 +42,8 @@ int ttg_graph_dpotrf(int N, int NB, double *tiles, long *stats) {
     stats[2] = rt.kernel_launches - k0;
     stats[3] = rt.max_round;
   }
+  if (info) *info = ttg::cancelled() ? (int)ttg::cancel_info() : 0;
+  ttg::clear_cancel();
   return 0;
 }
 

@@ -81,7 +81,10 @@ auto make_potrf(MatrixT &A, ttg::Edge<Key1, MatrixTile<typename MatrixT::element
 #endif
     if (host_info != 0) {
       std::cerr << "Factorization of tile " << K << " failed: " << host_info << std::endl;
-      ttg::abort();
+      // the reference ends the process here (potrf.h:183-187); this graph
+      // stops cleanly with LAPACK's info, the 1-based column of the matrix
+      ttg::cancel((long)K * (long)A.rows_in_tile() + host_info);
+      co_return;
     }
     co_await ttg::device::forward(
         ttg::device::broadcast<0, 1>(std::make_tuple(Key2(K, K), std::move(keylist)), std::move(tile_kk), out));

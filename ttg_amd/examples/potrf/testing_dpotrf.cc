@@ -110,6 +110,7 @@ int main(int argc, char **argv) {
       ttg::execute(world);
       ttg::fence(world);
       auto end = std::chrono::high_resolution_clock::now();
+      if (ttg::cancelled()) break;
       const double us = std::chrono::duration_cast<std::chrono::microseconds>(end - beg).count();
       std::cout << "TTG Execution Time (milliseconds) : " << us / 1e3 << " : Flops " << potrf::FLOPS_DPOTRF(N) << " "
                 << (potrf::FLOPS_DPOTRF(N) / 1e9) / (us / 1e6) << " GF/s" << std::endl;
@@ -136,8 +137,14 @@ int main(int argc, char **argv) {
       load->invoke();
       ttg::fence(world);
     }
-    std::vector<double> L = A.getLAPACKMatrix();
-    if (check_dpotrf(L, A0, N) == -1) ret = EXIT_FAILURE;
+    if (!ttg::cancelled()) {
+      std::vector<double> L = A.getLAPACKMatrix();
+      if (check_dpotrf(L, A0, N) == -1) ret = EXIT_FAILURE;
+    }
+  }
+  if (ttg::cancelled()) {
+    std::cout << "Factorization stopped: info = " << ttg::cancel_info() << std::endl;
+    ret = EXIT_FAILURE;
   }
   auto &rt = ttg::detail::Runtime::get();
   std::cout << "runtime: tasks " << rt.tasks_run << " rounds " << rt.rounds << " kernel launches "

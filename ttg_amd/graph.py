@@ -12,7 +12,8 @@ DRIVER = os.path.join(HERE, "bin", "testing_dpotrf")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "ttg_graph.h")
 
 SIGNATURES = {
-    "ttg_graph_dpotrf": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "ttg_graph_dpotrf": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p]),
     "ttg_graph_dplgsy": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_ulong,
                                         ctypes.c_void_p]),
 }
@@ -50,12 +51,15 @@ def _tiles(t, N, NB):
 
 def dpotrf(N, NB, tiles):
     """Factor host tiles in place with the task graph; returns the runtime's
-    statistics (tasks, rounds, kernel launches, largest round)."""
+    statistics (tasks, rounds, kernel launches, largest round) and LAPACK's
+    info (0, or the 1-based column of the first non-positive pivot, at which
+    the graph stopped)."""
     stats = (ctypes.c_long * 4)()
-    rc = lib().ttg_graph_dpotrf(N, NB, _tiles(tiles, N, NB), ctypes.addressof(stats))
+    info = ctypes.c_int(-1)
+    rc = lib().ttg_graph_dpotrf(N, NB, _tiles(tiles, N, NB), ctypes.addressof(stats), ctypes.addressof(info))
     if rc != 0:
         raise ValueError(f"ttg_graph_dpotrf(N={N}, NB={NB}) rejected its arguments")
-    return {"tasks": stats[0], "rounds": stats[1], "launches": stats[2], "max_round": stats[3]}
+    return {"tasks": stats[0], "rounds": stats[1], "launches": stats[2], "max_round": stats[3], "info": info.value}
 
 
 def dplgsy(N, NB, bump, seed, tiles):

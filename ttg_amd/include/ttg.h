@@ -29,10 +29,26 @@
 //     device/task.h:205-213, 677-682).  Up to two rounds are in flight, so the
 //     host prepares round r+1 while the GPU runs round r; a round that left
 //     no work on the stream completes at once.
-//   * Host tasks run on the calling thread; values they receive are pushed
-//     back to the host first (the push-out of parsec/ttg.h:3560-3562).
+//   * Host tasks run on the calling thread.  A buffer's host copy is brought
+//     up to date when host code first asks for it (Buffer::host_ptr), so a
+//     host task that only routes a tile costs no copy; the copy runs on a
+//     stream of its own, after the round that last wrote the buffer (the
+//     push-out of parsec/ttg.h:3524-3579, done lazily).  A host task that
+//     takes a value by && and touched its host copy leaves the host copy as
+//     the current one.
 //   * Values travel as shared objects: a broadcast shares one value among all
 //     receivers, and re-sending a task's own input forwards the same object.
+//     A task that takes a value by && (a writer) while other tasks still hold
+//     it waits until it is the last holder when its TT defers writers
+//     (set_defer_writer(true)), and otherwise works on a copy of its own
+//     (register_data_copy, parsec/ttg.h:958-1060).
+//   * A device task that waits (co_await wait(...)) resumes once its round
+//     has completed; its buffers are copied back on the copy stream, and the
+//     kernels it launches afterwards run on a stream of their own, so neither
+//     waits for the rounds issued since.
+//   * ttg::cancel(info) stops the graph: no further task starts, the rounds
+//     in flight complete, fence() returns, and the info is kept for the
+//     caller (Runtime::error_info).
 #pragma once
 
 #include <hip/hip_runtime_api.h>
@@ -176,7 +192,10 @@ struct BufferBase {
   std::size_t bytes = 0;
   bool host_valid = true;
   bool dev_valid = false;
+  uint64_t write_round = 0;  // the last scheduling round that selected the buffer for writing
   virtual ~BufferBase();
+  // the host copy up to date (a push-out from HBM if the HBM copy is newer)
+  void sync_host();
 };
 
 struct BufferUse {
@@ -206,8 +225,16 @@ class Buffer : public detail::BufferBase {
   }
   ~Buffer() override { release_dev(); }
 
-  T *host_ptr() { return static_cast<T *>(host); }
-  const T *host_ptr() const { return static_cast<const T *>(host); }
+  // Host access pushes the HBM copy back first when that one is newer: a
+  // host task that only routes a tile (never reads it) costs no copy.
+  T *host_ptr() {
+    sync_host();
+    return static_cast<T *>(host);
+  }
+  const T *host_ptr() const {
+    const_cast<Buffer *>(this)->sync_host();
+    return static_cast<const T *>(host);
+  }
   T *current_device_ptr() const { return static_cast<T *>(dev); }
   T *device_ptr() const { return static_cast<T *>(dev); }
   std::size_t size() const { return n_; }
@@ -304,6 +331,13 @@ struct TaskBase {
   virtual void run_host() = 0;                   // host task
   virtual std::coroutine_handle<> start() = 0;   // device task: run to first co_await
   virtual void finish() {}                       // release inputs
+  // The value of a && input that other tasks still hold, when the TT defers
+  // writers (set_defer_writer(true)): the task waits until it is the last
+  // holder.  nullptr: nothing to wait for.
+  virtual const void *writer_blocker() const { return nullptr; }
+  // replace every && input that other tasks still hold by a copy of its own
+  virtual void copy_shared_writers() {}
+  virtual void after_host_run() {}  // host writers: their host copies are now the newer ones
   // the current task's inputs, for forwarding re-sent values without copies
   std::vector<std::pair<const void *, std::shared_ptr<void>>> inputs;
   virtual ~TaskBase() = default;
@@ -330,6 +364,7 @@ class Runtime {
     hip_check(hipSetDevice(0), "hipSetDevice");
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipStreamCreateWithFlags(&copy_, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipStreamCreateWithFlags(&eager_, hipStreamNonBlocking), "hipStreamCreate");
     // TTG_ROUND_MIN / TTG_ROUND_MAX override the round sizes (below)
     if (const char *e = std::getenv("TTG_ROUND_MIN")) min_batch = std::max(1, std::atoi(e));
     if (const char *e = std::getenv("TTG_ROUND_MAX")) max_batch = std::max(1, std::atoi(e));
@@ -341,6 +376,7 @@ class Runtime {
     pool_.release();
     (void)hipStreamDestroy(stream_);
     (void)hipStreamDestroy(copy_);
+    (void)hipStreamDestroy(eager_);
     inited_ = false;
   }
   bool inited() const { return inited_; }
@@ -349,14 +385,72 @@ class Runtime {
   TaskBase *current() const { return current_; }
 
   void schedule(TaskBase *t) {
+    if (cancelled_) {  // a stopped graph runs nothing more
+      t->finish();
+      delete t;
+      return;
+    }
     t->seq = seq_++;
     ready_.push(t);
   }
 
+  // Stop the graph: no task runs after the current one (ready tasks and
+  // tasks that become ready are dropped, rounds in flight complete), and
+  // fence() returns.  The first failure's info is kept for the caller.
+  void cancel(long info) {
+    if (!cancelled_) error_info_ = info;
+    cancelled_ = true;
+  }
+  bool cancelled() const { return cancelled_; }
+  long error_info() const { return error_info_; }
+  // ready for the next graph
+  void clear_cancel() {
+    cancelled_ = false;
+    error_info_ = 0;
+  }
+
   void fence();
 
+  // Writers (tasks taking a value by &&) when their TT defers writers: a
+  // writer whose value other tasks still hold waits for it under that value
+  // and is woken when a holder releases it.  Returns false when the value
+  // already has a waiting writer (the caller then copies it).
+  bool defer(TaskBase *t, const void *value) {
+    if (waiting_.count(value)) return false;
+    waiting_.emplace(value, t);
+    return true;
+  }
+  bool has_waiting() const { return !waiting_.empty(); }
+  // a holder of `value` let it go: its waiting writer, if any, runs now or
+  // waits for the next value it shares
+  void released(const void *value) {
+    auto it = waiting_.find(value);
+    if (it == waiting_.end()) return;
+    TaskBase *t = it->second;
+    waiting_.erase(it);
+    admit(t);
+  }
+  // a task taken from the ready queue: false if it had to wait (deferred)
+  bool admit_check(TaskBase *t) {
+    const void *v = t->writer_blocker();
+    if (!v) return true;
+    if (!defer(t, v)) {
+      t->copy_shared_writers();
+      return true;
+    }
+    return false;
+  }
+  void admit(TaskBase *t) {
+    if (cancelled_) {
+      t->finish();
+      delete t;
+      return;
+    }
+    if (admit_check(t)) ready_.push(t);
+  }
+
   // statistics
-  long tasks_run = 0, rounds = 0, kernel_launches = 0;
+  long tasks_run = 0, rounds = 0, kernel_launches = 0, push_outs = 0;
   long max_round = 0;
   int max_inflight = 2;
   // Round sizes: forming a round costs host time per task (~1 us) while the
@@ -374,10 +468,24 @@ class Runtime {
       hip_check(hipMemcpyAsync(b->dev, b->host, b->bytes, hipMemcpyHostToDevice, stream_), "H2D");
     if (sync_in) b->dev_valid = true;
   }
-  void push_out(BufferBase *b) {  // device -> host, synchronous
+  // device -> host, synchronous, on a stream of its own; ordered after the
+  // round in flight that writes the buffer, if any (not behind every round
+  // queued since)
+  void push_out(BufferBase *b) {
     if (b->dev && b->dev_valid && !b->host_valid && b->host && b->bytes) {
+      if (b->write_round > completed_round_) {
+        bool found = false;
+        for (auto &r : inflight_)
+          if (r.id == b->write_round) {
+            found = true;
+            if (r.done) hip_check(hipStreamWaitEvent(copy_, r.done, 0), "hipStreamWaitEvent");
+          }
+        // the round being formed writes it: its kernels are not even queued
+        if (!found) fatal("host access to a buffer that a device task of the round being formed writes");
+      }
       hip_check(hipMemcpyAsync(b->host, b->dev, b->bytes, hipMemcpyDeviceToHost, copy_), "D2H");
       hip_check(hipStreamSynchronize(copy_), "hipStreamSynchronize");
+      ++push_outs;
     }
     b->host_valid = true;
   }
@@ -393,8 +501,10 @@ class Runtime {
   struct Round {
     std::vector<std::pair<TaskBase *, std::coroutine_handle<>>> tasks;
     hipEvent_t done = nullptr;
+    uint64_t id = 0;
   };
   void complete(Round &r);
+  void drop_ready();
   void run_device_round(std::vector<TaskBase *> &batch);
 
   bool inited_ = false;
@@ -403,7 +513,15 @@ class Runtime {
   // push-outs to host tasks: their data comes from rounds that have
   // completed, so the copy need not wait behind the rounds queued since
   hipStream_t copy_ = nullptr;
+  // kernels a device task launches after a wait: its data is complete, so
+  // they need not queue behind the rounds issued since
+  hipStream_t eager_ = nullptr;
   DevicePool pool_;
+  // deferred writers, each under the value it waits for
+  std::unordered_map<const void *, TaskBase *> waiting_;
+  uint64_t round_seq_ = 0, completed_round_ = 0;
+  bool cancelled_ = false;
+  long error_info_ = 0;
   std::priority_queue<TaskBase *, std::vector<TaskBase *>, TaskOrder> ready_;
   std::vector<Round> inflight_;
   uint64_t seq_ = 0;
@@ -411,6 +529,10 @@ class Runtime {
 };
 
 inline BufferBase::~BufferBase() = default;
+
+inline void BufferBase::sync_host() {
+  if (!host_valid && dev && dev_valid && Runtime::get().inited()) Runtime::get().push_out(this);
+}
 
 }  // namespace detail
 
@@ -788,22 +910,59 @@ class TT : public TTBase {
       else
         return static_cast<A>(v);
     }
+    template <std::size_t I>
+    static constexpr bool writes() {
+      return std::is_rvalue_reference_v<std::tuple_element_t<I, std::tuple<In...>>>;
+    }
     void bind_inputs() {
+      if (!tt->defer_writer_) copy_shared_writers();
       inputs.clear();
       for (auto &p : in) inputs.emplace_back(p.get(), p);
     }
-    void run_host() override {
-      bind_inputs();
-      // host bodies see the host copy of every buffer-carrying input
-      pull_host(std::make_index_sequence<NIn>{});
-      if constexpr (!TT::is_device) call(std::make_index_sequence<NIn>{});
+    const void *writer_blocker() const override {
+      if (!tt->defer_writer_) return nullptr;
+      return blocker(std::make_index_sequence<NIn>{});
     }
     template <std::size_t... I>
-    void pull_host(std::index_sequence<I...>) {
+    const void *blocker(std::index_sequence<I...>) const {
+      const void *v = nullptr;
+      ((v = v ? v : (writes<I>() && in[I].use_count() > 1 ? in[I].get() : nullptr)), ...);
+      return v;
+    }
+    void copy_shared_writers() override { copy_writers(std::make_index_sequence<NIn>{}); }
+    template <std::size_t... I>
+    void copy_writers(std::index_sequence<I...>) {
       (
           [&] {
-            using V = std::tuple_element_t<I, values_t>;
-            for (auto *b : ttg_buffers(*std::static_pointer_cast<V>(in[I]))) detail::Runtime::get().push_out(b);
+            if constexpr (writes<I>()) {
+              using V = std::tuple_element_t<I, values_t>;
+              if (in[I].use_count() > 1) {
+                if constexpr (std::is_copy_constructible_v<V>)
+                  in[I] = std::make_shared<V>(*std::static_pointer_cast<V>(in[I]));
+                else
+                  detail::fatal(tt->name_ + ": a writer of a shared value that cannot be copied");
+              }
+            }
+          }(),
+          ...);
+    }
+    void run_host() override {
+      bind_inputs();
+      if constexpr (!TT::is_device) call(std::make_index_sequence<NIn>{});
+    }
+    // a host writer that touched a buffer's host copy (host_ptr) leaves that
+    // copy as the current one
+    void after_host_run() override { invalidate_dev(std::make_index_sequence<NIn>{}); }
+    template <std::size_t... I>
+    void invalidate_dev(std::index_sequence<I...>) {
+      (
+          [&] {
+            if constexpr (writes<I>()) {
+              using V = std::tuple_element_t<I, values_t>;
+              if (in[I])
+                for (auto *b : ttg_buffers(*std::static_pointer_cast<V>(in[I])))
+                  if (b->host_valid) b->dev_valid = false;
+            }
           }(),
           ...);
     }
@@ -816,9 +975,20 @@ class TT : public TTBase {
         return {};
       }
     }
+    // release the inputs; a value now held by one other task may let a
+    // writer waiting for it run
     void finish() override {
       inputs.clear();
-      for (auto &p : in) p.reset();
+      auto &rt = detail::Runtime::get();
+      for (auto &p : in) {
+        if (p && p.use_count() == 2 && rt.has_waiting()) {
+          const void *v = p.get();
+          p.reset();
+          rt.released(v);
+        } else {
+          p.reset();
+        }
+      }
     }
   };
 
@@ -894,7 +1064,7 @@ class TT : public TTBase {
   std::tuple<detail::InTerminal<Key, std::decay_t<In>>...> termstore_;
   std::unordered_map<Key, std::unique_ptr<Inst>, detail::KeyHash<Key>> pending_;
   std::function<int(const Key &)> prio_;
-  bool defer_writer_ = true;
+  bool defer_writer_ = false;  // the reference's default (TTG_PARSEC_DEFER_WRITER, parsec/ttg.h:13)
 };
 
 // TT without a key and without inputs (started by invoke()).
@@ -1069,7 +1239,16 @@ inline void initialize(int, char **, int num_threads = -1) { detail::Runtime::ge
 inline void finalize() { detail::Runtime::get().fini(); }
 inline void execute(World = {}) {}
 inline void fence(World = {}) { detail::Runtime::get().fence(); }
+// ends the process, as the reference's ttg::abort (run.h:62)
 [[noreturn]] inline void abort() { std::abort(); }
+// Stop the running graph cleanly with `info` (for instance LAPACK's info of
+// a failed factorization): no further task starts, fence() returns, and
+// ttg::cancelled() / ttg::cancel_info() report it until the next
+// ttg::clear_cancel().
+inline void cancel(long info) { detail::Runtime::get().cancel(info); }
+inline bool cancelled() { return detail::Runtime::get().cancelled(); }
+inline long cancel_info() { return detail::Runtime::get().error_info(); }
+inline void clear_cancel() { detail::Runtime::get().clear_cancel(); }
 
 // ---------------------------------------------------------------------------
 // the scheduler loop
@@ -1079,6 +1258,7 @@ namespace detail {
 inline void Runtime::run_device_round(std::vector<TaskBase *> &batch) {
   using P = device::Task::promise_type;
   Round r;
+  r.id = ++round_seq_;
   device::detail::current_stream_v = stream_;
   device::detail::current_device_v = 0;
   // phase 1: every task runs to its select; buffers made resident
@@ -1096,6 +1276,7 @@ inline void Runtime::run_device_round(std::vector<TaskBase *> &batch) {
         if (!u.readonly) {
           u.buf->dev_valid = true;
           u.buf->host_valid = false;
+          u.buf->write_round = r.id;
         }
       }
     else if (p.state != device::State::Done)
@@ -1141,7 +1322,9 @@ inline void Runtime::complete(Round &r) {
   if (r.done) {
     hip_check(hipEventSynchronize(r.done), "hipEventSynchronize");
     events_.push_back(r.done);
+    r.done = nullptr;
   }
+  completed_round_ = r.id;
   for (auto &[t, hv] : r.tasks) {
     auto h = std::coroutine_handle<P>::from_address(hv.address());
     P &p = h.promise();
@@ -1150,12 +1333,17 @@ inline void Runtime::complete(Round &r) {
     while (p.state != device::State::Done) {
       const bool after_wait = p.state == device::State::Wait;
       if (after_wait) {
+        // the task's kernels are done (its round is): copy back on the copy
+        // stream, without waiting for the rounds issued since
+        bool copied = false;
         for (auto *b : p.wt.bufs) {
-          if (b->dev && b->host && b->bytes)
-            hip_check(hipMemcpyAsync(b->host, b->dev, b->bytes, hipMemcpyDeviceToHost, stream_), "D2H");
+          if (b->dev && b->host && b->bytes) {
+            hip_check(hipMemcpyAsync(b->host, b->dev, b->bytes, hipMemcpyDeviceToHost, copy_), "D2H");
+            copied = true;
+          }
           b->host_valid = true;
         }
-        hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+        if (copied) hip_check(hipStreamSynchronize(copy_), "hipStreamSynchronize");
       } else if (p.state == device::State::SendOut) {
         current_ = t;
         for (auto &s : p.sends) s();
@@ -1166,11 +1354,13 @@ inline void Runtime::complete(Round &r) {
       }
       p.state = device::State::Init;
       current_ = t;
-      // kernels issued after a wait run eagerly (not grouped) and complete
-      // before the task's next sends
+      // kernels issued after a wait run eagerly (not grouped) on a stream of
+      // their own and complete before the task's next sends
+      if (after_wait) device::detail::current_stream_v = eager_;
       h.resume();
+      device::detail::current_stream_v = stream_;
       current_ = nullptr;
-      if (after_wait) hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+      if (after_wait) hip_check(hipStreamSynchronize(eager_), "hipStreamSynchronize");
     }
     h.destroy();
     t->finish();
@@ -1179,27 +1369,52 @@ inline void Runtime::complete(Round &r) {
   }
 }
 
+inline void Runtime::drop_ready() {
+  while (!ready_.empty()) {
+    TaskBase *t = ready_.top();
+    ready_.pop();
+    t->finish();
+    delete t;
+  }
+  std::vector<TaskBase *> w;
+  for (auto &kv : waiting_) w.push_back(kv.second);
+  waiting_.clear();
+  for (TaskBase *t : w) {
+    t->finish();
+    delete t;
+  }
+}
+
 inline void Runtime::fence() {
   for (;;) {
+    if (cancelled_) drop_ready();
     // host tasks first (they feed the device rounds), highest priority first
     std::vector<TaskBase *> dev_batch;
     const int limit = inflight_.empty() ? min_batch
                                         : std::min(max_batch, std::max(min_batch, last_batch_ + last_batch_ / 2));
-    while (!ready_.empty()) {
+    while (!ready_.empty() && !cancelled_) {
       TaskBase *t = ready_.top();
+      if (t->is_device() && (int)dev_batch.size() >= limit) break;
+      ready_.pop();
+      if (!admit_check(t)) continue;  // a writer waiting for its readers
       if (t->is_device()) {
-        if ((int)dev_batch.size() >= limit) break;
-        ready_.pop();
         dev_batch.push_back(t);
         continue;
       }
-      ready_.pop();
       current_ = t;
       t->run_host();
+      t->after_host_run();
       current_ = nullptr;
       t->finish();
       delete t;
       ++tasks_run;
+    }
+    if (cancelled_) {
+      for (TaskBase *t : dev_batch) {
+        t->finish();
+        delete t;
+      }
+      dev_batch.clear();
     }
     if (!dev_batch.empty()) {
       if ((int)inflight_.size() >= max_inflight) {
@@ -1210,10 +1425,23 @@ inline void Runtime::fence() {
       run_device_round(dev_batch);
       continue;
     }
-    if (inflight_.empty()) break;
+    if (inflight_.empty()) {
+      if (waiting_.empty() || cancelled_) break;
+      // writers still waiting with nothing left to run: the holders of their
+      // values never run, so the writers take copies and go
+      std::vector<TaskBase *> w;
+      for (auto &kv : waiting_) w.push_back(kv.second);
+      waiting_.clear();
+      for (TaskBase *t : w) {
+        t->copy_shared_writers();
+        ready_.push(t);
+      }
+      continue;
+    }
     complete(inflight_.front());
     inflight_.erase(inflight_.begin());
   }
+  if (cancelled_) drop_ready();
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }
 
