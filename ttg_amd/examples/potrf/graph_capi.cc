@@ -20,7 +20,7 @@ extern "C" {
 // argument.
 int ttg_graph_dpotrf(int N, int NB, double *tiles, long *stats, int *info) {
   if (N <= 0 || NB <= 0 || NB > TTG_CHOL_MAX_TILE || !tiles) return -1;
-  ttg::initialize(1, nullptr, 1);
+  ttg::initialize(1, nullptr, -1);  // host workers: TTG_NUM_THREADS, else the runtime's default
   ttg::clear_cancel();
   auto &rt = ttg::detail::Runtime::get();
   const long t0 = rt.tasks_run, r0 = rt.rounds, k0 = rt.kernel_launches;
@@ -51,7 +51,7 @@ int ttg_graph_dpotrf(int N, int NB, double *tiles, long *stats, int *info) {
 // PLGSY graph.
 int ttg_graph_dplgsy(int N, int NB, double bump, unsigned long seed, double *tiles) {
   if (N <= 0 || NB <= 0 || !tiles) return -1;
-  ttg::initialize(1, nullptr, 1);
+  ttg::initialize(1, nullptr, -1);  // host workers: TTG_NUM_THREADS, else the runtime's default
   MatrixT<double> A(N, NB, tiles);
   ttg::Edge<Key2, void> startup("startup");
   ttg::Edge<Key2, MatrixTile<double>> result("To result");

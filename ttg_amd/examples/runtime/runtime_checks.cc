@@ -19,9 +19,15 @@
 //                        first task stops it: nothing after the stop runs,
 //                        fence() returns, the info is kept, and the runtime
 //                        runs the next graph in full.
+//   fanout_host          64 device tasks factor shifted tiles, 64 host tasks
+//                        (on the worker threads when -c > 1) check and edit
+//                        them, 64 device tasks read the edits: every push-out
+//                        and every send recorded on a worker arrives.
+// Usage: runtime_checks [-c threads]   (default 1: host tasks inline)
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <iostream>
 #include <sstream>
 #include <string>
@@ -319,15 +325,80 @@ void cancel_fanout() {
   report("cancel_fanout", why.str());
 }
 
+void fanout_host() {
+  constexpr int n = 64;
+  const std::vector<double> a = spd_values(kN);
+  std::vector<double> err(n, -1.0), edited_norm(n, -1.0), device_norm(n, -1.0);
+  std::vector<int> infos(n, -1);
+  ttg::Edge<int, Tile> e_p("to P"), e_h("to host"), e_r("to R");
+  auto src = ttg::make_tt(
+      [&](std::tuple<ttg::Out<int, Tile>> &out) {
+        for (int k = 0; k < n; ++k) {
+          std::vector<double> ak = a;
+          for (int i = 0; i < kN; ++i) ak[i + (std::size_t)i * kN] += k;
+          ttg::send<0>(k, make_tile(ak, kN), out);
+        }
+      },
+      ttg::edges(), ttg::edges(e_p), "source");
+  auto P = ttg::make_tt<ttg::ExecutionSpace::HIP>(
+      [&](const int &k, Tile &&t, std::tuple<ttg::Out<int, Tile>> &out) -> ttg::device::Task {
+        int info = -1;
+        auto si = ttg::make_scratch(&info, ttg::scope::Allocate);
+        co_await ttg::device::select(t.buffer(), si);
+        check_launch(ttg_chol_dpotrf_tile(ttg::device::current_stream(), (int)t.cols(),
+                                          t.buffer().current_device_ptr(), (int)t.lda(), si.device_ptr(), 0));
+        co_await ttg::device::wait(si);
+        infos[k] = info;
+        co_await ttg::device::send<0>(k, std::move(t), out);
+      },
+      ttg::edges(e_p), ttg::edges(e_h), "P");
+  auto H = ttg::make_tt(
+      [&](const int &k, Tile &&t, std::tuple<ttg::Out<int, Tile>> &out) {
+        std::vector<double> L(t.data(), t.data() + t.size()), ak = a;
+        for (int i = 0; i < kN; ++i) ak[i + (std::size_t)i * kN] += k;
+        err[k] = factor_error(L, ak, kN);
+        t.data()[1] = 100.0 + k;  // (1, 0): part of L
+        edited_norm[k] = host_norm(t.data(), t.size());
+        ttg::send<0>(k, std::move(t), out);
+      },
+      ttg::edges(e_h), ttg::edges(e_r), "host check");
+  auto R = ttg::make_tt<ttg::ExecutionSpace::HIP>(
+      [&](const int &k, const Tile &t, std::tuple<> &) -> ttg::device::Task {
+        double nrm = -1.0;
+        auto sn = ttg::make_scratch(&nrm, ttg::scope::Allocate);
+        co_await ttg::device::select(t.buffer(), sn);
+        check_launch(ttg_chol_dnrm2(ttg::device::current_stream(), (int)t.size(), t.buffer().current_device_ptr(),
+                                    sn.device_ptr()));
+        co_await ttg::device::wait(sn);
+        device_norm[k] = nrm;
+      },
+      ttg::edges(e_r), ttg::edges(), "R");
+  src->invoke();
+  ttg::fence();
+  std::ostringstream why;
+  for (int k = 0; k < n; ++k) {
+    if (infos[k] != 0) why << "P(" << k << ") info " << infos[k] << "; ";
+    if (!(err[k] >= 0.0 && err[k] <= 1e-13)) why << "H(" << k << ") factor error " << err[k] << "; ";
+    if (!close(device_norm[k], edited_norm[k], 1e-12))
+      why << "R(" << k << ") saw norm " << device_norm[k] << ", the host edit gives " << edited_norm[k] << "; ";
+  }
+  report("fanout_host", why.str());
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
-  ttg::initialize(argc, argv, 1);
+  int threads = 1;
+  for (int i = 1; i + 1 < argc; ++i)
+    if (!std::strcmp(argv[i], "-c")) threads = std::atoi(argv[i + 1]);
+  ttg::initialize(argc, argv, threads);
+  std::cout << "threads " << ttg::detail::Runtime::get().num_threads() << std::endl;
   reader_writer(true);
   reader_writer(false);
   host_route();
   cancel_chain();
   cancel_fanout();
+  fanout_host();
   ttg::finalize();
   std::cout << (g_failures ? "runtime checks FAILED" : "runtime checks passed") << std::endl;
   return g_failures ? EXIT_FAILURE : EXIT_SUCCESS;

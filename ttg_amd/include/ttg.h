@@ -29,7 +29,11 @@
 //     device/task.h:205-213, 677-682).  Up to two rounds are in flight, so the
 //     host prepares round r+1 while the GPU runs round r; a round that left
 //     no work on the stream completes at once.
-//   * Host tasks run on the calling thread.  A buffer's host copy is brought
+//   * Host tasks run on a pool of worker threads (ttg::initialize's thread
+//     count minus the scheduling thread; run.h:30-47); a worker's sends are
+//     recorded and delivered by the scheduling thread when the task ends, so
+//     the graph's bookkeeping stays single-threaded.  With one thread, host
+//     tasks run inline.  A buffer's host copy is brought
 //     up to date when host code first asks for it (Buffer::host_ptr), so a
 //     host task that only routes a tile costs no copy; the copy runs on a
 //     stream of its own, after the round that last wrote the buffer (the
@@ -57,8 +61,10 @@
 #include <array>
 #include <atomic>
 #include <cassert>
+#include <condition_variable>
 #include <coroutine>
 #include <cstdint>
+#include <deque>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -66,10 +72,12 @@
 #include <iostream>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <queue>
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <type_traits>
 #include <unordered_map>
@@ -157,6 +165,7 @@ inline void hip_check(hipError_t e, const char *what) {
 class DevicePool {
  public:
   void *get(std::size_t bytes) {
+    std::lock_guard<std::mutex> g(mu_);
     auto it = free_.find(bytes);
     if (it != free_.end() && !it->second.empty()) {
       void *p = it->second.back();
@@ -169,9 +178,11 @@ class DevicePool {
     return p;
   }
   void put(void *p, std::size_t bytes) {
+    std::lock_guard<std::mutex> g(mu_);
     if (p) free_[bytes].push_back(p);
   }
   void release() {
+    std::lock_guard<std::mutex> g(mu_);
     for (auto &kv : free_)
       for (void *p : kv.second) (void)hipFree(p);
     free_.clear();
@@ -181,6 +192,7 @@ class DevicePool {
  private:
   std::map<std::size_t, std::vector<void *>> free_;
   long allocated_ = 0;
+  std::mutex mu_;  // values may die on a host worker
 };
 
 // ---------------------------------------------------------------------------
@@ -368,11 +380,26 @@ class Runtime {
     // TTG_ROUND_MIN / TTG_ROUND_MAX override the round sizes (below)
     if (const char *e = std::getenv("TTG_ROUND_MIN")) min_batch = std::max(1, std::atoi(e));
     if (const char *e = std::getenv("TTG_ROUND_MAX")) max_batch = std::max(1, std::atoi(e));
+    // threads: the scheduling thread plus host workers; -1 takes
+    // TTG_NUM_THREADS, else 5 (host tasks here are short: copies back of
+    // finished tiles, routing), at most the machine's hardware threads
+    int nt = nthreads;
+    if (nt <= 0) {
+      const char *e = std::getenv("TTG_NUM_THREADS");
+      nt = e ? std::max(1, std::atoi(e)) : 5;
+      nt = std::min<int>(nt, std::max(1u, std::thread::hardware_concurrency()));
+    }
+    nthreads_ = nt;
+    worker_streams_.resize(nt - 1);
+    for (auto &ws : worker_streams_)
+      hip_check(hipStreamCreateWithFlags(&ws, hipStreamNonBlocking), "hipStreamCreate");
     inited_ = true;
   }
   void fini() {
     if (!inited_) return;
     fence();
+    for (auto ws : worker_streams_) (void)hipStreamDestroy(ws);
+    worker_streams_.clear();
     pool_.release();
     (void)hipStreamDestroy(stream_);
     (void)hipStreamDestroy(copy_);
@@ -383,6 +410,7 @@ class Runtime {
   hipStream_t stream() const { return stream_; }
   DevicePool &pool() { return pool_; }
   TaskBase *current() const { return current_; }
+  int num_threads() const { return nthreads_; }
 
   void schedule(TaskBase *t) {
     if (cancelled_) {  // a stopped graph runs nothing more
@@ -450,7 +478,8 @@ class Runtime {
   }
 
   // statistics
-  long tasks_run = 0, rounds = 0, kernel_launches = 0, push_outs = 0;
+  long tasks_run = 0, rounds = 0, kernel_launches = 0;
+  std::atomic<long> push_outs{0};
   long max_round = 0;
   int max_inflight = 2;
   // Round sizes: forming a round costs host time per task (~1 us) while the
@@ -462,32 +491,49 @@ class Runtime {
   int max_batch = 1 << 20;
   int last_batch_ = 0;
 
+  // buffer validity flags: the scheduling thread and host workers
+  // (push-outs) both change them
+  std::mutex &flag_mutex() { return flag_mu_; }
+
   void make_resident(BufferBase *b, bool sync_in) {
     if (!b->dev) b->dev = pool_.get(b->bytes);
     if (sync_in && b->host_valid && !b->dev_valid && b->bytes)
       hip_check(hipMemcpyAsync(b->dev, b->host, b->bytes, hipMemcpyHostToDevice, stream_), "H2D");
     if (sync_in) b->dev_valid = true;
   }
-  // device -> host, synchronous, on a stream of its own; ordered after the
-  // round in flight that writes the buffer, if any (not behind every round
-  // queued since)
+  // device -> host, synchronous, on a copy stream of the calling thread;
+  // ordered after the round in flight that writes the buffer, if any (not
+  // behind every round queued since)
   void push_out(BufferBase *b) {
-    if (b->dev && b->dev_valid && !b->host_valid && b->host && b->bytes) {
-      if (b->write_round > completed_round_) {
+    std::unique_lock<std::mutex> lk(flag_mu_);
+    if (!(b->dev && b->dev_valid && !b->host_valid && b->host && b->bytes)) {
+      b->host_valid = true;
+      return;
+    }
+    const uint64_t wr = b->write_round;
+    lk.unlock();
+    hipStream_t cs = worker_copy_ ? worker_copy_ : copy_;
+    if (wr > completed_round_.load()) {
+      if (!worker_copy_) {
         bool found = false;
         for (auto &r : inflight_)
-          if (r.id == b->write_round) {
+          if (r.id == wr) {
             found = true;
-            if (r.done) hip_check(hipStreamWaitEvent(copy_, r.done, 0), "hipStreamWaitEvent");
+            if (r.done) hip_check(hipStreamWaitEvent(cs, r.done, 0), "hipStreamWaitEvent");
           }
         // the round being formed writes it: its kernels are not even queued
         if (!found) fatal("host access to a buffer that a device task of the round being formed writes");
+      } else {
+        // a worker waits for the scheduling thread to complete that round
+        std::unique_lock<std::mutex> rl(round_mu_);
+        round_cv_.wait(rl, [&] { return completed_round_.load() >= wr; });
       }
-      hip_check(hipMemcpyAsync(b->host, b->dev, b->bytes, hipMemcpyDeviceToHost, copy_), "D2H");
-      hip_check(hipStreamSynchronize(copy_), "hipStreamSynchronize");
-      ++push_outs;
     }
+    hip_check(hipMemcpyAsync(b->host, b->dev, b->bytes, hipMemcpyDeviceToHost, cs), "D2H");
+    hip_check(hipStreamSynchronize(cs), "hipStreamSynchronize");
+    lk.lock();
     b->host_valid = true;
+    ++push_outs;
   }
   void release(BufferBase *b) {
     if (b->dev) pool_.put(b->dev, b->bytes);
@@ -495,9 +541,93 @@ class Runtime {
     b->dev_valid = false;
   }
 
-  TaskBase *current_ = nullptr;
+  // the task whose body runs on this thread
+  static inline thread_local TaskBase *current_ = nullptr;
+  // sends of a host task on a worker, delivered by the scheduling thread
+  static inline thread_local std::vector<std::function<void()>> *captured_ = nullptr;
 
  private:
+  static inline thread_local hipStream_t worker_copy_ = nullptr;
+  // The workers live for one fence(): started when it begins and joined
+  // before it returns, so no thread outlives the graph (or the process's
+  // own teardown).
+  void start_workers() {
+    if (!workers_.empty() || worker_streams_.empty()) return;
+    stop_ = false;
+    for (hipStream_t ws : worker_streams_) workers_.emplace_back([this, ws] { worker_main(ws); });
+  }
+  void stop_workers() {
+    if (workers_.empty()) return;
+    {
+      std::lock_guard<std::mutex> g(pool_mu_);
+      stop_ = true;
+    }
+    pool_cv_.notify_all();
+    for (auto &w : workers_) w.join();
+    workers_.clear();
+  }
+  void worker_main(hipStream_t ws) {
+    hip_check(hipSetDevice(0), "hipSetDevice");
+    worker_copy_ = ws;
+    for (;;) {
+      TaskBase *t = nullptr;
+      {
+        std::unique_lock<std::mutex> lk(pool_mu_);
+        pool_cv_.wait(lk, [&] { return stop_ || !host_q_.empty(); });
+        if (host_q_.empty()) break;  // stopping
+        t = host_q_.front();
+        host_q_.pop_front();
+      }
+      std::vector<std::function<void()>> sends;
+      captured_ = &sends;
+      current_ = t;
+      t->run_host();
+      t->after_host_run();
+      current_ = nullptr;
+      captured_ = nullptr;
+      {
+        std::lock_guard<std::mutex> g(pool_mu_);
+        done_q_.emplace_back(t, std::move(sends));
+      }
+      done_cv_.notify_one();
+    }
+    worker_copy_ = nullptr;
+  }
+  // host tasks the workers finished: deliver their sends, release inputs
+  void collect(bool block) {
+    std::vector<std::pair<TaskBase *, std::vector<std::function<void()>>>> done;
+    {
+      std::unique_lock<std::mutex> lk(pool_mu_);
+      if (block) done_cv_.wait(lk, [&] { return !done_q_.empty(); });
+      done.swap(done_q_);
+    }
+    for (auto &[t, sends] : done) {
+      for (auto &f : sends) f();
+      t->finish();
+      delete t;
+      ++tasks_run;
+      --outstanding_;
+    }
+  }
+  void run_host_task(TaskBase *t) {
+    if (workers_.empty()) {
+      current_ = t;
+      t->run_host();
+      t->after_host_run();
+      current_ = nullptr;
+      t->finish();
+      delete t;
+      ++tasks_run;
+      return;
+    }
+    ++outstanding_;
+    {
+      std::lock_guard<std::mutex> g(pool_mu_);
+      host_q_.push_back(t);
+    }
+    pool_cv_.notify_one();
+  }
+
   struct Round {
     std::vector<std::pair<TaskBase *, std::coroutine_handle<>>> tasks;
     hipEvent_t done = nullptr;
@@ -519,7 +649,19 @@ class Runtime {
   DevicePool pool_;
   // deferred writers, each under the value it waits for
   std::unordered_map<const void *, TaskBase *> waiting_;
-  uint64_t round_seq_ = 0, completed_round_ = 0;
+  uint64_t round_seq_ = 0;
+  std::atomic<uint64_t> completed_round_{0};
+  std::mutex flag_mu_, round_mu_;
+  std::condition_variable round_cv_;
+  // host workers
+  std::vector<std::thread> workers_;
+  std::vector<hipStream_t> worker_streams_;  // one copy stream per worker
+  std::mutex pool_mu_;
+  std::condition_variable pool_cv_, done_cv_;
+  std::deque<TaskBase *> host_q_;
+  std::vector<std::pair<TaskBase *, std::vector<std::function<void()>>>> done_q_;
+  long outstanding_ = 0;  // host tasks handed to workers, not yet collected
+  bool stop_ = false;
   bool cancelled_ = false;
   long error_info_ = 0;
   std::priority_queue<TaskBase *, std::vector<TaskBase *>, TaskOrder> ready_;
@@ -723,13 +865,23 @@ class Out {
   Out() = default;
   void connect(std::shared_ptr<detail::EdgeImpl<Key, Value>> e) { edges_.push_back(std::move(e)); }
 
+  // a host task on a worker thread records its sends; the scheduling
+  // thread delivers them when the task ends
   template <typename V = Value, typename = std::enable_if_t<!std::is_void_v<V>>>
   void send_shared(const Key &k, const std::shared_ptr<V> &v) {
+    if (auto *cap = detail::Runtime::captured_) {
+      cap->push_back([this, k, v] { this->send_shared(k, v); });
+      return;
+    }
     for (auto &e : edges_)
       for (auto *c : e->consumers) c->deliver(k, v);
   }
   void sendk(const Key &k) {
     static_assert(std::is_void_v<Value>, "sendk needs a void-valued terminal");
+    if (auto *cap = detail::Runtime::captured_) {
+      cap->push_back([this, k] { this->sendk(k); });
+      return;
+    }
     for (auto &e : edges_)
       for (auto *c : e->consumers) c->deliver(k);
   }
@@ -959,9 +1111,11 @@ class TT : public TTBase {
           [&] {
             if constexpr (writes<I>()) {
               using V = std::tuple_element_t<I, values_t>;
-              if (in[I])
+              if (in[I]) {
+                std::lock_guard<std::mutex> g(detail::Runtime::get().flag_mutex());
                 for (auto *b : ttg_buffers(*std::static_pointer_cast<V>(in[I])))
                   if (b->host_valid) b->dev_valid = false;
+              }
             }
           }(),
           ...);
@@ -1270,6 +1424,7 @@ inline void Runtime::run_device_round(std::vector<TaskBase *> &batch) {
     P &p = h.promise();
     if (p.state == device::State::Select)
       for (auto &u : p.sel.uses) {
+        std::lock_guard<std::mutex> g(flag_mu_);
         make_resident(u.buf, u.sync_in);
         // the HBM copy is the current one from here on: the task's kernels
         // write every buffer it did not select read-only
@@ -1324,7 +1479,11 @@ inline void Runtime::complete(Round &r) {
     events_.push_back(r.done);
     r.done = nullptr;
   }
-  completed_round_ = r.id;
+  {
+    std::lock_guard<std::mutex> g(round_mu_);
+    completed_round_ = r.id;
+  }
+  if (!workers_.empty()) round_cv_.notify_all();
   for (auto &[t, hv] : r.tasks) {
     auto h = std::coroutine_handle<P>::from_address(hv.address());
     P &p = h.promise();
@@ -1386,7 +1545,9 @@ inline void Runtime::drop_ready() {
 }
 
 inline void Runtime::fence() {
+  start_workers();
   for (;;) {
+    if (outstanding_) collect(false);
     if (cancelled_) drop_ready();
     // host tasks first (they feed the device rounds), highest priority first
     std::vector<TaskBase *> dev_batch;
@@ -1401,13 +1562,7 @@ inline void Runtime::fence() {
         dev_batch.push_back(t);
         continue;
       }
-      current_ = t;
-      t->run_host();
-      t->after_host_run();
-      current_ = nullptr;
-      t->finish();
-      delete t;
-      ++tasks_run;
+      run_host_task(t);
     }
     if (cancelled_) {
       for (TaskBase *t : dev_batch) {
@@ -1426,6 +1581,10 @@ inline void Runtime::fence() {
       continue;
     }
     if (inflight_.empty()) {
+      if (outstanding_) {  // host tasks still running on workers
+        collect(true);
+        continue;
+      }
       if (waiting_.empty() || cancelled_) break;
       // writers still waiting with nothing left to run: the holders of their
       // values never run, so the writers take copies and go
@@ -1442,6 +1601,7 @@ inline void Runtime::fence() {
     inflight_.erase(inflight_.begin());
   }
   if (cancelled_) drop_ready();
+  stop_workers();
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }
 
