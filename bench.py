@@ -69,6 +69,8 @@ def parse():
                    help="order of the secondary CPU sample on the reference's PLGSY matrix (0: skip)")
     p.add_argument("--no-check", action="store_true",
                    help="skip the factor backward errors (||A - L L^T|| of the mixed and FP64 factors)")
+    p.add_argument("--no-tune", action="store_true",
+                   help="skip the schedule autotuner (the factorization then runs the library's default knobs)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-profile", action="store_true")
     p.add_argument("--no-fp64", action="store_true", help="skip the plain-FP64 comparison solve")
@@ -496,6 +498,17 @@ def main():
             raise RuntimeError(f"{mode}: solve failed: info={int(info.item())} backward error={berr}")
         return ev0.elapsed_time(ev1) / steps, wall, berr
 
+    # ---- schedule: the autotuner's choice of the factorization knobs for
+    # this workload (coordinate descent from the defaults, before the timed
+    # region; cached per device and workload)
+    schedule = None
+    if not a.no_tune and a.mode != "fp64":
+        import ttg_amd.autotune as T
+
+        default = {k: C.get_knob(k) for k in C.KNOBS}
+        choice, trials = T.tune_schedule(make_step(a.mode), N, NB, prec[a.mode], nrhs, use_cache=False)
+        schedule = {"choice": choice, "default": default, "trials": trials}
+
     ms_step, wall, berr = timed(a.mode, a.steps, a.warmup)
     it_main = iters[a.mode][0]
     checks = {}
@@ -598,6 +611,7 @@ def main():
         "fp64_backward_error": berr_fp64,
         "speedup_vs_fp64": None if ms_fp64 is None else round(ms_fp64 / ms_step, 3),
         "wall_s": round(wall, 3),
+        "schedule": schedule,
         "roofline": roof,
         "cpu_baseline": cpu,
         "graph_driver": graph,

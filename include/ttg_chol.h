@@ -68,6 +68,29 @@ int ttg_chol_num_devices(void);
  * the PaRSEC runtime. */
 int ttg_chol_release_plans(void);
 
+/* Schedule knobs of the planned factorizations, read when a plan is built
+ * (plans are cached per shape: call ttg_chol_release_plans() after a
+ * change).  A value set here wins over the environment variable named in
+ * brackets, which wins over the default:
+ *   GROUP       panels per bulk split-FP16 update, 2..8  [TTG_CHOL_GROUP, 3]
+ *   POTRF_CUS   CUs reserved for the diagonal chain of the low-precision
+ *               plans, 0 (none) or 4..64                 [TTG_CHOL_POTRF_CUS, 8]
+ *   TILE_GROUP  side of the square tile groups the bulk update walks, 1..8
+ *                                                        [TTG_CHOL_TILE_GROUP, 2]
+ *   PAIRED      1: grouped split-FP16 schedule, 0: one panel per update
+ *                                                        [TTG_CHOL_PAIRED, 1]
+ * value < 0 clears the setting.  ttg_chol_get_knob returns the value a plan
+ * built now would use.  No reference counterpart: the reference's schedule
+ * is PaRSEC's, steered by the priorities of examples/potrf/potrf.h:743-752;
+ * these knobs are what the autotuner (ttg_amd/autotune.py) searches. */
+#define TTG_CHOL_KNOB_GROUP 0
+#define TTG_CHOL_KNOB_POTRF_CUS 1
+#define TTG_CHOL_KNOB_TILE_GROUP 2
+#define TTG_CHOL_KNOB_PAIRED 3
+#define TTG_CHOL_KNOB_COUNT 4
+int ttg_chol_set_knob(int knob, int value);
+int ttg_chol_get_knob(int knob);
+
 /* ---------------------------------------------------------------- */
 /* per-tile operations (FP64)                                        */
 /* ---------------------------------------------------------------- */

@@ -66,3 +66,52 @@ def test_grouped_schedule_matches_unpaired(dev, oracle, monkeypatch, G, r1):
     Xu = _solve(C, dA, N, NB, Bh, r1 + 1, C.PREC_F16X2, dev)
     assert np.array_equal(Xp, Xu[:, :r1])
     assert oracle.solve_backward_error(N, NB, A0, Xp, Bh[:, :r1]) <= 1e-12
+
+
+def test_knobs_set_get_and_refuse(dev):
+    import ttg_amd.chol as C
+
+    try:
+        C.set_knob("group", 4)
+        assert C.get_knob("group") == 4
+        C.set_knob("group", None)
+        assert C.get_knob("group") == 3
+        for name, bad in (("group", 1), ("group", 9), ("tile_group", 0), ("paired", 2), ("potrf_cus", 65)):
+            with pytest.raises(C.CholError):
+                C.set_knob(name, bad)
+    finally:
+        for k in C.KNOBS:
+            C.set_knob(k, None)
+        C.release_plans()
+
+
+def test_tuned_schedule_is_bitwise_the_default(dev, oracle, tmp_path, monkeypatch):
+    """The autotuner's search (every candidate runs the same arithmetic in
+    another order of independent work) ends with a choice whose solution is
+    bitwise the default schedule's; the trials record every candidate."""
+    import ttg_amd.autotune as T
+    import ttg_amd.chol as C
+
+    monkeypatch.setenv("TTG_AMD_TUNING", str(tmp_path / "tuning.json"))
+    N, NB, r = 5120, 512, 4
+    sigma2 = sum(k * k for k in range(-127, 128)) / 255.0 / 127.0 ** 2
+    A0 = oracle.spd_tiles(N, NB, 4 * sigma2 / (1e4 - 1), 29)
+    Bh = np.random.default_rng(9).uniform(-1, 1, (N, r))
+    dA = to_dev(A0, dev)
+    try:
+        T.apply_schedule({k: None for k in C.KNOBS})
+        Xd = _solve(C, dA, N, NB, Bh, r, C.PREC_F16X2, dev)
+        dB = torch.from_numpy(np.ascontiguousarray(Bh.T)).to(dev).t()
+        X = torch.zeros(r, N, dtype=torch.float64, device=dev).t()
+
+        def run():
+            C.dposv_mixed(dA, N, NB, dB, X, prec=C.PREC_F16X2, tol=1e-300, max_iter=2, check=False)
+
+        choice, trials = T.tune_schedule(run, N, NB, C.PREC_F16X2, r, reps=1, warmup=1, use_cache=False)
+        print("choice", choice, "trials", trials, flush=True)
+        assert set(choice) == set(C.KNOBS) and len(trials) >= 5
+        assert C.knobs() == choice
+        Xt = _solve(C, dA, N, NB, Bh, r, C.PREC_F16X2, dev)
+        assert np.array_equal(Xt, Xd)
+    finally:
+        T.apply_schedule({k: None for k in C.KNOBS})

@@ -1,12 +1,16 @@
 """Autotuner for the launch parameters of libttg_chol.
 
-Two tunables, both measured on the GPU that will run the work and cached per
-device and workload signature:
+Three tunables, all measured on the GPU that will run the work and cached
+per device and workload signature:
 
 * the batched solver's launch configuration (waves per workgroup and
   persistent workgroups per CU, `ttg_chol_batched_config`) for a batch's
   size distribution;
-* the tile size NB of the tiled mixed-precision solve for an order N.
+* the tile size NB of the tiled mixed-precision solve for an order N;
+* the factorization's schedule knobs (`ttg_chol_set_knob`): panels per
+  bulk update, CUs reserved for the diagonal chain, the update's tile-group
+  side and the grouped/one-panel schedule, searched by coordinate descent
+  from the defaults.
 
 Each candidate runs `warmup` untimed and `reps` timed times (HIP events on the
 calling stream); the median wins.  Results go to a JSON cache
@@ -157,3 +161,66 @@ def tune_tiled_nb(N, candidates=(256, 512), prec=None, nrhs=16, reps=1, warmup=1
     best = min(ms, key=ms.get)
     _store(key, {"NB": best, "ms": ms})
     return best, ms
+
+
+# candidate values per schedule knob, searched in this order
+SCHEDULE_SPACE = (("paired", (1, 0)), ("group", (3, 2, 4)), ("potrf_cus", (8, 0, 16)), ("tile_group", (2, 1, 4)))
+
+
+def schedule_signature(N, NB, prec, nrhs):
+    return f"schedule:N={N}:NB={NB}:prec={prec}:nrhs={nrhs}"
+
+
+def apply_schedule(choice):
+    """Set the knobs of `choice` (a dict knob -> value; None clears) and drop
+    the cached plans so that the next solve builds its plan with them."""
+    import ttg_amd.chol as C
+
+    for k, v in choice.items():
+        C.set_knob(k, v)
+    C.release_plans()
+
+
+def tune_schedule(run, N, NB, prec, nrhs, reps=2, warmup=1, min_gain=0.01, use_cache=True):
+    """Schedule knobs for the planned solve `run()` (one call = one solve on
+    the caller's resident data, of order N, tile NB, precision prec, nrhs
+    right-hand sides).  Coordinate descent from the library's defaults: each
+    knob in turn takes the candidate with the lowest median time, provided it
+    beats the current best by `min_gain` (relative), so a noisy win never
+    displaces the default.  The knobs are left set to the choice.  Returns
+    (choice, trials) with trials = [{"knobs": ..., "ms": ...}, ...]."""
+    import ttg_amd.chol as C
+
+    key = f"{_device_name()}|{schedule_signature(N, NB, prec, nrhs)}"
+    if use_cache:
+        hit = _load().get(key)
+        if hit is not None:
+            apply_schedule(hit["choice"])
+            return dict(hit["choice"]), hit["trials"]
+    apply_schedule({k: None for k in C.KNOBS})
+    current = C.knobs()
+    trials = []
+    seen = {}
+
+    def measure(knobs):
+        sig = tuple(sorted(knobs.items()))
+        if sig not in seen:
+            apply_schedule(knobs)
+            seen[sig] = _time_events(run, warmup, reps)
+            trials.append({"knobs": dict(knobs), "ms": round(seen[sig], 3)})
+        return seen[sig]
+
+    best_ms = measure(current)
+    for name, values in SCHEDULE_SPACE:
+        if name == "group" and not current["paired"]:
+            continue  # one panel per update: the group size is unused
+        for v in values:
+            if v == current[name]:
+                continue
+            cand = dict(current, **{name: v})
+            ms = measure(cand)
+            if ms < best_ms * (1.0 - min_gain):
+                current, best_ms = cand, ms
+    apply_schedule(current)
+    _store(key, {"choice": current, "trials": trials})
+    return current, trials

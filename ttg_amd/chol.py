@@ -39,6 +39,8 @@ SIGNATURES = {
     "ttg_chol_dresidual_tiled": (_i, [_p, _i, _i, _p, _i, _p, _i, _p, _i, _p, _i]),
     "ttg_chol_dtrtri_tiled": (_i, [_p, _i, _i, _p, _p]),
     "ttg_chol_release_plans": (_i, []),
+    "ttg_chol_set_knob": (_i, [_i, _i]),
+    "ttg_chol_get_knob": (_i, [_i]),
     "ttg_chol_dlauum_tiled": (_i, [_p, _i, _i, _p]),
     "ttg_chol_dpotri_tiled": (_i, [_p, _i, _i, _p, _p]),
     "ttg_chol_plgsy_tiled": (_i, [_p, _i, _i, _d, _ull, _p]),
@@ -392,6 +394,28 @@ def release_plans():
     """Free every cached plan and workspace of the library (synchronises the
     device); the next call on a matrix builds its plan again."""
     _check(lib().ttg_chol_release_plans(), "release_plans")
+
+
+# schedule knobs of the planned factorizations (include/ttg_chol.h)
+KNOBS = {"group": 0, "potrf_cus": 1, "tile_group": 2, "paired": 3}
+
+
+def set_knob(name, value):
+    """Set a schedule knob (None clears it); plans built afterwards use it,
+    so the caller releases the cached plans (release_plans) after a change."""
+    _check(lib().ttg_chol_set_knob(KNOBS[name], -1 if value is None else int(value)), f"set_knob({name})")
+
+
+def get_knob(name):
+    """The value of a schedule knob that a plan built now would use."""
+    v = lib().ttg_chol_get_knob(KNOBS[name])
+    if v < 0:
+        _check(v, f"get_knob({name})")
+    return v
+
+
+def knobs():
+    return {k: get_knob(k) for k in KNOBS}
 
 
 def dtrtri_tiled(tiles, N, NB, info=None, stream=None):

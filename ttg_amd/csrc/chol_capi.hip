@@ -166,6 +166,21 @@ bool group_record_plgsy(hipStream_t s, const PlgsyOp &op) {
 
 using namespace ttgc;
 
+namespace {
+// process-wide schedule knobs; -1: not set (environment or default)
+int g_knobs[TTG_CHOL_KNOB_COUNT] = {-1, -1, -1, -1};
+std::mutex g_knob_mu;
+}  // namespace
+
+int ttgc::plan_knob(int which, const char *env, int def) {
+  {
+    std::lock_guard<std::mutex> g(g_knob_mu);
+    if (which >= 0 && which < TTG_CHOL_KNOB_COUNT && g_knobs[which] >= 0) return g_knobs[which];
+  }
+  if (const char *e = env ? getenv(env) : nullptr) return atoi(e);
+  return def;
+}
+
 extern "C" {
 
 const char *ttg_chol_version(void) { return "ttg_chol 0.1 (gfx950)"; }
@@ -176,6 +191,29 @@ int ttg_chol_num_devices(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
   return n;
+}
+
+int ttg_chol_set_knob(int knob, int value) {
+  clear_error();
+  static const int lo[TTG_CHOL_KNOB_COUNT] = {2, 0, 1, 0}, hi[TTG_CHOL_KNOB_COUNT] = {8, 64, 8, 1};
+  if (knob < 0 || knob >= TTG_CHOL_KNOB_COUNT || (value >= 0 && (value < lo[knob] || value > hi[knob]))) {
+    set_error("set_knob: knob %d cannot take %d", knob, value);
+    return TTG_CHOL_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(g_knob_mu);
+  g_knobs[knob] = value < 0 ? -1 : value;
+  return TTG_CHOL_OK;
+}
+
+int ttg_chol_get_knob(int knob) {
+  clear_error();
+  switch (knob) {
+    case TTG_CHOL_KNOB_GROUP: return plan_knob(knob, "TTG_CHOL_GROUP", 3);
+    case TTG_CHOL_KNOB_POTRF_CUS: return plan_knob(knob, "TTG_CHOL_POTRF_CUS", 8);
+    case TTG_CHOL_KNOB_TILE_GROUP: return plan_knob(knob, "TTG_CHOL_TILE_GROUP", 2);
+    case TTG_CHOL_KNOB_PAIRED: return plan_knob(knob, "TTG_CHOL_PAIRED", 1);
+    default: set_error("get_knob: no knob %d", knob); return TTG_CHOL_EINVAL;
+  }
 }
 
 int ttg_chol_release_plans(void) {

@@ -29,6 +29,10 @@ inline int hip_fail(hipError_t e, const char *what) {
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Schedule knob `which` (TTG_CHOL_KNOB_*) for a plan being built: the value
+// ttg_chol_set_knob gave, else the environment variable `env`, else `def`.
+int plan_knob(int which, const char *env, int def);
+
 // Two streams whose HSA queues split the CUs of the current device: `few` may
 // only use CUs 0 .. want-1 of the device mask, `rest` the others (KFD deals
 // the mask bits round-robin over the XCDs, so want = 8 is one CU per XCD).

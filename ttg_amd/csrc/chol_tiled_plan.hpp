@@ -106,7 +106,7 @@ struct TiledPlanT {
 
   int build() {
     NT = (N + NB - 1) / NB;
-    if (const char *et = getenv("TTG_CHOL_TILE_GROUP")) TG = std::max(1, std::min(8, atoi(et)));
+    TG = std::max(1, std::min(8, plan_knob(TTG_CHOL_KNOB_TILE_GROUP, "TTG_CHOL_TILE_GROUP", TG)));
     const int dstride = ((NB + 31) / 32) * 32 * 32;
     TTGC_HIP(d_dinv.reserve(sizeof(T) * dstride));
     TTGC_HIP(d_info.reserve(sizeof(int)));
@@ -145,13 +145,11 @@ struct TiledPlanT {
     if (!upd.empty()) memcpy(host.data() + o_upd, upd.data(), upd.size() * sizeof(UpdOp<T>));
     TTGC_HIP(hipMemcpy(d_ops.ptr, host.data(), total, hipMemcpyHostToDevice));
     if (lowp) {
-      const char *e = getenv("TTG_CHOL_PAIRED");
-      const char *eg = getenv("TTG_CHOL_GROUP");
-      // groups of 3 steps: with the 8-wave POTRF the panel chain is short
-      // enough that a third panel per bulk update pays (measured 431 -> 425 ms
-      // per solve at N=65536; groups of 4 were no faster than 3)
-      G = eg ? std::max(2, std::min(8, atoi(eg))) : 3;
-      paired = (flags & TTG_CHOL_F_LOOKAHEAD) && NT > G && !(e && atoi(e) == 0);
+      // groups of 3 steps by default: with the 8-wave POTRF the panel chain
+      // is short enough that a third panel per bulk update pays (measured
+      // 431 -> 425 ms per solve at N=65536; groups of 4 were no faster)
+      G = std::max(2, std::min(8, plan_knob(TTG_CHOL_KNOB_GROUP, "TTG_CHOL_GROUP", 3)));
+      paired = (flags & TTG_CHOL_F_LOOKAHEAD) && NT > G && plan_knob(TTG_CHOL_KNOB_PAIRED, "TTG_CHOL_PAIRED", 1) != 0;
       int rc = paired ? build_f16_paired() : build_f16(upd);
       if (rc) return rc;
     }
@@ -166,8 +164,7 @@ struct TiledPlanT {
   // POTRF stream on CUs 0 .. R-1 of the mask, update stream on the rest; on
   // any failure both stay unset and the plain two-stream schedule is used.
   void reserve_cus() {
-    const char *e = getenv("TTG_CHOL_POTRF_CUS");
-    const int want = e ? atoi(e) : (sizeof(T) == 4 ? 8 : 0);
+    const int want = plan_knob(TTG_CHOL_KNOB_POTRF_CUS, "TTG_CHOL_POTRF_CUS", sizeof(T) == 4 ? 8 : 0);
     hipStream_t d = nullptr, m = nullptr;
     if (make_split_streams(want, &d, &m)) {
       s_diag = d;
