@@ -143,6 +143,57 @@ int ttg_chol_dgemm_nt(void *stream, int m, int n, int k, double alpha, const dou
 int ttg_chol_dnrm2(void *stream, int n, const double *x, double *d_result);
 
 /* ---------------------------------------------------------------- */
+/* per-tile operations of the POINV graphs (FP64)                    */
+/* ---------------------------------------------------------------- */
+/* The tile operations the reference's TRTRI and LAUUM graphs call on the
+ * host through LAPACK++/BLAS++ (examples/potrf/trtri_L.h, lauum.h), as
+ * stream-ordered device calls.  Same conventions as above: column-major,
+ * device pointers, tile orders <= TTG_CHOL_MAX_TILE for the triangular
+ * operands, no host synchronization, launch-groupable; the work tiles the
+ * triangular ones need come from the stream's (or the group's) workspace. */
+
+/* A := A^{-1} for the lower triangle of the n x n tile A (non-unit); the
+ * strict upper triangle is not referenced.  *d_info receives 0 or
+ * info_offset + the 1-based index of the first exact zero on the diagonal
+ * (the inverse is then not meaningful).  Replaces lapack::trtri(Lower,
+ * NonUnit, ...) (examples/potrf/trtri_L.h:32). */
+int ttg_chol_dtrtri_tile(void *stream, int n, double *A, int lda, int *d_info, int info_offset);
+
+/* B := alpha L^{-1} B, L m x m lower non-unit, B m x n.  Replaces
+ * blas::trsm(Left, Lower, NoTrans, NonUnit, m, n, alpha, L, B)
+ * (examples/potrf/trtri_L.h:64-71; the graph uses alpha = 1). */
+int ttg_chol_dtrsm_lln(void *stream, int m, int n, double alpha, const double *L, int ldl, double *B, int ldb);
+
+/* B := alpha B L^{-1}, L n x n lower non-unit, B m x n.  Replaces
+ * blas::trsm(Right, Lower, NoTrans, NonUnit, m, n, alpha, L, B)
+ * (examples/potrf/trtri_L.h:109-116; the graph uses alpha = -1). */
+int ttg_chol_dtrsm_rln(void *stream, int m, int n, double alpha, const double *L, int ldl, double *B, int ldb);
+
+/* C := alpha A B + beta C, A m x k, B k x n.  Replaces
+ * blas::gemm(NoTrans, NoTrans, ...) (examples/potrf/trtri_L.h:179-185). */
+int ttg_chol_dgemm_nn(void *stream, int m, int n, int k, double alpha, const double *A, int lda, const double *B,
+                      int ldb, double beta, double *C, int ldc);
+
+/* Lower triangle of the n x n tile A := L^T L, L its lower triangle; the
+ * strict upper triangle is neither read nor written.  Replaces
+ * lapack::lauum(Lower, ...) (examples/potrf/lauum.h:36). */
+int ttg_chol_dlauum_tile(void *stream, int n, double *A, int lda);
+
+/* B := alpha L^T B, L m x m lower non-unit, B m x n.  Replaces
+ * blas::trmm(Left, Lower, Trans, NonUnit, ...) (examples/potrf/lauum.h:71-77). */
+int ttg_chol_dtrmm_llt(void *stream, int m, int n, double alpha, const double *L, int ldl, double *B, int ldb);
+
+/* C := alpha A^T A + beta C on the lower triangle of the n x n tile C, A
+ * k x n.  Replaces blas::syrk(Lower, Trans, ...) (examples/potrf/lauum.h:116-121). */
+int ttg_chol_dsyrk_lt(void *stream, int n, int k, double alpha, const double *A, int lda, double beta, double *C,
+                      int ldc);
+
+/* C := alpha A^T B + beta C, A k x m, B k x n.  Replaces
+ * blas::gemm(Trans, NoTrans, ...) (examples/potrf/lauum.h:160-166). */
+int ttg_chol_dgemm_tn(void *stream, int m, int n, int k, double alpha, const double *A, int lda, const double *B,
+                      int ldb, double beta, double *C, int ldc);
+
+/* ---------------------------------------------------------------- */
 /* launch grouping                                                   */
 /* ---------------------------------------------------------------- */
 

@@ -208,6 +208,52 @@ def priority(kind, nt, k0, k1=0, k2=0):
 POTRF, TRSM, SYRK, GEMM = 0, 1, 2, 3
 
 
+# ---- per-tile POINV operations (column-major numpy tiles, in place)
+
+
+def _fc(a):
+    if not (a.dtype == np.float64 and a.flags.f_contiguous):
+        raise TypeError("tiles must be Fortran-ordered float64 arrays")
+    return a
+
+
+def tile_trsm_rln(L, B, alpha=-1.0):
+    """B := alpha B L^{-1} (trtri_L.h:109-116)."""
+    lib().oracle_dtrsm_rln(B.shape[0], B.shape[1], alpha, ptr(_fc(L)), L.shape[0], ptr(_fc(B)), B.shape[0])
+
+
+def tile_trsm_lln(L, B):
+    """B := L^{-1} B (trtri_L.h:64-71, alpha = 1)."""
+    lib().oracle_dtrsm_lln(B.shape[0], B.shape[1], ptr(_fc(L)), L.shape[0], ptr(_fc(B)), B.shape[0])
+
+
+def tile_trmm_llt(L, B):
+    """B := L^T B (lauum.h:71-77, alpha = 1)."""
+    lib().oracle_dtrmm_llt(B.shape[0], B.shape[1], ptr(_fc(L)), L.shape[0], ptr(_fc(B)), B.shape[0])
+
+
+def tile_syrk_lt(A, C, alpha=1.0, beta=1.0):
+    """lower(C) := alpha A^T A + beta C, A k x n (lauum.h:116-121)."""
+    lib().oracle_dsyrk_lt(C.shape[0], A.shape[0], alpha, ptr(_fc(A)), A.shape[0], beta, ptr(_fc(C)), C.shape[0])
+
+
+def tile_gemm_xn(ta, A, B, C, alpha=1.0, beta=1.0):
+    """C := alpha op(A) B + beta C, op(A) = A^T if ta (trtri_L.h:179-185, lauum.h:160-166)."""
+    k = A.shape[0] if ta else A.shape[1]
+    lib().oracle_dgemm_xn(int(ta), C.shape[0], C.shape[1], k, alpha, ptr(_fc(A)), A.shape[0], ptr(_fc(B)),
+                          B.shape[0], beta, ptr(_fc(C)), C.shape[0])
+
+
+def tile_trtri(A):
+    """lower(A) := lower(A)^{-1}; returns LAPACK info (trtri_L.h:32)."""
+    return lib().oracle_dtrtri_tile(A.shape[0], ptr(_fc(A)), A.shape[0])
+
+
+def tile_lauum(A):
+    """lower(A) := L^T L (lauum.h:36)."""
+    lib().oracle_dlauum_tile(A.shape[0], ptr(_fc(A)), A.shape[0])
+
+
 def trtri_tiled(N, NB, tiles):
     """In place L := L^{-1} over lower tiles (examples/potrf/trtri_L.h order)."""
     return lib().oracle_dtrtri_tiled(N, NB, ptr(tiles))

@@ -40,6 +40,14 @@ SIGNATURES = {
     "ttg_chol_dtrtri_tiled": (_i, [_p, _i, _i, _p, _p]),
     "ttg_chol_release_plans": (_i, []),
     "ttg_chol_set_knob": (_i, [_i, _i]),
+    "ttg_chol_dtrtri_tile": (_i, [_p, _i, _p, _i, _p, _i]),
+    "ttg_chol_dtrsm_lln": (_i, [_p, _i, _i, _d, _p, _i, _p, _i]),
+    "ttg_chol_dtrsm_rln": (_i, [_p, _i, _i, _d, _p, _i, _p, _i]),
+    "ttg_chol_dgemm_nn": (_i, [_p, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p, _i]),
+    "ttg_chol_dlauum_tile": (_i, [_p, _i, _p, _i]),
+    "ttg_chol_dtrmm_llt": (_i, [_p, _i, _i, _d, _p, _i, _p, _i]),
+    "ttg_chol_dsyrk_lt": (_i, [_p, _i, _i, _d, _p, _i, _d, _p, _i]),
+    "ttg_chol_dgemm_tn": (_i, [_p, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p, _i]),
     "ttg_chol_get_knob": (_i, [_i]),
     "ttg_chol_dlauum_tiled": (_i, [_p, _i, _i, _p]),
     "ttg_chol_dpotri_tiled": (_i, [_p, _i, _i, _p, _p]),
@@ -261,6 +269,85 @@ def dgemm_nt(A, B, C, alpha=-1.0, beta=1.0, stream=None):
     _require_cuda(A, B, C)
     _check(lib().ttg_chol_dgemm_nt(_stream(stream), m, n, k, alpha, _ptr(A), _ld(A), _ptr(B), _ld(B), beta,
                                    _ptr(C), _ld(C)), "dgemm_nt")
+
+
+# ---- tile operations of the POINV graphs (examples/potrf/trtri_L.h, lauum.h)
+
+
+def dtrtri_tile(A, info=None, info_offset=0, stream=None):
+    """A := A^{-1} on its lower triangle (non-unit)."""
+    n = A.shape[0] if A.dim() == 2 else -1
+    _check_tile(A, "A", n, n)
+    _check_info(info)
+    _require_cuda(A, info)
+    _check(lib().ttg_chol_dtrtri_tile(_stream(stream), n, _ptr(A), _ld(A), _ptr(info), info_offset),
+           "dtrtri_tile")
+
+
+def _trsm_like(fn, name, L, B, alpha, left, stream):
+    _check_tile(B, "B")
+    m, n = B.shape
+    order = m if left else n
+    _check_tile(L, "L", order, order)
+    _require_cuda(L, B)
+    _check(getattr(lib(), fn)(_stream(stream), m, n, alpha, _ptr(L), _ld(L), _ptr(B), _ld(B)), name)
+
+
+def dtrsm_lln(L, B, alpha=1.0, stream=None):
+    """B := alpha L^{-1} B (L lower, non-unit)."""
+    _trsm_like("ttg_chol_dtrsm_lln", "dtrsm_lln", L, B, alpha, True, stream)
+
+
+def dtrsm_rln(L, B, alpha=-1.0, stream=None):
+    """B := alpha B L^{-1} (L lower, non-unit)."""
+    _trsm_like("ttg_chol_dtrsm_rln", "dtrsm_rln", L, B, alpha, False, stream)
+
+
+def dtrmm_llt(L, B, alpha=1.0, stream=None):
+    """B := alpha L^T B (L lower, non-unit)."""
+    _trsm_like("ttg_chol_dtrmm_llt", "dtrmm_llt", L, B, alpha, True, stream)
+
+
+def dlauum_tile(A, stream=None):
+    """lower(A) := L^T L, L = lower(A)."""
+    n = A.shape[0] if A.dim() == 2 else -1
+    _check_tile(A, "A", n, n)
+    _require_cuda(A)
+    _check(lib().ttg_chol_dlauum_tile(_stream(stream), n, _ptr(A), _ld(A)), "dlauum_tile")
+
+
+def dgemm_nn(A, B, C, alpha=1.0, beta=1.0, stream=None):
+    """C := alpha A B + beta C."""
+    _check_tile(A, "A")
+    m, k = A.shape
+    _check_tile(B, "B", k, None)
+    n = B.shape[1]
+    _check_tile(C, "C", m, n)
+    _require_cuda(A, B, C)
+    _check(lib().ttg_chol_dgemm_nn(_stream(stream), m, n, k, alpha, _ptr(A), _ld(A), _ptr(B), _ld(B), beta,
+                                   _ptr(C), _ld(C)), "dgemm_nn")
+
+
+def dgemm_tn(A, B, C, alpha=1.0, beta=1.0, stream=None):
+    """C := alpha A^T B + beta C (A k x m)."""
+    _check_tile(A, "A")
+    k, m = A.shape
+    _check_tile(B, "B", k, None)
+    n = B.shape[1]
+    _check_tile(C, "C", m, n)
+    _require_cuda(A, B, C)
+    _check(lib().ttg_chol_dgemm_tn(_stream(stream), m, n, k, alpha, _ptr(A), _ld(A), _ptr(B), _ld(B), beta,
+                                   _ptr(C), _ld(C)), "dgemm_tn")
+
+
+def dsyrk_lt(A, C, alpha=1.0, beta=1.0, stream=None):
+    """C := alpha A^T A + beta C on the lower triangle (A k x n)."""
+    _check_tile(A, "A")
+    k, n = A.shape
+    _check_tile(C, "C", n, n)
+    _require_cuda(A, C)
+    _check(lib().ttg_chol_dsyrk_lt(_stream(stream), n, k, alpha, _ptr(A), _ld(A), beta, _ptr(C), _ld(C)),
+           "dsyrk_lt")
 
 
 def dnrm2(x, out, stream=None):

@@ -12,6 +12,7 @@
 
 #include "chol_internal.hpp"
 #include "chol_kernels.hpp"
+#include "chol_tileops.hpp"
 
 namespace ttgc {
 
@@ -114,23 +115,76 @@ static int scratch_ptr(int dev, double **out) {
 // ---------------------------------------------------------------------------
 struct UpdKey {
   double alpha, beta;
-  bool operator<(const UpdKey &o) const { return std::tie(alpha, beta) < std::tie(o.alpha, o.beta); }
+  bool ta, tb;
+  bool operator<(const UpdKey &o) const { return std::tie(alpha, beta, ta, tb) < std::tie(o.alpha, o.beta, o.ta, o.tb); }
 };
+
+// Pointers of a recorded op that point into the group's work area hold the
+// byte offset there until the group runs; `rel` bits name them.
+constexpr uint8_t kRelA = 1, kRelB = 2;
+
 struct Group {
   bool active = false;
   hipStream_t stream = nullptr;
   std::vector<PotrfOp> potrf;
-  std::vector<TrsmOp<double>> trsm;
-  std::vector<PotrfOp> trsm_dinv;  // dinv jobs for the TRSMs (A = L tile)
+  // TRSMs B := B L^{-T} (trans) and B := B L^{-1} (notrans), each with the
+  // job that forms the inverses of L's 32 x 32 diagonal blocks; rel kRelB:
+  // B is a work tile
+  std::vector<TrsmOp<double>> trsm, trsm_n;
+  std::vector<PotrfOp> trsm_dinv, trsm_n_dinv;
+  std::vector<uint8_t> trsm_rel, trsm_n_rel;
   std::map<UpdKey, std::vector<UpdOp<double>>> upd;
+  std::map<UpdKey, std::vector<uint8_t>> upd_rel;  // kRelA / kRelB: A / B in the work area
+  // passes of the POINV tile operations: before the TRSMs and updates, and
+  // after them (rel kRelA: src, kRelB: dst in the work area)
+  std::vector<TopOp> pre, post;
+  std::vector<uint8_t> pre_rel, post_rel;
+  std::vector<DiagZeroOp> zero;
+  size_t work = 0;  // bytes of work tiles
   struct Norm {
     int n;
     const double *x;
     double *r;
   };
   std::vector<Norm> norm_after;  // norms of memory the group writes
+  struct Write {
+    const double *p;
+    int m, n, ld;
+  };
+  std::vector<Write> writes;  // outputs of the POINV tile operations
   std::vector<PlgsyOp> plgsy;
   int last_launches = 0;
+
+  void clear() {
+    potrf.clear();
+    trsm.clear();
+    trsm_n.clear();
+    trsm_dinv.clear();
+    trsm_n_dinv.clear();
+    trsm_rel.clear();
+    trsm_n_rel.clear();
+    upd.clear();
+    upd_rel.clear();
+    pre.clear();
+    post.clear();
+    pre_rel.clear();
+    post_rel.clear();
+    zero.clear();
+    work = 0;
+    norm_after.clear();
+    writes.clear();
+    plgsy.clear();
+  }
+  // a work tile of m x n doubles (leading dimension m), as an offset
+  double *work_tile(int m, int n) {
+    const size_t off = work;
+    work += ((size_t)m * n * sizeof(double) + 255) & ~size_t(255);
+    return reinterpret_cast<double *>(off);
+  }
+  void add_upd(const UpdKey &k, const UpdOp<double> &op, uint8_t rel) {
+    upd[k].push_back(op);
+    upd_rel[k].push_back(rel);
+  }
 };
 static thread_local Group g_group;
 
@@ -149,9 +203,11 @@ bool group_defer_nrm2(hipStream_t s, int n, const double *x, double *r) {
   bool written = false;
   for (auto &o : g.plgsy) written = written || overlaps(x, n, o.A, o.m, o.n, (int)o.lda);
   for (auto &o : g.potrf) written = written || overlaps(x, n, o.A, o.n, o.n, o.lda);
-  for (auto &o : g.trsm) written = written || overlaps(x, n, o.B, o.m, o.n, o.ldb);
+  for (size_t i = 0; i < g.trsm.size(); ++i)
+    written = written || (!(g.trsm_rel[i] & kRelB) && overlaps(x, n, g.trsm[i].B, g.trsm[i].m, g.trsm[i].n, g.trsm[i].ldb));
   for (auto &kv : g.upd)
     for (auto &o : kv.second) written = written || overlaps(x, n, o.C, o.m, o.n, o.ldc);
+  for (auto &w : g.writes) written = written || overlaps(x, n, w.p, w.m, w.n, w.ld);
   if (written) g.norm_after.push_back(Group::Norm{n, x, r});
   return written;
 }
@@ -160,6 +216,256 @@ bool group_record_plgsy(hipStream_t s, const PlgsyOp &op) {
   if (!grouping(s)) return false;
   g_group.plgsy.push_back(op);
   return true;
+}
+
+// ---------------------------------------------------------------------------
+// POINV tile operations as passes of a group (work tiles are offsets into the
+// group's work area until it runs)
+// ---------------------------------------------------------------------------
+static void rec_trsm(Group &g, bool trans, const double *L, int ldl, int order, double *B, int m, int ldb,
+                     uint8_t rel) {
+  TrsmOp<double> op{L, B, nullptr, m, order, ldl, ldb};
+  PotrfOp dj{const_cast<double *>(L), nullptr, nullptr, order, ldl, 0, 0};
+  if (trans) {
+    g.trsm.push_back(op);
+    g.trsm_dinv.push_back(dj);
+    g.trsm_rel.push_back(rel);
+  } else {
+    g.trsm_n.push_back(op);
+    g.trsm_n_dinv.push_back(dj);
+    g.trsm_n_rel.push_back(rel);
+  }
+}
+
+static void rec_pass(std::vector<TopOp> &v, std::vector<uint8_t> &rel, const TopOp &op, uint8_t r) {
+  v.push_back(op);
+  rel.push_back(r);
+}
+
+// A := A^{-1} (lower, non-unit): W := I; W := W L^{-T} = L^{-T}; lower(A) := lower(W^T)
+static void rec_trtri(Group &g, int n, double *A, int lda, int *d_info, int info_offset) {
+  if (d_info) g.zero.push_back(DiagZeroOp{A, d_info, n, lda, info_offset});
+  double *W = g.work_tile(n, n);
+  rec_pass(g.pre, g.pre_rel, TopOp{nullptr, W, n, n, n, n, kTopIdentity, 1.0}, kRelB);
+  rec_trsm(g, true, A, lda, n, W, n, n, kRelB);
+  rec_pass(g.post, g.post_rel, TopOp{W, A, n, n, n, lda, kTopTransLower, 1.0}, kRelA);
+  g.writes.push_back(Group::Write{A, n, n, lda});
+}
+
+// B (m x n) := alpha L^{-1} B = alpha (B^T L^{-T})^T, L m x m
+static void rec_trsm_lln(Group &g, int m, int n, double alpha, const double *L, int ldl, double *B, int ldb) {
+  double *W = g.work_tile(n, m);
+  rec_pass(g.pre, g.pre_rel, TopOp{B, W, m, n, ldb, n, kTopTrans, 1.0}, kRelB);
+  rec_trsm(g, true, L, ldl, m, W, n, n, kRelB);
+  rec_pass(g.post, g.post_rel, TopOp{W, B, n, m, n, ldb, kTopTrans, alpha}, kRelA);
+  g.writes.push_back(Group::Write{B, m, n, ldb});
+}
+
+// B (m x n) := alpha B L^{-1}, L n x n
+static void rec_trsm_rln(Group &g, int m, int n, double alpha, const double *L, int ldl, double *B, int ldb) {
+  rec_trsm(g, false, L, ldl, n, B, m, ldb, 0);
+  if (alpha != 1.0) rec_pass(g.post, g.post_rel, TopOp{B, B, m, n, ldb, ldb, kTopCopy, alpha}, 0);
+  g.writes.push_back(Group::Write{B, m, n, ldb});
+}
+
+// lower(A) := L^T L (L the lower triangle of A): W := lower(A); lower(A) := W^T W
+static void rec_lauum(Group &g, int n, double *A, int lda) {
+  double *W = g.work_tile(n, n);
+  rec_pass(g.pre, g.pre_rel, TopOp{A, W, n, n, lda, n, kTopLowerZero, 1.0}, kRelB);
+  g.add_upd(UpdKey{1.0, 0.0, true, true}, UpdOp<double>{W, W, A, n, n, n, n, n, lda, 1}, kRelA | kRelB);
+  g.writes.push_back(Group::Write{A, n, n, lda});
+}
+
+// B (m x n) := alpha L^T B, L m x m lower: W1 := lower(L), W2 := B; B := alpha W1^T W2
+static void rec_trmm_llt(Group &g, int m, int n, double alpha, const double *L, int ldl, double *B, int ldb) {
+  double *W1 = g.work_tile(m, m), *W2 = g.work_tile(m, n);
+  rec_pass(g.pre, g.pre_rel, TopOp{L, W1, m, m, ldl, m, kTopLowerZero, 1.0}, kRelB);
+  rec_pass(g.pre, g.pre_rel, TopOp{B, W2, m, n, ldb, m, kTopCopy, 1.0}, kRelB);
+  g.add_upd(UpdKey{alpha, 0.0, true, true}, UpdOp<double>{W1, W2, B, m, n, m, m, m, ldb, 0}, kRelA | kRelB);
+  g.writes.push_back(Group::Write{B, m, n, ldb});
+}
+
+static int run_group(Group &g, hipStream_t s);
+
+// a group of its own for a call outside any open group
+template <typename F>
+static int run_alone(hipStream_t s, F &&record) {
+  Group local;
+  record(local);
+  return run_group(local, s);
+}
+
+// Run the recorded work of a group on stream s: every descriptor array and
+// the work tiles live in one staging buffer of the stream, filled by one
+// copy.  The ops of a group are independent of each other (the tasks of one
+// scheduling round), so the kinds run in a fixed order; the passes of one
+// POINV operation are ordered by it: diagonal checks, pre passes, TRSM
+// inverses, TRSMs, updates, post passes, then deferred norms.
+static int run_group(Group &g, hipStream_t s) {
+  StreamWork &w = stream_work(s);
+  size_t off = 0;
+  auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+  auto place = [&](size_t bytes) {
+    const size_t o = off;
+    off = align(off + bytes);
+    return o;
+  };
+  const size_t o_plgsy = place(g.plgsy.size() * sizeof(PlgsyOp));
+  const size_t o_potrf = place(g.potrf.size() * sizeof(PotrfOp));
+  const size_t o_zero = place(g.zero.size() * sizeof(DiagZeroOp));
+  const size_t o_pre = place(g.pre.size() * sizeof(TopOp));
+  const size_t o_post = place(g.post.size() * sizeof(TopOp));
+  const size_t ndinv = g.trsm_dinv.size() + g.trsm_n_dinv.size();
+  const size_t o_dinvops = place(ndinv * sizeof(PotrfOp));
+  const size_t o_trsm = place(g.trsm.size() * sizeof(TrsmOp<double>));
+  const size_t o_trsm_n = place(g.trsm_n.size() * sizeof(TrsmOp<double>));
+  std::vector<size_t> o_upd;
+  for (auto &kv : g.upd) o_upd.push_back(place(kv.second.size() * sizeof(UpdOp<double>)));
+  const size_t desc_bytes = off;  // descriptors end here: copied from the host
+  size_t dinv_bytes = 0;
+  for (auto &d : g.trsm_dinv) dinv_bytes += sizeof(double) * dinv_doubles(d.n);
+  for (auto &d : g.trsm_n_dinv) dinv_bytes += sizeof(double) * dinv_doubles(d.n);
+  const size_t dinv_off = place(dinv_bytes);
+  const size_t work_off = place(g.work);
+  if (off == 0) {
+    g.last_launches = 0;
+    return TTG_CHOL_OK;
+  }
+  TTGC_HIP(w.ops.reserve(off));
+  char *base = static_cast<char *>(w.ops.ptr);
+  char *work = base + work_off;
+  auto reloc = [&](const double *p) {
+    return reinterpret_cast<double *>(work + reinterpret_cast<uintptr_t>(p));
+  };
+  // inverse-block workspace of the TRSMs and the work tiles
+  {
+    size_t p = dinv_off;
+    for (size_t i = 0; i < g.trsm.size(); ++i) {
+      g.trsm_dinv[i].dinv = reinterpret_cast<double *>(base + p);
+      g.trsm[i].dinv = g.trsm_dinv[i].dinv;
+      if (g.trsm_rel[i] & kRelB) g.trsm[i].B = reloc(g.trsm[i].B);
+      p += sizeof(double) * dinv_doubles(g.trsm_dinv[i].n);
+    }
+    for (size_t i = 0; i < g.trsm_n.size(); ++i) {
+      g.trsm_n_dinv[i].dinv = reinterpret_cast<double *>(base + p);
+      g.trsm_n[i].dinv = g.trsm_n_dinv[i].dinv;
+      if (g.trsm_n_rel[i] & kRelB) g.trsm_n[i].B = reloc(g.trsm_n[i].B);
+      p += sizeof(double) * dinv_doubles(g.trsm_n_dinv[i].n);
+    }
+  }
+  for (size_t i = 0; i < g.pre.size(); ++i) {
+    if (g.pre_rel[i] & kRelA) g.pre[i].src = reloc(g.pre[i].src);
+    if (g.pre_rel[i] & kRelB) g.pre[i].dst = reloc(g.pre[i].dst);
+  }
+  for (size_t i = 0; i < g.post.size(); ++i) {
+    if (g.post_rel[i] & kRelA) g.post[i].src = reloc(g.post[i].src);
+    if (g.post_rel[i] & kRelB) g.post[i].dst = reloc(g.post[i].dst);
+  }
+  for (auto &kv : g.upd) {
+    auto &rel = g.upd_rel[kv.first];
+    for (size_t i = 0; i < kv.second.size(); ++i) {
+      if (rel[i] & kRelA) kv.second[i].A = reloc(kv.second[i].A);
+      if (rel[i] & kRelB) kv.second[i].B = reloc(kv.second[i].B);
+    }
+  }
+  TTGC_HIP(w.host_reserve(desc_bytes));
+  char *host = w.host;
+  auto put = [&](size_t o, const void *src, size_t bytes) {
+    if (bytes) memcpy(host + o, src, bytes);
+  };
+  put(o_plgsy, g.plgsy.data(), g.plgsy.size() * sizeof(PlgsyOp));
+  put(o_potrf, g.potrf.data(), g.potrf.size() * sizeof(PotrfOp));
+  put(o_zero, g.zero.data(), g.zero.size() * sizeof(DiagZeroOp));
+  put(o_pre, g.pre.data(), g.pre.size() * sizeof(TopOp));
+  put(o_post, g.post.data(), g.post.size() * sizeof(TopOp));
+  put(o_dinvops, g.trsm_dinv.data(), g.trsm_dinv.size() * sizeof(PotrfOp));
+  put(o_dinvops + g.trsm_dinv.size() * sizeof(PotrfOp), g.trsm_n_dinv.data(), g.trsm_n_dinv.size() * sizeof(PotrfOp));
+  put(o_trsm, g.trsm.data(), g.trsm.size() * sizeof(TrsmOp<double>));
+  put(o_trsm_n, g.trsm_n.data(), g.trsm_n.size() * sizeof(TrsmOp<double>));
+  {
+    size_t i = 0;
+    for (auto &kv : g.upd) put(o_upd[i++], kv.second.data(), kv.second.size() * sizeof(UpdOp<double>));
+  }
+  if (desc_bytes) {
+    TTGC_HIP(hipMemcpyAsync(base, host, desc_bytes, hipMemcpyHostToDevice, s));
+    TTGC_HIP(hipEventRecord(w.copied, s));
+    w.pending = true;
+  }
+  auto max_dims = [](const std::vector<TopOp> &v, int &mm, int &mn) {
+    mm = mn = 0;
+    for (auto &o : v) {
+      mm = std::max(mm, o.m);
+      mn = std::max(mn, o.n);
+    }
+  };
+  int launches = 0;
+  if (!g.plgsy.empty()) {
+    int max_n = 0;
+    for (auto &o : g.plgsy) max_n = std::max(max_n, o.n);
+    TTGC_HIP(launch_plgsy_batched(s, reinterpret_cast<PlgsyOp *>(base + o_plgsy), (int)g.plgsy.size(), max_n));
+    launches += ((int)g.plgsy.size() + 65534) / 65535;
+  }
+  PotrfOp none{};
+  if (!g.potrf.empty()) {
+    TTGC_HIP(launch_potrf(s, reinterpret_cast<PotrfOp *>(base + o_potrf), (int)g.potrf.size(), none, nullptr));
+    ++launches;
+  }
+  if (!g.zero.empty()) {
+    TTGC_HIP(launch_diag_zero(s, reinterpret_cast<DiagZeroOp *>(base + o_zero), (int)g.zero.size()));
+    ++launches;
+  }
+  if (!g.pre.empty()) {
+    int mm, mn;
+    max_dims(g.pre, mm, mn);
+    TTGC_HIP(launch_tile_ops(s, reinterpret_cast<TopOp *>(base + o_pre), (int)g.pre.size(), mm, mn));
+    ++launches;
+  }
+  if (ndinv) {
+    TTGC_HIP(launch_dinv(s, reinterpret_cast<PotrfOp *>(base + o_dinvops), (int)ndinv, none));
+    ++launches;
+  }
+  TrsmOp<double> tnone{};
+  if (!g.trsm.empty()) {
+    int max_m = 0;
+    for (auto &t : g.trsm) max_m = std::max(max_m, t.m);
+    TTGC_HIP(launch_trsm<double>(s, reinterpret_cast<TrsmOp<double> *>(base + o_trsm), (int)g.trsm.size(), tnone,
+                                 max_m, nullptr, true));
+    ++launches;
+  }
+  if (!g.trsm_n.empty()) {
+    int max_m = 0;
+    for (auto &t : g.trsm_n) max_m = std::max(max_m, t.m);
+    TTGC_HIP(launch_trsm<double>(s, reinterpret_cast<TrsmOp<double> *>(base + o_trsm_n), (int)g.trsm_n.size(),
+                                 tnone, max_m, nullptr, false));
+    ++launches;
+  }
+  {
+    size_t i = 0;
+    for (auto &kv : g.upd) {
+      int max_m = 0, max_n = 0;
+      bool fast = true;
+      for (auto &u : kv.second) {
+        max_m = std::max(max_m, u.m);
+        max_n = std::max(max_n, u.n);
+      }
+      for (auto &u : kv.second) fast = fast && update_fast(u, max_m);
+      UpdOp<double> unone{};
+      TTGC_HIP(launch_update<double>(s, reinterpret_cast<UpdOp<double> *>(base + o_upd[i]), (int)kv.second.size(),
+                                     unone, max_m, max_n, kv.first.alpha, kv.first.beta, fast, nullptr, kv.first.tb,
+                                     kv.first.ta));
+      ++launches;
+      ++i;
+    }
+  }
+  if (!g.post.empty()) {
+    int mm, mn;
+    max_dims(g.post, mm, mn);
+    TTGC_HIP(launch_tile_ops(s, reinterpret_cast<TopOp *>(base + o_post), (int)g.post.size(), mm, mn));
+    ++launches;
+  }
+  for (auto &nm : g.norm_after) TTGC_HIP(launch_nrm2(s, nm.n, nm.x, nm.r));
+  g.last_launches = launches;
+  return TTG_CHOL_OK;
 }
 
 }  // namespace ttgc
@@ -262,6 +568,7 @@ int ttg_chol_dtrsm_rlt(void *stream, int m, int n, const double *L, int ldl, dou
   if (grouping(s)) {
     g_group.trsm.push_back(op);
     g_group.trsm_dinv.push_back(dj);
+    g_group.trsm_rel.push_back(0);
     return TTG_CHOL_OK;
   }
   StreamWork &w = stream_work(s);
@@ -327,7 +634,7 @@ static int update_entry(const char *name, hipStream_t s, int m, int n, int k, do
   }
   UpdOp<double> op{A, B, C, m, n, k, lda, ldb, ldc, lower};
   if (grouping(s)) {
-    g_group.upd[UpdKey{alpha, beta}].push_back(op);
+    g_group.add_upd(UpdKey{alpha, beta, false, false}, op, 0);
     return TTG_CHOL_OK;
   }
   TTGC_HIP(launch_update<double>(s, nullptr, 1, op, m, n, alpha, beta, update_fast(op, m), nullptr));
@@ -346,6 +653,132 @@ int ttg_chol_dgemm_nt(void *stream, int m, int n, int k, double alpha, const dou
   return update_entry("dgemm_nt", as_stream(stream), m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 0);
 }
 
+// ---- POINV tile operations (examples/potrf/trtri_L.h, lauum.h)
+static bool bad_tile(int n, int ld, const void *p) { return n < 0 || n > TTG_CHOL_MAX_TILE || ld < std::max(1, n) || (n > 0 && !p); }
+
+int ttg_chol_dtrtri_tile(void *stream, int n, double *A, int lda, int *d_info, int info_offset) {
+  clear_error();
+  if (bad_tile(n, lda, A)) {
+    set_error("dtrtri_tile: bad argument (n=%d lda=%d)", n, lda);
+    return TTG_CHOL_EINVAL;
+  }
+  hipStream_t s = as_stream(stream);
+  if (n == 0) {  // only *d_info := 0 (by the diagonal check of an empty tile)
+    if (!d_info) return TTG_CHOL_OK;
+    const DiagZeroOp z{A, d_info, 0, lda, info_offset};
+    if (grouping(s)) {
+      g_group.zero.push_back(z);
+      return TTG_CHOL_OK;
+    }
+    return run_alone(s, [&](Group &g) { g.zero.push_back(z); });
+  }
+  if (grouping(s)) {
+    rec_trtri(g_group, n, A, lda, d_info, info_offset);
+    return TTG_CHOL_OK;
+  }
+  return run_alone(s, [&](Group &g) { rec_trtri(g, n, A, lda, d_info, info_offset); });
+}
+
+static int trsm_args(const char *name, int m, int n, int order, const double *L, int ldl, const double *B, int ldb) {
+  if (m < 0 || n < 0 || order > TTG_CHOL_MAX_TILE || m > TTG_CHOL_MAX_TILE || n > TTG_CHOL_MAX_TILE ||
+      ldl < std::max(1, order) || ldb < std::max(1, m) || (m > 0 && n > 0 && (!L || !B))) {
+    set_error("%s: bad argument (m=%d n=%d ldl=%d ldb=%d)", name, m, n, ldl, ldb);
+    return TTG_CHOL_EINVAL;
+  }
+  return TTG_CHOL_OK;
+}
+
+int ttg_chol_dtrsm_lln(void *stream, int m, int n, double alpha, const double *L, int ldl, double *B, int ldb) {
+  clear_error();
+  if (int rc = trsm_args("dtrsm_lln", m, n, m, L, ldl, B, ldb)) return rc;
+  if (m == 0 || n == 0) return TTG_CHOL_OK;
+  hipStream_t s = as_stream(stream);
+  if (grouping(s)) {
+    rec_trsm_lln(g_group, m, n, alpha, L, ldl, B, ldb);
+    return TTG_CHOL_OK;
+  }
+  return run_alone(s, [&](Group &g) { rec_trsm_lln(g, m, n, alpha, L, ldl, B, ldb); });
+}
+
+int ttg_chol_dtrsm_rln(void *stream, int m, int n, double alpha, const double *L, int ldl, double *B, int ldb) {
+  clear_error();
+  if (int rc = trsm_args("dtrsm_rln", m, n, n, L, ldl, B, ldb)) return rc;
+  if (m == 0 || n == 0) return TTG_CHOL_OK;
+  hipStream_t s = as_stream(stream);
+  if (grouping(s)) {
+    rec_trsm_rln(g_group, m, n, alpha, L, ldl, B, ldb);
+    return TTG_CHOL_OK;
+  }
+  return run_alone(s, [&](Group &g) { rec_trsm_rln(g, m, n, alpha, L, ldl, B, ldb); });
+}
+
+int ttg_chol_dlauum_tile(void *stream, int n, double *A, int lda) {
+  clear_error();
+  if (bad_tile(n, lda, A)) {
+    set_error("dlauum_tile: bad argument (n=%d lda=%d)", n, lda);
+    return TTG_CHOL_EINVAL;
+  }
+  if (n == 0) return TTG_CHOL_OK;
+  hipStream_t s = as_stream(stream);
+  if (grouping(s)) {
+    rec_lauum(g_group, n, A, lda);
+    return TTG_CHOL_OK;
+  }
+  return run_alone(s, [&](Group &g) { rec_lauum(g, n, A, lda); });
+}
+
+int ttg_chol_dtrmm_llt(void *stream, int m, int n, double alpha, const double *L, int ldl, double *B, int ldb) {
+  clear_error();
+  if (int rc = trsm_args("dtrmm_llt", m, n, m, L, ldl, B, ldb)) return rc;
+  if (m == 0 || n == 0) return TTG_CHOL_OK;
+  hipStream_t s = as_stream(stream);
+  if (grouping(s)) {
+    rec_trmm_llt(g_group, m, n, alpha, L, ldl, B, ldb);
+    return TTG_CHOL_OK;
+  }
+  return run_alone(s, [&](Group &g) { rec_trmm_llt(g, m, n, alpha, L, ldl, B, ldb); });
+}
+
+// C := alpha op(A) op(B) + beta C with op(A) = A (ta = 0, m x k) or A^T
+// (ta = 1, A stored k x m) and op(B) = B (k x n); lower: only C's lower
+// triangle (C square)
+static int update_tb_entry(const char *name, hipStream_t s, bool ta, int m, int n, int k, double alpha,
+                           const double *A, int lda, const double *B, int ldb, double beta, double *C, int ldc,
+                           int lower) {
+  if (m < 0 || n < 0 || k < 0 || lda < std::max(1, ta ? k : m) || ldb < std::max(1, k) || ldc < std::max(1, m) ||
+      (m > 0 && n > 0 && (!C || (k > 0 && (!A || !B))))) {
+    set_error("%s: bad argument (m=%d n=%d k=%d lda=%d ldb=%d ldc=%d)", name, m, n, k, lda, ldb, ldc);
+    return TTG_CHOL_EINVAL;
+  }
+  if (m == 0 || n == 0) return TTG_CHOL_OK;
+  if (k == 0) alpha = 0.0;  // C := beta C
+  UpdOp<double> op{A, B, C, m, n, k, lda, ldb, ldc, lower};
+  const UpdKey key{alpha, beta, ta, true};
+  if (grouping(s)) {
+    g_group.add_upd(key, op, 0);
+    return TTG_CHOL_OK;
+  }
+  return run_alone(s, [&](Group &g) { g.add_upd(key, op, 0); });
+}
+
+int ttg_chol_dgemm_nn(void *stream, int m, int n, int k, double alpha, const double *A, int lda, const double *B,
+                      int ldb, double beta, double *C, int ldc) {
+  clear_error();
+  return update_tb_entry("dgemm_nn", as_stream(stream), false, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 0);
+}
+
+int ttg_chol_dgemm_tn(void *stream, int m, int n, int k, double alpha, const double *A, int lda, const double *B,
+                      int ldb, double beta, double *C, int ldc) {
+  clear_error();
+  return update_tb_entry("dgemm_tn", as_stream(stream), true, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 0);
+}
+
+int ttg_chol_dsyrk_lt(void *stream, int n, int k, double alpha, const double *A, int lda, double beta, double *C,
+                      int ldc) {
+  clear_error();
+  return update_tb_entry("dsyrk_lt", as_stream(stream), true, n, n, k, alpha, A, lda, A, lda, beta, C, ldc, 1);
+}
+
 int ttg_chol_group_begin(void *stream) {
   clear_error();
   if (g_group.active) {
@@ -354,12 +787,7 @@ int ttg_chol_group_begin(void *stream) {
   }
   g_group.active = true;
   g_group.stream = as_stream(stream);
-  g_group.potrf.clear();
-  g_group.trsm.clear();
-  g_group.trsm_dinv.clear();
-  g_group.upd.clear();
-  g_group.norm_after.clear();
-  g_group.plgsy.clear();
+  g_group.clear();
   return TTG_CHOL_OK;
 }
 
@@ -371,105 +799,8 @@ int ttg_chol_group_end(void *stream) {
     set_error("group_end: no group open on this stream");
     return TTG_CHOL_ESTATE;
   }
-  Group &g = g_group;
-  g.active = false;
-  hipStream_t s = g.stream;
-  StreamWork &w = stream_work(s);
-  // one staging buffer holds every descriptor array of the group
-  size_t off = 0;
-  auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
-  size_t o_plgsy = off;
-  off = align(off + g.plgsy.size() * sizeof(PlgsyOp));
-  size_t o_potrf = off;
-  off = align(off + g.potrf.size() * sizeof(PotrfOp));
-  size_t o_dinvops = off;
-  off = align(off + g.trsm_dinv.size() * sizeof(PotrfOp));
-  size_t o_trsm = off;
-  off = align(off + g.trsm.size() * sizeof(TrsmOp<double>));
-  std::vector<size_t> o_upd;
-  for (auto &kv : g.upd) {
-    o_upd.push_back(off);
-    off = align(off + kv.second.size() * sizeof(UpdOp<double>));
-  }
-  size_t dinv_off = off;
-  size_t dinv_bytes = 0;
-  for (auto &d : g.trsm_dinv) dinv_bytes += sizeof(double) * dinv_doubles(d.n);
-  off = align(off + dinv_bytes);
-  if (off == 0) {
-    g.last_launches = 0;
-    return TTG_CHOL_OK;
-  }
-  TTGC_HIP(w.ops.reserve(off));
-  char *base = static_cast<char *>(w.ops.ptr);
-  // inverse-block workspace for the TRSMs lives in the same buffer
-  {
-    size_t p = dinv_off;
-    for (size_t i = 0; i < g.trsm.size(); ++i) {
-      g.trsm_dinv[i].dinv = reinterpret_cast<double *>(base + p);
-      g.trsm[i].dinv = g.trsm_dinv[i].dinv;
-      p += sizeof(double) * dinv_doubles(g.trsm_dinv[i].n);
-    }
-  }
-  TTGC_HIP(w.host_reserve(dinv_off));
-  char *host = w.host;
-  if (!g.plgsy.empty()) memcpy(host + o_plgsy, g.plgsy.data(), g.plgsy.size() * sizeof(PlgsyOp));
-  if (!g.potrf.empty()) memcpy(host + o_potrf, g.potrf.data(), g.potrf.size() * sizeof(PotrfOp));
-  if (!g.trsm_dinv.empty())
-    memcpy(host + o_dinvops, g.trsm_dinv.data(), g.trsm_dinv.size() * sizeof(PotrfOp));
-  if (!g.trsm.empty()) memcpy(host + o_trsm, g.trsm.data(), g.trsm.size() * sizeof(TrsmOp<double>));
-  {
-    size_t i = 0;
-    for (auto &kv : g.upd) {
-      memcpy(host + o_upd[i], kv.second.data(), kv.second.size() * sizeof(UpdOp<double>));
-      ++i;
-    }
-  }
-  if (dinv_off) {
-    TTGC_HIP(hipMemcpyAsync(base, host, dinv_off, hipMemcpyHostToDevice, s));
-    TTGC_HIP(hipEventRecord(w.copied, s));
-    w.pending = true;
-  }
-  int launches = 0;
-  if (!g.plgsy.empty()) {
-    int max_n = 0;
-    for (auto &o : g.plgsy) max_n = std::max(max_n, o.n);
-    TTGC_HIP(launch_plgsy_batched(s, reinterpret_cast<PlgsyOp *>(base + o_plgsy), (int)g.plgsy.size(), max_n));
-    launches += ((int)g.plgsy.size() + 65534) / 65535;
-  }
-  PotrfOp none{};
-  if (!g.potrf.empty()) {
-    TTGC_HIP(launch_potrf(s, reinterpret_cast<PotrfOp *>(base + o_potrf), (int)g.potrf.size(), none, nullptr));
-    ++launches;
-  }
-  if (!g.trsm.empty()) {
-    TTGC_HIP(launch_dinv(s, reinterpret_cast<PotrfOp *>(base + o_dinvops), (int)g.trsm_dinv.size(), none));
-    int max_m = 0;
-    for (auto &t : g.trsm) max_m = std::max(max_m, t.m);
-    TrsmOp<double> tnone{};
-    TTGC_HIP(launch_trsm<double>(s, reinterpret_cast<TrsmOp<double> *>(base + o_trsm), (int)g.trsm.size(), tnone,
-                                 max_m, nullptr));
-    launches += 2;
-  }
-  {
-    size_t i = 0;
-    for (auto &kv : g.upd) {
-      int max_m = 0, max_n = 0;
-      bool fast = true;
-      for (auto &u : kv.second) {
-        max_m = std::max(max_m, u.m);
-        max_n = std::max(max_n, u.n);
-      }
-      for (auto &u : kv.second) fast = fast && update_fast(u, max_m);
-      UpdOp<double> unone{};
-      TTGC_HIP(launch_update<double>(s, reinterpret_cast<UpdOp<double> *>(base + o_upd[i]), (int)kv.second.size(),
-                                     unone, max_m, max_n, kv.first.alpha, kv.first.beta, fast, nullptr));
-      ++launches;
-      ++i;
-    }
-  }
-  for (auto &nm : g.norm_after) TTGC_HIP(launch_nrm2(s, nm.n, nm.x, nm.r));
-  g.last_launches = launches;
-  return TTG_CHOL_OK;
+  g_group.active = false;
+  return run_group(g_group, g_group.stream);
 }
 
 }  // extern "C"

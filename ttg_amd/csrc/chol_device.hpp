@@ -293,11 +293,12 @@ struct StageLoaderT {
 // GEMM core: acc (BM x BN block, spread over WM x WN waves) += A * B^T where
 // A is rows [i0, i0+BM) and B is rows [j0, j0+BN) of two column-major
 // matrices, summed over k in [kbeg, kend) (TB: B's element (j, k) is read
-// from B[k + j*ldb] instead).  Both operands are staged through
+// from B[k + j*ldb] instead; TA: A's element (i, k) from A[k + i*lda]).
+// Both operands are staged through
 // LDS in BK-wide slabs, double buffered with a register prefetch of the next
 // slab while MFMAs consume the current one.
 // ----------------------------------------------------------------------------
-template <typename T, int BM, int BN, int BK, int WM, int WN, bool TB = false>
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool TB = false, bool TA = false>
 struct GemmCore {
   static constexpr int NW = WM * WN;
   static constexpr int NTH = NW * kWave;
@@ -353,7 +354,9 @@ struct GemmCore {
                                        long ldb, int i0, int j0, int kbeg, int kend, int mlim, int nlim) {
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     const int wm = wave / WN, wn = wave % WN;
-    StageLoader<T, BM, BK, NTH, LDA_S> la;
+    using LoaderA = typename std::conditional<TA, StageLoaderT<T, BM, BK, NTH, LDA_S>,
+                                              StageLoader<T, BM, BK, NTH, LDA_S>>::type;
+    LoaderA la;
     using LoaderB = typename std::conditional<TB, StageLoaderT<T, BN, BK, NTH, LDB_S>,
                                               StageLoader<T, BN, BK, NTH, LDB_S>>::type;
     LoaderB lb;

@@ -359,31 +359,31 @@ __global__ __launch_bounds__(256) void trsm_kernel(const TrsmOp<T> *__restrict__
 // ---------------------------------------------------------------------------
 // Two block shapes: 128 x 128 for tile updates, 32 x 64 ("skinny") when
 // the output has at most 32 rows (the right-hand-side panels of the solves).
-template <typename T, bool TB, bool SKINNY>
+template <typename T, bool TB, bool SKINNY, bool TA>
 struct UpdCfg;
-template <bool TB>
-struct UpdCfg<double, TB, false> {
-  using Core = GemmCore<double, 128, 128, 16, 2, 2, TB>;
+template <bool TB, bool TA>
+struct UpdCfg<double, TB, false, TA> {
+  using Core = GemmCore<double, 128, 128, 16, 2, 2, TB, TA>;
 };
-template <bool TB>
-struct UpdCfg<float, TB, false> {
-  using Core = GemmCore<float, 128, 128, 32, 2, 2, TB>;
+template <bool TB, bool TA>
+struct UpdCfg<float, TB, false, TA> {
+  using Core = GemmCore<float, 128, 128, 32, 2, 2, TB, TA>;
 };
-template <bool TB>
-struct UpdCfg<double, TB, true> {
-  using Core = GemmCore<double, 32, 64, 32, 1, 4, TB>;
+template <bool TB, bool TA>
+struct UpdCfg<double, TB, true, TA> {
+  using Core = GemmCore<double, 32, 64, 32, 1, 4, TB, TA>;
 };
-template <bool TB>
-struct UpdCfg<float, TB, true> {
-  using Core = GemmCore<float, 32, 64, 32, 1, 4, TB>;
+template <bool TB, bool TA>
+struct UpdCfg<float, TB, true, TA> {
+  using Core = GemmCore<float, 32, 64, 32, 1, 4, TB, TA>;
 };
 
 // FAST applies to both operands for the 128 x 128 shape and to the B operand
 // only for the skinny shape (whose A rows are predicated).
-template <typename T, bool FAST, bool TB, bool SKINNY>
+template <typename T, bool FAST, bool TB, bool SKINNY, bool TA = false>
 __device__ __forceinline__ void update_body(const UpdOp<T> *__restrict__ ops, UpdOp<T> one, int bpo, int nbn,
                                                      T alpha, T beta, const int *__restrict__ d_abort) {
-  using Core = typename UpdCfg<T, TB, SKINNY>::Core;
+  using Core = typename UpdCfg<T, TB, SKINNY, TA>::Core;
   constexpr int BM = SKINNY ? 32 : 128;
   constexpr int BN = SKINNY ? 64 : 128;
   const int b = xcd_remap(blockIdx.x, gridDim.x);
@@ -417,6 +417,15 @@ __device__ __forceinline__ void update_body(const UpdOp<T> *__restrict__ ops, Up
   Core core;
   core.zero();
   core.template run2<FAST && !SKINNY, FAST>(sm, op.A, op.lda, op.B, op.ldb, i0, j0, 0, op.k, op.m, op.n);
+  // one epilogue formula on every path (alpha acc rounded, then one fused
+  // multiply-add with beta old), so that the unpredicated and the predicated
+  // paths, and hence a lone launch and a batched one, round alike
+  auto epi = [&](T acc, T old) -> T {
+    if constexpr (sizeof(T) == 8)
+      return __builtin_fma(beta, old, alpha * acc);
+    else
+      return __builtin_fmaf(beta, old, alpha * acc);
+  };
 
   const bool diag = op.lower && bi == bj;
   if constexpr (PRE) {
@@ -430,8 +439,7 @@ __device__ __forceinline__ void update_body(const UpdOp<T> *__restrict__ ops, Up
         for (int r = 0; r < 4; ++r) {
           const int row = i0 + Core::row_of(wm, i, lane, r);
           if (!diag || row >= col) {
-            const T v = alpha * core.acc[i][j][r];
-            pc[row] = (beta == T(0)) ? v : v + beta * cpre[i][j][r];
+            pc[row] = (beta == T(0)) ? alpha * core.acc[i][j][r] : epi(core.acc[i][j][r], cpre[i][j][r]);
           }
         }
       }
@@ -453,16 +461,14 @@ __device__ __forceinline__ void update_body(const UpdOp<T> *__restrict__ ops, Up
 #pragma unroll
           for (int r = 0; r < 4; ++r) old[r] = pc[i0 + Core::row_of(wm, i, lane, r)];
 #pragma unroll
-          for (int r = 0; r < 4; ++r) pc[i0 + Core::row_of(wm, i, lane, r)] = alpha * core.acc[i][j][r] + beta * old[r];
+          for (int r = 0; r < 4; ++r) pc[i0 + Core::row_of(wm, i, lane, r)] = epi(core.acc[i][j][r], old[r]);
         }
       } else {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int row = i0 + Core::row_of(wm, i, lane, r);
-          if (row < op.m && col < op.n && (!diag || row >= col)) {
-            const T v = alpha * core.acc[i][j][r];
-            pc[row] = (beta == T(0)) ? v : v + beta * pc[row];
-          }
+          if (row < op.m && col < op.n && (!diag || row >= col))
+            pc[row] = (beta == T(0)) ? alpha * core.acc[i][j][r] : epi(core.acc[i][j][r], pc[row]);
         }
       }
     }
@@ -476,6 +482,16 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 2 : 1) void update_kernel(con
                                                      T alpha, T beta, const int *__restrict__ d_abort) {
   if (d_abort && *d_abort) return;
   update_body<T, FAST, TB, SKINNY>(ops, one, bpo, nbn, alpha, beta, d_abort);
+}
+
+// C := alpha A^T op(B) + beta C (A stored k x m): the TRMM, SYRK and GEMM of
+// the LAUUM graph (FP64 only)
+template <bool FAST, bool TB, bool SKINNY>
+__global__ __launch_bounds__(256, 2) void update_ta_kernel(const UpdOp<double> *__restrict__ ops, UpdOp<double> one,
+                                                        int bpo, int nbn, double alpha, double beta,
+                                                        const int *__restrict__ d_abort) {
+  if (d_abort && *d_abort) return;
+  update_body<double, FAST, TB, SKINNY, true>(ops, one, bpo, nbn, alpha, beta, d_abort);
 }
 
 // The same update, as its own kernel symbol for the matrix generator (so
@@ -525,13 +541,35 @@ hipError_t launch_trsm(hipStream_t s, const TrsmOp<T> *ops, int nops, const Trsm
 
 template <typename T, bool SKINNY>
 static hipError_t launch_update_shape(hipStream_t s, const UpdOp<T> *ops, int nops, const UpdOp<T> &one, int max_m,
-                                      int max_n, T alpha, T beta, bool fast, const int *d_abort, bool tb) {
+                                      int max_n, T alpha, T beta, bool fast, const int *d_abort, bool tb, bool ta) {
   constexpr int BM = SKINNY ? 32 : 128, BN = SKINNY ? 64 : 128;
   const int nbm = (max_m + BM - 1) / BM, nbn = (max_n + BN - 1) / BN;
   const int bpo = nbm * nbn;
   const long grid = (long)nops * bpo;
   if (grid > 0x7fffffffL) return hipErrorInvalidConfiguration;
   const dim3 g((unsigned)grid), b(256);
+  if (ta) {
+    if constexpr (sizeof(T) == 8) {
+      if (tb) {
+        if (fast)
+          hipLaunchKernelGGL((update_ta_kernel<true, true, SKINNY>), g, b, 0, s, ops, one, bpo, nbn, alpha, beta,
+                             d_abort);
+        else
+          hipLaunchKernelGGL((update_ta_kernel<false, true, SKINNY>), g, b, 0, s, ops, one, bpo, nbn, alpha, beta,
+                             d_abort);
+      } else {
+        if (fast)
+          hipLaunchKernelGGL((update_ta_kernel<true, false, SKINNY>), g, b, 0, s, ops, one, bpo, nbn, alpha, beta,
+                             d_abort);
+        else
+          hipLaunchKernelGGL((update_ta_kernel<false, false, SKINNY>), g, b, 0, s, ops, one, bpo, nbn, alpha, beta,
+                             d_abort);
+      }
+      return hipGetLastError();
+    } else {
+      return hipErrorInvalidValue;  // transposed A is built for FP64 only
+    }
+  }
   if (tb) {
     if (fast)
       hipLaunchKernelGGL((update_kernel<T, true, true, SKINNY>), g, b, 0, s, ops, one, bpo, nbn, alpha, beta, d_abort);
@@ -551,10 +589,11 @@ static hipError_t launch_update_shape(hipStream_t s, const UpdOp<T> *ops, int no
 
 template <typename T>
 hipError_t launch_update(hipStream_t s, const UpdOp<T> *ops, int nops, const UpdOp<T> &one, int max_m, int max_n,
-                         T alpha, T beta, bool fast, const int *d_abort, bool tb) {
+                         T alpha, T beta, bool fast, const int *d_abort, bool tb, bool ta) {
   if (nops <= 0 || max_m <= 0 || max_n <= 0) return hipSuccess;
-  if (max_m <= 32) return launch_update_shape<T, true>(s, ops, nops, one, max_m, max_n, alpha, beta, fast, d_abort, tb);
-  return launch_update_shape<T, false>(s, ops, nops, one, max_m, max_n, alpha, beta, fast, d_abort, tb);
+  if (max_m <= 32)
+    return launch_update_shape<T, true>(s, ops, nops, one, max_m, max_n, alpha, beta, fast, d_abort, tb, ta);
+  return launch_update_shape<T, false>(s, ops, nops, one, max_m, max_n, alpha, beta, fast, d_abort, tb, ta);
 }
 
 hipError_t launch_gen_gemm(hipStream_t s, const UpdOp<double> *ops, int nops, int max_m, int max_n, double alpha,
@@ -579,8 +618,8 @@ template hipError_t launch_trsm<double>(hipStream_t, const TrsmOp<double> *, int
 template hipError_t launch_trsm<float>(hipStream_t, const TrsmOp<float> *, int, const TrsmOp<float> &, int,
                                        const int *, bool);
 template hipError_t launch_update<double>(hipStream_t, const UpdOp<double> *, int, const UpdOp<double> &, int, int,
-                                          double, double, bool, const int *, bool);
+                                          double, double, bool, const int *, bool, bool);
 template hipError_t launch_update<float>(hipStream_t, const UpdOp<float> *, int, const UpdOp<float> &, int, int,
-                                         float, float, bool, const int *, bool);
+                                         float, float, bool, const int *, bool, bool);
 
 }  // namespace ttgc

@@ -60,9 +60,11 @@ hipError_t launch_trsm(hipStream_t s, const TrsmOp<T> *ops, int nops, const Trsm
 // 32 rows (max_m <= 32) use a 32 x 64 block shape.
 // tb: the B operand's element (j, k) is read from B[k + j*ldb], i.e. the
 // update is C := alpha A B + beta C with B stored k x n column-major.
+// ta (FP64 only): the A operand's element (i, k) is read from A[k + i*lda],
+// i.e. op(A) = A^T with A stored k x m column-major.
 template <typename T>
 hipError_t launch_update(hipStream_t s, const UpdOp<T> *ops, int nops, const UpdOp<T> &one, int max_m, int max_n,
-                         T alpha, T beta, bool fast, const int *d_abort, bool tb = false);
+                         T alpha, T beta, bool fast, const int *d_abort, bool tb = false, bool ta = false);
 // C := alpha A B^T + beta C for the SPD generator (same code as the update,
 // separate kernel symbol).
 hipError_t launch_gen_gemm(hipStream_t s, const UpdOp<double> *ops, int nops, int max_m, int max_n, double alpha,
