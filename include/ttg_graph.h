@@ -25,6 +25,19 @@ extern "C" {
  * the process instead, potrf.h:183-187).  Returns 0, -1 on a bad argument. */
 int ttg_graph_dpotrf(int N, int NB, double *tiles, long *stats, int *info);
 
+/* The POINV graphs of examples/potrf/trtri_L.h, lauum.h and potri.h as
+ * device-task graphs over the tiles (load -> graph -> result writer, then
+ * fence), with stats as above:
+ *   ttg_graph_dtrtri  L := L^{-1} (lower, non-unit); *info (optional): 0 or
+ *                     LAPACK's info of an exactly zero diagonal entry (the
+ *                     graph then stops; the reference asserts, trtri_L.h:33)
+ *   ttg_graph_dlauum  A := L^T L (lower)
+ *   ttg_graph_dpotri  A := (L L^T)^{-1} from the factor L (TRTRI then LAUUM)
+ * Return 0, -1 on a bad argument. */
+int ttg_graph_dtrtri(int N, int NB, double *tiles, long *stats, int *info);
+int ttg_graph_dlauum(int N, int NB, double *tiles, long *stats);
+int ttg_graph_dpotri(int N, int NB, double *tiles, long *stats, int *info);
+
 /* PLGSY(N, bump, seed) into the tiles with one device task per tile (the
  * PLGSY template task of examples/potrf/plgsy.h:7-50; values bit-identical
  * to CORE_dplgsy).  Returns 0, -1 on a bad argument. */

@@ -194,3 +194,15 @@ def test_poinv_driver_check_and_lines():
     kinds = [l.split("nthreads= ")[1].split()[1] for l in r.stdout.splitlines()
              if l.startswith("POINV (POTRF+TRTRI+LAUUM)")]
     assert kinds == ["POTRF", "TRTRI", "LAUUM", "TTG"]
+
+
+@pytest.mark.parametrize("seq,kind,n_lines", [(0, "POTRF+POTRI", 3), (1, "POTRF+TRTRI+LAUUM", 4), (2, "POINV", 1)])
+def test_poinv_driver_task_graphs(seq, kind, n_lines):
+    """bin/testing_dpoinv -graph: the reference's POTRF, POTRI, TRTRI and
+    LAUUM task graphs on the TTG runtime (device tasks over a host matrix),
+    the inversion check and the reference driver's lines, for each -seq."""
+    r = _run_poinv_driver(["-N", "700", "-t", "64", "-seq", str(seq), "-nruns", "1", "-graph"])
+    assert r.returncode == 0
+    assert "Inversion is CORRECT" in r.stdout
+    lines = [l for l in r.stdout.splitlines() if l.startswith(f"POINV ({kind})")]
+    assert len(lines) == n_lines and all("TTG Execution Time (milliseconds) : " in l for l in lines)

@@ -4,11 +4,50 @@
 #include <cstring>
 
 #include "ttg_graph.h"
+#include "lauum.h"
 #include "plgsy.h"
 #include "pmw.h"
 #include "potrf.h"
+#include "potri.h"
+#include "trtri_L.h"
 #include "result.h"
 #include "ttg.h"
+
+namespace {
+
+// load -> graph -> result writer over the host tiles, then fence; the
+// statistics and the info of a stopped graph as ttg_graph_dpotrf's
+template <typename MakeGraph>
+int run_tile_graph(int N, int NB, double *tiles, long *stats, int *info, MakeGraph &&make_graph) {
+  if (N <= 0 || NB <= 0 || NB > TTG_CHOL_MAX_TILE || !tiles) return -1;
+  ttg::initialize(1, nullptr, -1);
+  ttg::clear_cancel();
+  auto &rt = ttg::detail::Runtime::get();
+  const long t0 = rt.tasks_run, r0 = rt.rounds, k0 = rt.kernel_launches;
+  rt.max_round = 0;
+  {
+    MatrixT<double> A(N, NB, tiles);
+    ttg::Edge<Key2, MatrixTile<double>> to_graph("To graph"), to_result("To Result");
+    auto load = make_load_tt(A, to_graph, true);
+    auto graph = make_graph(A, to_graph, to_result);
+    auto result_ttg = make_result_ttg(A, to_result, true);
+    ttg::make_graph_executable(load.get());
+    load->invoke();
+    ttg::execute();
+    ttg::fence();
+  }
+  if (stats) {
+    stats[0] = rt.tasks_run - t0;
+    stats[1] = rt.rounds - r0;
+    stats[2] = rt.kernel_launches - k0;
+    stats[3] = rt.max_round;
+  }
+  if (info) *info = ttg::cancelled() ? (int)ttg::cancel_info() : 0;
+  ttg::clear_cancel();
+  return 0;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -45,6 +84,29 @@ int ttg_graph_dpotrf(int N, int NB, double *tiles, long *stats, int *info) {
   if (info) *info = ttg::cancelled() ? (int)ttg::cancel_info() : 0;
   ttg::clear_cancel();
   return 0;
+}
+
+
+// L := L^{-1} on the lower tiles with the TRTRI graph (trtri_L.h).  *info: 0
+// or LAPACK's info of an exactly singular diagonal.
+int ttg_graph_dtrtri(int N, int NB, double *tiles, long *stats, int *info) {
+  return run_tile_graph(N, NB, tiles, stats, info, [](auto &A, auto &in, auto &out) {
+    return trtri_lower::make_trtri_ttg(A, in, out, true);
+  });
+}
+
+// A := L^T L on the lower tiles with the LAUUM graph (lauum.h).
+int ttg_graph_dlauum(int N, int NB, double *tiles, long *stats) {
+  return run_tile_graph(N, NB, tiles, stats, nullptr, [](auto &A, auto &in, auto &out) {
+    return lauum::make_lauum_ttg(A, in, out, true);
+  });
+}
+
+// A^{-1} from the factor L (lower tiles) with the POTRI graph (potri.h).
+int ttg_graph_dpotri(int N, int NB, double *tiles, long *stats, int *info) {
+  return run_tile_graph(N, NB, tiles, stats, info, [](auto &A, auto &in, auto &out) {
+    return potri::make_potri_ttg(A, in, out, true);
+  });
 }
 
 // Generate PLGSY(N, bump, seed) into `tiles` (host) with the device-task

@@ -14,6 +14,11 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "ttg_graph.h")
 SIGNATURES = {
     "ttg_graph_dpotrf": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p]),
+    "ttg_graph_dtrtri": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p]),
+    "ttg_graph_dlauum": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "ttg_graph_dpotri": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p]),
     "ttg_graph_dplgsy": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_ulong,
                                         ctypes.c_void_p]),
 }
@@ -60,6 +65,37 @@ def dpotrf(N, NB, tiles):
     if rc != 0:
         raise ValueError(f"ttg_graph_dpotrf(N={N}, NB={NB}) rejected its arguments")
     return {"tasks": stats[0], "rounds": stats[1], "launches": stats[2], "max_round": stats[3], "info": info.value}
+
+
+def _stats(stats, info=None):
+    d = {"tasks": stats[0], "rounds": stats[1], "launches": stats[2], "max_round": stats[3]}
+    if info is not None:
+        d["info"] = info.value
+    return d
+
+
+def dtrtri(N, NB, tiles):
+    """L := L^{-1} on host lower tiles with the TRTRI task graph."""
+    stats, info = (ctypes.c_long * 4)(), ctypes.c_int(-1)
+    if lib().ttg_graph_dtrtri(N, NB, _tiles(tiles, N, NB), ctypes.addressof(stats), ctypes.addressof(info)) != 0:
+        raise ValueError(f"ttg_graph_dtrtri(N={N}, NB={NB}) rejected its arguments")
+    return _stats(stats, info)
+
+
+def dlauum(N, NB, tiles):
+    """A := L^T L on host lower tiles with the LAUUM task graph."""
+    stats = (ctypes.c_long * 4)()
+    if lib().ttg_graph_dlauum(N, NB, _tiles(tiles, N, NB), ctypes.addressof(stats)) != 0:
+        raise ValueError(f"ttg_graph_dlauum(N={N}, NB={NB}) rejected its arguments")
+    return _stats(stats)
+
+
+def dpotri(N, NB, tiles):
+    """A := (L L^T)^{-1} from the factor in host lower tiles with the POTRI task graph."""
+    stats, info = (ctypes.c_long * 4)(), ctypes.c_int(-1)
+    if lib().ttg_graph_dpotri(N, NB, _tiles(tiles, N, NB), ctypes.addressof(stats), ctypes.addressof(info)) != 0:
+        raise ValueError(f"ttg_graph_dpotri(N={N}, NB={NB}) rejected its arguments")
+    return _stats(stats, info)
 
 
 def dplgsy(N, NB, bump, seed, tiles):
