@@ -143,6 +143,22 @@ int ttg_chol_dgemm_nt(void *stream, int m, int n, int k, double alpha, const dou
 int ttg_chol_dnrm2(void *stream, int n, const double *x, double *d_result);
 
 /* ---------------------------------------------------------------- */
+/* per-tile operations on FP32 tiles                                 */
+/* ---------------------------------------------------------------- */
+/* The four operations above for float tiles: what the reference's task
+ * bodies, templated on the element type (examples/potrf/potrf.h:90-631),
+ * call for MatrixT<float> (hipsolverSpotrf, hipblasStrsm, hipblasSsyrk,
+ * hipblasSgemm).  POTRF factors in FP64 internally (a workspace from the
+ * stream's or the group's work area) and rounds the factor to FP32;
+ * *d_info as for ttg_chol_dpotrf_tile. */
+int ttg_chol_spotrf_tile(void *stream, int n, float *A, int lda, int *d_info, int info_offset);
+int ttg_chol_strsm_rlt(void *stream, int m, int n, const float *L, int ldl, float *B, int ldb);
+int ttg_chol_ssyrk_ln(void *stream, int n, int k, float alpha, const float *A, int lda, float beta, float *C,
+                      int ldc);
+int ttg_chol_sgemm_nt(void *stream, int m, int n, int k, float alpha, const float *A, int lda, const float *B,
+                      int ldb, float beta, float *C, int ldc);
+
+/* ---------------------------------------------------------------- */
 /* per-tile operations of the POINV graphs (FP64)                    */
 /* ---------------------------------------------------------------- */
 /* The tile operations the reference's TRTRI and LAUUM graphs call on the
@@ -232,6 +248,15 @@ int ttg_chol_dpotrf_tiled(void *stream, int N, int NB, double *tiles, int *d_inf
  * (B is N x nrhs, leading dimension ldb, overwritten by X).
  * (No counterpart in the reference: POTRS for the iterative refinement.) */
 int ttg_chol_dpotrs_tiled(void *stream, int N, int NB, const double *tiles, int nrhs, double *B, int ldb);
+
+/* B := (L L^T)^{-1} B with an FP32 factor L (lower tiles, FP64 right-hand
+ * sides): the refinement solve of a mixed-precision factorization made
+ * elsewhere (for instance by the task graph over float tiles).  The sweeps
+ * run in FP32 on B rounded to FP32; the result is widened back.  The tiles
+ * must be square NB x NB blocks: the padding of the last tile row and
+ * column of the diagonal tile identity, the rest zero (ttg_chol_dpotrf_tiled
+ * leaves it so).  No reference counterpart. */
+int ttg_chol_spotrs_tiled(void *stream, int N, int NB, const float *tiles, int nrhs, double *B, int ldb);
 
 /* R := B - A X, with A symmetric given by its lower tiles (FP64). */
 int ttg_chol_dresidual_tiled(void *stream, int N, int NB, const double *tiles, int nrhs, const double *X,

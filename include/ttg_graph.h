@@ -38,6 +38,19 @@ int ttg_graph_dtrtri(int N, int NB, double *tiles, long *stats, int *info);
 int ttg_graph_dlauum(int N, int NB, double *tiles, long *stats);
 int ttg_graph_dpotri(int N, int NB, double *tiles, long *stats, int *info);
 
+/* Mixed-precision SPD solve whose factorization runs through the task
+ * graph: the potrf graph over FP32 tiles (device tasks on libttg_chol's S
+ * tile kernels, the reference's bodies instantiated for float), then FP64
+ * iterative refinement on the device (FP64 residual, FP32-factor solves)
+ * until max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf) <=
+ * tol or max_iter corrections.  A: host FP64 lower tiles (unchanged); B and
+ * X: host N x nrhs column-major (leading dimensions ldb, ldx).  *iters
+ * (optional): corrections applied; *berr (optional): final backward error;
+ * *info (optional): 0, or LAPACK's info of the FP32 factorization (X is then
+ * not computed).  Returns 0, -1 on a bad argument, -2 on a device error. */
+int ttg_graph_dposv_mixed(int N, int NB, const double *A, int nrhs, const double *B, int ldb, double *X, int ldx,
+                          double tol, int max_iter, int *iters, double *berr, int *info);
+
 /* PLGSY(N, bump, seed) into the tiles with one device task per tile (the
  * PLGSY template task of examples/potrf/plgsy.h:7-50; values bit-identical
  * to CORE_dplgsy).  Returns 0, -1 on a bad argument. */

@@ -183,3 +183,34 @@ def test_graph_factor_independent_of_round_sizes(oracle, tmp_path):
             a = oracle.tile_view(got, N, NB, m, n)
             b = oracle.tile_view(one, N, NB, m, n)
             assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), (m, n)
+
+
+@pytest.mark.parametrize("N,NB,kappa", [(2048, 256, 1e4), (1000, 128, 1e3), (4096, 512, 1e5)])
+def test_graph_mixed_solve(oracle, N, NB, kappa):
+    """Mixed precision through the task graph: the potrf graph over FP32
+    tiles (device tasks on the S tile kernels) and FP64 refinement reach the
+    FP64 backward error bound 1e-12, checked by the oracle; ragged N
+    exercises the padding of the FP32 factor."""
+    import ttg_amd.graph as G
+
+    sigma2 = sum(k * k for k in range(-127, 128)) / 255.0 / 127.0 ** 2
+    A0 = oracle.spd_tiles(N, NB, 4 * sigma2 / (kappa - 1), 31)
+    B = np.random.default_rng(3).uniform(-1, 1, (N, 4))
+    print(f"graph mixed solve N={N} NB={NB} kappa={kappa:g}", flush=True)
+    X, st = G.dposv_mixed(N, NB, A0.copy(), B, tol=1e-12, max_iter=30)
+    assert st["info"] == 0, st
+    assert 1 <= st["iters"] <= 30 and st["berr"] <= 1e-12, st
+    assert oracle.solve_backward_error(N, NB, A0, X, B) <= 1e-12
+
+
+def test_graph_mixed_solve_reports_info(oracle):
+    """A matrix that is not positive definite: the FP32 graph stops with
+    LAPACK's info and nothing is solved."""
+    import ttg_amd.graph as G
+
+    N, NB, col = 1000, 128, 300
+    A0 = oracle.spd_tiles(N, NB, 1.0, 5)
+    t, i = (col - 1) // NB, (col - 1) % NB
+    oracle.tile_view(A0, N, NB, t, t)[i, i] = -1.0
+    _, st = G.dposv_mixed(N, NB, A0, np.ones((N, 1)))
+    assert st["info"] == col, st

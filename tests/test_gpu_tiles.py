@@ -268,3 +268,53 @@ def test_plgsy_tiles_in_group_match_single_launches(dev, oracle):
         assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), (m, n)
         ref = oracle.tile_view(want, N, NB, m, n)[:mb, :nb]
         assert np.array_equal(a.view(np.uint64), ref.view(np.uint64)), (m, n)
+
+
+def test_float_tile_ops_against_fp64_oracle(dev, oracle):
+    """The FP32 tile operations of the graph over float tiles: GEMM and SYRK
+    bitwise on small integers (exact in FP32), POTRF and TRSM within FP32
+    rounding of the FP64 oracle (8 n eps_32 relative)."""
+    import ttg_amd.chol as C
+
+    rng = np.random.default_rng(4)
+    n, k = 256, 200
+
+    def cm32(a):
+        return torch.from_numpy(np.ascontiguousarray(np.asarray(a, dtype=np.float32).T)).to(dev).t()
+
+    def host(t):
+        torch.cuda.synchronize()
+        return t.t().cpu().numpy().T.astype(np.float64)
+
+    lib = C.lib()
+    A = rng.integers(-8, 9, size=(n, k)).astype(np.float64)
+    Bm = rng.integers(-8, 9, size=(n, k)).astype(np.float64)
+    C0 = rng.integers(-8, 9, size=(n, n)).astype(np.float64)
+    dA, dB, dC = cm32(A), cm32(Bm), cm32(C0)
+    assert lib.ttg_chol_sgemm_nt(None, n, n, k, -1.0, dA.data_ptr(), n, dB.data_ptr(), n, 1.0, dC.data_ptr(), n) == 0
+    assert np.array_equal(host(dC), C0 - A @ Bm.T)
+    dC = cm32(C0)
+    assert lib.ttg_chol_ssyrk_ln(None, n, k, -1.0, dA.data_ptr(), n, 1.0, dC.data_ptr(), n) == 0
+    got = host(dC)
+    assert np.array_equal(np.tril(got), np.tril(C0 - A @ A.T))
+    assert np.array_equal(np.triu(got, 1), np.triu(C0, 1))
+    # POTRF and TRSM
+    M = rng.uniform(-1, 1, (n, n))
+    S = M @ M.T / n + np.eye(n)
+    S32 = S.astype(np.float32).astype(np.float64)
+    want = np.asfortranarray(S32.copy())
+    assert oracle.lib().oracle_dpotrf_tile(n, oracle.ptr(want), n) == 0
+    L64 = np.tril(want)
+    dS = cm32(S32)
+    info = torch.full((1,), -3, dtype=torch.int32, device=dev)
+    assert lib.ttg_chol_spotrf_tile(None, n, dS.data_ptr(), n, info.data_ptr(), 0) == 0
+    L32 = np.tril(host(dS))
+    assert int(info.item()) == 0
+    eps32 = np.finfo(np.float32).eps
+    assert np.abs(L32 - L64).max() <= 8 * n * eps32 * np.abs(L64).max()
+    Bt = rng.uniform(-1, 1, (100, n))
+    dBt = cm32(Bt)
+    dL = cm32(L32)
+    assert lib.ttg_chol_strsm_rlt(None, 100, n, dL.data_ptr(), n, dBt.data_ptr(), 100) == 0
+    X = host(dBt)
+    assert np.abs(X @ L32.T - Bt.astype(np.float32)).max() <= 8 * n * eps32 * np.abs(Bt).max() * np.abs(L32).max()

@@ -40,6 +40,11 @@ SIGNATURES = {
     "ttg_chol_dtrtri_tiled": (_i, [_p, _i, _i, _p, _p]),
     "ttg_chol_release_plans": (_i, []),
     "ttg_chol_set_knob": (_i, [_i, _i]),
+    "ttg_chol_spotrf_tile": (_i, [_p, _i, _p, _i, _p, _i]),
+    "ttg_chol_strsm_rlt": (_i, [_p, _i, _i, _p, _i, _p, _i]),
+    "ttg_chol_ssyrk_ln": (_i, [_p, _i, _i, ctypes.c_float, _p, _i, ctypes.c_float, _p, _i]),
+    "ttg_chol_sgemm_nt": (_i, [_p, _i, _i, _i, ctypes.c_float, _p, _i, _p, _i, ctypes.c_float, _p, _i]),
+    "ttg_chol_spotrs_tiled": (_i, [_p, _i, _i, _p, _i, _p, _i]),
     "ttg_chol_dtrtri_tile": (_i, [_p, _i, _p, _i, _p, _i]),
     "ttg_chol_dtrsm_lln": (_i, [_p, _i, _i, _d, _p, _i, _p, _i]),
     "ttg_chol_dtrsm_rln": (_i, [_p, _i, _i, _d, _p, _i, _p, _i]),

@@ -140,6 +140,11 @@ struct Group {
   std::vector<TopOp> pre, post;
   std::vector<uint8_t> pre_rel, post_rel;
   std::vector<DiagZeroOp> zero;
+  // FP32 tiles (the graph over float tiles): POTRF (its FP64 workspace pw in
+  // the work area), TRSM B := B L^{-T} with its inverse-block jobs, updates
+  std::vector<PotrfOpT<float>> potrf_f, trsm_f_dinv;
+  std::vector<TrsmOp<float>> trsm_f;
+  std::map<UpdKey, std::vector<UpdOp<float>>> upd_f;
   size_t work = 0;  // bytes of work tiles
   struct Norm {
     int n;
@@ -170,6 +175,10 @@ struct Group {
     pre_rel.clear();
     post_rel.clear();
     zero.clear();
+    potrf_f.clear();
+    trsm_f_dinv.clear();
+    trsm_f.clear();
+    upd_f.clear();
     work = 0;
     norm_after.clear();
     writes.clear();
@@ -321,10 +330,16 @@ static int run_group(Group &g, hipStream_t s) {
   const size_t o_trsm_n = place(g.trsm_n.size() * sizeof(TrsmOp<double>));
   std::vector<size_t> o_upd;
   for (auto &kv : g.upd) o_upd.push_back(place(kv.second.size() * sizeof(UpdOp<double>)));
+  const size_t o_potrf_f = place(g.potrf_f.size() * sizeof(PotrfOpT<float>));
+  const size_t o_dinvops_f = place(g.trsm_f_dinv.size() * sizeof(PotrfOpT<float>));
+  const size_t o_trsm_f = place(g.trsm_f.size() * sizeof(TrsmOp<float>));
+  std::vector<size_t> o_upd_f;
+  for (auto &kv : g.upd_f) o_upd_f.push_back(place(kv.second.size() * sizeof(UpdOp<float>)));
   const size_t desc_bytes = off;  // descriptors end here: copied from the host
   size_t dinv_bytes = 0;
   for (auto &d : g.trsm_dinv) dinv_bytes += sizeof(double) * dinv_doubles(d.n);
   for (auto &d : g.trsm_n_dinv) dinv_bytes += sizeof(double) * dinv_doubles(d.n);
+  for (auto &d : g.trsm_f_dinv) dinv_bytes += sizeof(float) * dinv_doubles(d.n);
   const size_t dinv_off = place(dinv_bytes);
   const size_t work_off = place(g.work);
   if (off == 0) {
@@ -352,7 +367,13 @@ static int run_group(Group &g, hipStream_t s) {
       if (g.trsm_n_rel[i] & kRelB) g.trsm_n[i].B = reloc(g.trsm_n[i].B);
       p += sizeof(double) * dinv_doubles(g.trsm_n_dinv[i].n);
     }
+    for (size_t i = 0; i < g.trsm_f.size(); ++i) {
+      g.trsm_f_dinv[i].dinv = reinterpret_cast<float *>(base + p);
+      g.trsm_f[i].dinv = g.trsm_f_dinv[i].dinv;
+      p += sizeof(float) * dinv_doubles(g.trsm_f_dinv[i].n);
+    }
   }
+  for (auto &o : g.potrf_f) o.pw = reloc(o.pw);
   for (size_t i = 0; i < g.pre.size(); ++i) {
     if (g.pre_rel[i] & kRelA) g.pre[i].src = reloc(g.pre[i].src);
     if (g.pre_rel[i] & kRelB) g.pre[i].dst = reloc(g.pre[i].dst);
@@ -385,6 +406,13 @@ static int run_group(Group &g, hipStream_t s) {
   {
     size_t i = 0;
     for (auto &kv : g.upd) put(o_upd[i++], kv.second.data(), kv.second.size() * sizeof(UpdOp<double>));
+  }
+  put(o_potrf_f, g.potrf_f.data(), g.potrf_f.size() * sizeof(PotrfOpT<float>));
+  put(o_dinvops_f, g.trsm_f_dinv.data(), g.trsm_f_dinv.size() * sizeof(PotrfOpT<float>));
+  put(o_trsm_f, g.trsm_f.data(), g.trsm_f.size() * sizeof(TrsmOp<float>));
+  {
+    size_t i = 0;
+    for (auto &kv : g.upd_f) put(o_upd_f[i++], kv.second.data(), kv.second.size() * sizeof(UpdOp<float>));
   }
   if (desc_bytes) {
     TTGC_HIP(hipMemcpyAsync(base, host, desc_bytes, hipMemcpyHostToDevice, s));
@@ -453,6 +481,42 @@ static int run_group(Group &g, hipStream_t s) {
       TTGC_HIP(launch_update<double>(s, reinterpret_cast<UpdOp<double> *>(base + o_upd[i]), (int)kv.second.size(),
                                      unone, max_m, max_n, kv.first.alpha, kv.first.beta, fast, nullptr, kv.first.tb,
                                      kv.first.ta));
+      ++launches;
+      ++i;
+    }
+  }
+  // FP32 tiles
+  if (!g.potrf_f.empty()) {
+    PotrfOpT<float> fnone{};
+    TTGC_HIP(launch_potrf<float>(s, reinterpret_cast<PotrfOpT<float> *>(base + o_potrf_f), (int)g.potrf_f.size(),
+                                 fnone, nullptr));
+    ++launches;
+  }
+  if (!g.trsm_f.empty()) {
+    PotrfOpT<float> fnone{};
+    TTGC_HIP(launch_dinv<float>(s, reinterpret_cast<PotrfOpT<float> *>(base + o_dinvops_f), (int)g.trsm_f_dinv.size(),
+                                fnone));
+    int max_m = 0;
+    for (auto &t : g.trsm_f) max_m = std::max(max_m, t.m);
+    TrsmOp<float> tfnone{};
+    TTGC_HIP(launch_trsm<float>(s, reinterpret_cast<TrsmOp<float> *>(base + o_trsm_f), (int)g.trsm_f.size(), tfnone,
+                                max_m, nullptr, true));
+    launches += 2;
+  }
+  {
+    size_t i = 0;
+    for (auto &kv : g.upd_f) {
+      int max_m = 0, max_n = 0;
+      bool fast = true;
+      for (auto &u : kv.second) {
+        max_m = std::max(max_m, u.m);
+        max_n = std::max(max_n, u.n);
+      }
+      for (auto &u : kv.second) fast = fast && update_fast(u, max_m);
+      UpdOp<float> unone{};
+      TTGC_HIP(launch_update<float>(s, reinterpret_cast<UpdOp<float> *>(base + o_upd_f[i]), (int)kv.second.size(),
+                                    unone, max_m, max_n, (float)kv.first.alpha, (float)kv.first.beta, fast, nullptr,
+                                    false, false));
       ++launches;
       ++i;
     }
@@ -651,6 +715,78 @@ int ttg_chol_dgemm_nt(void *stream, int m, int n, int k, double alpha, const dou
                       int ldb, double beta, double *C, int ldc) {
   clear_error();
   return update_entry("dgemm_nt", as_stream(stream), m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 0);
+}
+
+// ---- FP32 tiles: the potrf graph over float tiles (the reference's bodies
+// are templated on the element type, potrf.h:90-631, and call the S
+// variants of the vendor routines for float)
+int ttg_chol_spotrf_tile(void *stream, int n, float *A, int lda, int *d_info, int info_offset) {
+  clear_error();
+  if (n < 0 || n > TTG_CHOL_MAX_TILE || lda < std::max(1, n) || (n > 0 && !A)) {
+    set_error("spotrf_tile: bad argument (n=%d lda=%d)", n, lda);
+    return TTG_CHOL_EINVAL;
+  }
+  hipStream_t s = as_stream(stream);
+  auto rec = [&](Group &g) {
+    PotrfOpT<float> op{A, nullptr, d_info, n, lda, info_offset, 1, g.work_tile(std::max(n, 1), std::max(n, 1))};
+    g.potrf_f.push_back(op);
+  };
+  if (n == 0 && !d_info) return TTG_CHOL_OK;
+  if (grouping(s)) {
+    rec(g_group);
+    return TTG_CHOL_OK;
+  }
+  return run_alone(s, rec);
+}
+
+int ttg_chol_strsm_rlt(void *stream, int m, int n, const float *L, int ldl, float *B, int ldb) {
+  clear_error();
+  if (m < 0 || n < 0 || n > TTG_CHOL_MAX_TILE || ldl < std::max(1, n) || ldb < std::max(1, m) ||
+      (m > 0 && n > 0 && (!L || !B))) {
+    set_error("strsm_rlt: bad argument (m=%d n=%d ldl=%d ldb=%d)", m, n, ldl, ldb);
+    return TTG_CHOL_EINVAL;
+  }
+  if (m == 0 || n == 0) return TTG_CHOL_OK;
+  hipStream_t s = as_stream(stream);
+  auto rec = [&](Group &g) {
+    g.trsm_f.push_back(TrsmOp<float>{L, B, nullptr, m, n, ldl, ldb});
+    g.trsm_f_dinv.push_back(PotrfOpT<float>{const_cast<float *>(L), nullptr, nullptr, n, ldl, 0, 0, nullptr});
+  };
+  if (grouping(s)) {
+    rec(g_group);
+    return TTG_CHOL_OK;
+  }
+  return run_alone(s, rec);
+}
+
+static int update_f_entry(const char *name, hipStream_t s, int m, int n, int k, float alpha, const float *A, int lda,
+                          const float *B, int ldb, float beta, float *C, int ldc, int lower) {
+  if (m < 0 || n < 0 || k < 0 || lda < std::max(1, m) || ldb < std::max(1, n) || ldc < std::max(1, m) ||
+      (m > 0 && n > 0 && (!C || (k > 0 && (!A || !B))))) {
+    set_error("%s: bad argument (m=%d n=%d k=%d lda=%d ldb=%d ldc=%d)", name, m, n, k, lda, ldb, ldc);
+    return TTG_CHOL_EINVAL;
+  }
+  if (m == 0 || n == 0) return TTG_CHOL_OK;
+  if (k == 0) alpha = 0.0f;
+  const UpdOp<float> op{A, B, C, m, n, k, lda, ldb, ldc, lower};
+  const UpdKey key{alpha, beta, false, false};
+  if (grouping(s)) {
+    g_group.upd_f[key].push_back(op);
+    return TTG_CHOL_OK;
+  }
+  return run_alone(s, [&](Group &g) { g.upd_f[key].push_back(op); });
+}
+
+int ttg_chol_ssyrk_ln(void *stream, int n, int k, float alpha, const float *A, int lda, float beta, float *C,
+                      int ldc) {
+  clear_error();
+  return update_f_entry("ssyrk_ln", as_stream(stream), n, n, k, alpha, A, lda, A, lda, beta, C, ldc, 1);
+}
+
+int ttg_chol_sgemm_nt(void *stream, int m, int n, int k, float alpha, const float *A, int lda, const float *B,
+                      int ldb, float beta, float *C, int ldc) {
+  clear_error();
+  return update_f_entry("sgemm_nt", as_stream(stream), m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 0);
 }
 
 // ---- POINV tile operations (examples/potrf/trtri_L.h, lauum.h)

@@ -610,6 +610,34 @@ int ttg_chol_dpotrs_tiled(void *stream, int N, int NB, const double *tiles, int 
   return TTG_CHOL_OK;
 }
 
+int ttg_chol_spotrs_tiled(void *stream, int N, int NB, const float *tiles, int nrhs, double *B, int ldb) {
+  clear_error();
+  if (N < 0 || NB <= 0 || NB > TTG_CHOL_MAX_TILE || NB % 32 != 0 || nrhs < 0 || ldb < std::max(1, N) ||
+      (N > 0 && nrhs > 0 && (!tiles || !B))) {
+    set_error("spotrs_tiled: bad argument (N=%d NB=%d nrhs=%d ldb=%d)", N, NB, nrhs, ldb);
+    return TTG_CHOL_EINVAL;
+  }
+  if (N == 0 || nrhs == 0) return TTG_CHOL_OK;
+  if (nrhs > 32) {
+    for (int j0 = 0; j0 < nrhs; j0 += 32) {
+      int rc = ttg_chol_spotrs_tiled(stream, N, NB, tiles, std::min(32, nrhs - j0), B + (long)j0 * ldb, ldb);
+      if (rc != TTG_CHOL_OK) return rc;
+    }
+    return TTG_CHOL_OK;
+  }
+  hipStream_t s = as_stream(stream);
+  int rc = 0;
+  SolvePlan<float> *p = get_solve_plan<float>(N, NB, tiles, nrhs, &rc);
+  if (!p) return rc;
+  rc = p->prepare(s);
+  if (rc != TTG_CHOL_OK) return rc;
+  TTGC_HIP(launch_to_rows<double, float>(s, N, p->Np, nrhs, B, ldb, p->bt()));
+  rc = p->run_sweeps(s);
+  if (rc != TTG_CHOL_OK) return rc;
+  TTGC_HIP(launch_from_rows<float, double>(s, N, nrhs, p->bt(), B, ldb));
+  return TTG_CHOL_OK;
+}
+
 int ttg_chol_dresidual_tiled(void *stream, int N, int NB, const double *tiles, int nrhs, const double *X, int ldx,
                              const double *B, int ldb, double *R, int ldr) {
   clear_error();

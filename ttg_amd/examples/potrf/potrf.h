@@ -2,7 +2,8 @@
 // devreal/ttg) on the MI355X runtime: the same template tasks, keys,
 // successor keys, dispatch and DPLASMA priorities; the device bodies call
 // libttg_chol's gfx950 kernels (include/ttg_chol.h) on
-// ttg::device::current_stream() instead of hipSOLVER/hipBLAS.  The runtime's
+// ttg::device::current_stream() instead of hipSOLVER/hipBLAS, for double
+// or float tiles as the reference's templates do.  The runtime's
 // launch grouping turns all the tile calls of a scheduling round into a few
 // batched launches.
 // Built with DEBUG_TILES_VALUES, every task also takes the device 2-norm of
@@ -31,6 +32,36 @@ inline double FLOPS_DPOTRF(double n) { return FMULS_POTRF(n) + FADDS_POTRF(n); }
 
 inline void check_launch(int rc) {
   if (rc != TTG_CHOL_OK) ttg::detail::fatal(std::string("tile kernel: ") + ttg_chol_last_error());
+}
+
+// The tile kernels for the element type: the D routines for double tiles,
+// the S routines for float tiles (the reference's bodies are templated on
+// MatrixT::element_type and reach hipsolverSpotrf/hipblasS* for float).
+inline int tile_potrf(void *s, int n, double *A, int lda, int *info, int off) {
+  return ttg_chol_dpotrf_tile(s, n, A, lda, info, off);
+}
+inline int tile_potrf(void *s, int n, float *A, int lda, int *info, int off) {
+  return ttg_chol_spotrf_tile(s, n, A, lda, info, off);
+}
+inline int tile_trsm(void *s, int m, int n, const double *L, int ldl, double *B, int ldb) {
+  return ttg_chol_dtrsm_rlt(s, m, n, L, ldl, B, ldb);
+}
+inline int tile_trsm(void *s, int m, int n, const float *L, int ldl, float *B, int ldb) {
+  return ttg_chol_strsm_rlt(s, m, n, L, ldl, B, ldb);
+}
+inline int tile_syrk(void *s, int n, int k, double alpha, const double *A, int lda, double beta, double *C, int ldc) {
+  return ttg_chol_dsyrk_ln(s, n, k, alpha, A, lda, beta, C, ldc);
+}
+inline int tile_syrk(void *s, int n, int k, double alpha, const float *A, int lda, double beta, float *C, int ldc) {
+  return ttg_chol_ssyrk_ln(s, n, k, (float)alpha, A, lda, (float)beta, C, ldc);
+}
+inline int tile_gemm(void *s, int m, int n, int k, double alpha, const double *A, int lda, const double *B, int ldb,
+                     double beta, double *C, int ldc) {
+  return ttg_chol_dgemm_nt(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+inline int tile_gemm(void *s, int m, int n, int k, double alpha, const float *A, int lda, const float *B, int ldb,
+                     double beta, float *C, int ldc) {
+  return ttg_chol_sgemm_nt(s, m, n, k, (float)alpha, A, lda, B, ldb, (float)beta, C, ldc);
 }
 
 #ifdef DEBUG_TILES_VALUES
@@ -68,9 +99,8 @@ auto make_potrf(MatrixT &A, ttg::Edge<Key1, MatrixTile<typename MatrixT::element
 #else
     co_await ttg::device::select(tile_kk.buffer(), dev_info);
 #endif
-    check_launch(ttg_chol_dpotrf_tile(ttg::device::current_stream(), (int)tile_kk.cols(),
-                                      tile_kk.buffer().current_device_ptr(), (int)tile_kk.lda(),
-                                      dev_info.device_ptr(), 0));
+    check_launch(tile_potrf(ttg::device::current_stream(), (int)tile_kk.cols(), tile_kk.buffer().current_device_ptr(),
+                            (int)tile_kk.lda(), dev_info.device_ptr(), 0));
 #ifdef DEBUG_TILES_VALUES
     device_norm(tile_kk, norms_s.device_ptr() + 1);
     co_await ttg::device::wait(dev_info, norms_s);
@@ -125,9 +155,9 @@ auto make_trsm(MatrixT &A, ttg::Edge<Key2, MatrixTile<typename MatrixT::element_
 #else
     co_await ttg::device::select(tile_kk.buffer(), tile_mk.buffer());
 #endif
-    check_launch(ttg_chol_dtrsm_rlt(ttg::device::current_stream(), (int)tile_mk.rows(), (int)tile_mk.cols(),
-                                    tile_kk.buffer().current_device_ptr(), (int)tile_kk.lda(),
-                                    tile_mk.buffer().current_device_ptr(), (int)tile_mk.lda()));
+    check_launch(tile_trsm(ttg::device::current_stream(), (int)tile_mk.rows(), (int)tile_mk.cols(),
+                           tile_kk.buffer().current_device_ptr(), (int)tile_kk.lda(),
+                           tile_mk.buffer().current_device_ptr(), (int)tile_mk.lda()));
 #ifdef DEBUG_TILES_VALUES
     device_norm(tile_mk, norms_s.device_ptr() + 2);
     co_await ttg::device::wait(norms_s);
@@ -167,9 +197,9 @@ auto make_syrk(MatrixT &A, ttg::Edge<Key2, MatrixTile<typename MatrixT::element_
 #else
     co_await ttg::device::select(tile_kk.buffer(), tile_mk.buffer());
 #endif
-    check_launch(ttg_chol_dsyrk_ln(ttg::device::current_stream(), (int)tile_mk.rows(), (int)tile_mk.cols(), -1.0,
-                                   tile_mk.buffer().current_device_ptr(), (int)tile_mk.lda(), 1.0,
-                                   tile_kk.buffer().current_device_ptr(), (int)tile_kk.lda()));
+    check_launch(tile_syrk(ttg::device::current_stream(), (int)tile_mk.rows(), (int)tile_mk.cols(), -1.0,
+                           tile_mk.buffer().current_device_ptr(), (int)tile_mk.lda(), 1.0,
+                           tile_kk.buffer().current_device_ptr(), (int)tile_kk.lda()));
 #ifdef DEBUG_TILES_VALUES
     device_norm(tile_kk, norms_s.device_ptr() + 2);
     co_await ttg::device::wait(norms_s);
@@ -212,10 +242,10 @@ auto make_gemm(MatrixT &A, ttg::Edge<Key3, MatrixTile<typename MatrixT::element_
 #else
     co_await ttg::device::select(tile_mk.buffer(), tile_nk.buffer(), tile_mn.buffer());
 #endif
-    check_launch(ttg_chol_dgemm_nt(ttg::device::current_stream(), (int)tile_mk.rows(), (int)tile_nk.rows(),
-                                   (int)tile_nk.cols(), -1.0, tile_mk.buffer().current_device_ptr(),
-                                   (int)tile_mk.lda(), tile_nk.buffer().current_device_ptr(), (int)tile_nk.lda(), 1.0,
-                                   tile_mn.buffer().current_device_ptr(), (int)tile_mn.lda()));
+    check_launch(tile_gemm(ttg::device::current_stream(), (int)tile_mk.rows(), (int)tile_nk.rows(),
+                           (int)tile_nk.cols(), -1.0, tile_mk.buffer().current_device_ptr(), (int)tile_mk.lda(),
+                           tile_nk.buffer().current_device_ptr(), (int)tile_nk.lda(), 1.0,
+                           tile_mn.buffer().current_device_ptr(), (int)tile_mn.lda()));
 #ifdef DEBUG_TILES_VALUES
     device_norm(tile_mn, norms_s.device_ptr() + 3);
     co_await ttg::device::wait(norms_s);

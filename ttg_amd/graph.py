@@ -19,6 +19,10 @@ SIGNATURES = {
     "ttg_graph_dlauum": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     "ttg_graph_dpotri": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p]),
+    "ttg_graph_dposv_mixed": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                             ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                             ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_void_p]),
     "ttg_graph_dplgsy": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_ulong,
                                         ctypes.c_void_p]),
 }
@@ -96,6 +100,27 @@ def dpotri(N, NB, tiles):
     if lib().ttg_graph_dpotri(N, NB, _tiles(tiles, N, NB), ctypes.addressof(stats), ctypes.addressof(info)) != 0:
         raise ValueError(f"ttg_graph_dpotri(N={N}, NB={NB}) rejected its arguments")
     return _stats(stats, info)
+
+
+def dposv_mixed(N, NB, tiles, B, tol=1e-12, max_iter=30):
+    """Solve A X = B (A: host FP64 lower tiles, B: N x nrhs) with the FP32
+    factorization run by the task graph and FP64 refinement; returns
+    (X, {"iters", "berr", "info"})."""
+    B = np.asfortranarray(B, dtype=np.float64)
+    if B.ndim == 1:
+        B = B.reshape(-1, 1, order="F")
+    if B.shape[0] != N:
+        raise ValueError(f"B has {B.shape[0]} rows, expected N={N}")
+    X = np.zeros_like(B, order="F")
+    it, be, info = ctypes.c_int(-1), ctypes.c_double(float("nan")), ctypes.c_int(-1)
+    rc = lib().ttg_graph_dposv_mixed(N, NB, _tiles(tiles, N, NB), B.shape[1], B.ctypes.data, N, X.ctypes.data, N,
+                                     float(tol), int(max_iter), ctypes.addressof(it), ctypes.addressof(be),
+                                     ctypes.addressof(info))
+    if rc == -1:
+        raise ValueError(f"ttg_graph_dposv_mixed(N={N}, NB={NB}) rejected its arguments")
+    if rc != 0:
+        raise RuntimeError(f"ttg_graph_dposv_mixed failed on the device ({rc})")
+    return X, {"iters": it.value, "berr": be.value, "info": info.value}
 
 
 def dplgsy(N, NB, bump, seed, tiles):
