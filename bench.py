@@ -54,9 +54,17 @@ def parse():
     p.add_argument("--nb", type=int, default=512)
     p.add_argument("--nrhs", type=int, default=None)
     p.add_argument("--kappa", type=float, default=None)
-    p.add_argument("--mode", choices=["f16x2", "fp32", "fp64", "f16", "bf16"], default=None,
-                   help="f16x2: FP32 factor with split-FP16 MFMA trailing updates + FP64 refinement (the metric); "
-                        "fp32: FP32 MFMA updates; fp64: plain FP64 factor")
+    p.add_argument("--mode", choices=["auto", "f16x2", "fp32", "fp64", "f16", "bf16"], default=None,
+                   help="auto: the autotuner picks the fastest of f16x2 + refinement, f16 + refinement and f16 + "
+                        "CG that reaches --tol (the metric's default); f16x2: FP32 factor with split-FP16 MFMA "
+                        "trailing updates; f16 / bf16: one FP16 / BF16 product per update; fp32: FP32 MFMA "
+                        "updates; fp64: plain FP64 factor")
+    p.add_argument("--refine", choices=["ir", "cg"], default=None,
+                   help="FP64 refinement of a fixed --mode: ir (iterative refinement, the default) or cg "
+                        "(conjugate gradients preconditioned by the low-precision factor)")
+    p.add_argument("--shift", type=float, default=0.0,
+                   help="cg: factor A with its diagonal scaled by 1 + shift (lets an ill-conditioned A's "
+                        "low-precision factorization finish)")
     p.add_argument("--max-iter", type=int, default=30)
     p.add_argument("--tol", type=float, default=1e-12)
     p.add_argument("--seed", type=int, default=2024)
@@ -86,7 +94,7 @@ def parse():
                    help="only run a plain device copy of MIB MiB three times (calibrates the HBM byte counters) "
                         "and exit")
     a = p.parse_args()
-    preset = {"metric": (65536, 16, 1e4, "f16x2"), "low-precision": (98304, 8, 1e6, "f16x2"),
+    preset = {"metric": (65536, 16, 1e4, "auto"), "low-precision": (98304, 8, 1e6, "f16x2"),
               "fp64": (32768, 16, 1e4, "fp64")}[a.preset]
     a.n = preset[0] if a.n is None else a.n
     a.nrhs = preset[1] if a.nrhs is None else a.nrhs
@@ -99,7 +107,7 @@ METRIC = "Cholesky solve TFLOP/s to 1e-12 backward err, N=65536: mixed-prec vs F
 
 
 def metric_name(a):
-    if (a.n, a.nrhs, a.kappa, a.mode) == (65536, 16, 1e4, "f16x2"):
+    if (a.n, a.nrhs, a.kappa) == (65536, 16, 1e4) and a.mode != "fp64":
         return METRIC
     what = {"f16x2": "mixed-prec (split-FP16 updates) vs FP64", "fp32": "mixed-prec (FP32 updates) vs FP64",
             "fp64": "plain FP64", "f16": "mixed-prec (FP16 updates) vs FP64",
@@ -468,11 +476,38 @@ def main():
     prec = {"f16x2": C.PREC_F16X2, "fp32": C.PREC_FP32, "fp64": C.PREC_FP64, "f16": C.PREC_F16, "bf16": C.PREC_BF16}
     iters = {}
 
+    refine = {}  # mode -> "ir" / "cg"
+
     def make_step(mode):
         def step():
-            it, be = C.dposv_mixed(A0, N, NB, B, X, prec=prec[mode], tol=a.tol, max_iter=a.max_iter, info=info)
+            if refine.get(mode) == "cg":
+                it, be = C.dposv_krylov(A0, N, NB, B, X, prec=prec[mode], shift=a.shift, tol=a.tol,
+                                        max_iter=max(a.max_iter, 200), info=info)
+            else:
+                it, be = C.dposv_mixed(A0, N, NB, B, X, prec=prec[mode], tol=a.tol, max_iter=a.max_iter,
+                                       info=info)
             iters[mode] = (it, be)
+            if not (be <= a.tol) or int(info.item()) != 0:
+                raise RuntimeError(f"{mode}/{refine.get(mode, 'ir')}: backward error {be}, info {int(info.item())}")
         return step
+
+    # ---- method: the autotuner's choice of factor precision and refinement
+    # (each candidate run once to build its plans, then timed once; the
+    # fastest that reaches the tolerance wins)
+    method = None
+    if a.mode == "auto":
+        import ttg_amd.autotune as T
+
+        def candidate(mode, how):
+            refine[mode] = how
+            return make_step(mode)
+
+        choice, mtrials = T.tune_method([("f16x2", "ir"), ("f16", "ir"), ("f16", "cg")], candidate)
+        a.mode = choice[0]
+        refine[a.mode] = choice[1]
+        method = {"choice": {"mode": choice[0], "refine": choice[1]}, "trials": mtrials}
+    else:
+        refine[a.mode] = a.refine or "ir"
 
     def backward_error():
         return C.backward_error(A0, N, NB, X, B)
@@ -611,6 +646,7 @@ def main():
         "fp64_backward_error": berr_fp64,
         "speedup_vs_fp64": None if ms_fp64 is None else round(ms_fp64 / ms_step, 3),
         "wall_s": round(wall, 3),
+        "method": method or {"choice": {"mode": a.mode, "refine": refine.get(a.mode, "ir")}},
         "schedule": schedule,
         "roofline": roof,
         "cpu_baseline": cpu,

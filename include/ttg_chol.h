@@ -318,6 +318,21 @@ int ttg_chol_dpotri_tiled(void *stream, int N, int NB, double *tiles, int *d_inf
 int ttg_chol_dposv_mixed(void *stream, int N, int NB, const double *A, int nrhs, const double *B, int ldb, double *X,
                          int ldx, int prec, double tol, int max_iter, int *d_info, int *iters, double *berr);
 
+/* A X = B by preconditioned conjugate gradients in FP64, preconditioned by
+ * the factor that ttg_chol_dposv_mixed would use (precision prec) of A with
+ * its diagonal scaled by 1 + shift (0 <= shift <= 1; a small shift, of the
+ * order of the working precision's unit roundoff, lets a low-precision
+ * factorization of an ill-conditioned A finish).  One CG per right-hand
+ * side from X = M^{-1} B; each iteration is one FP64 product with A and one
+ * solve with the factor.  Stops when every column's backward error (as
+ * ttg_chol_dposv_mixed) is <= tol on the true residual, or after max_iter
+ * iterations (TTG_CHOL_ENOCONV).  *iters: CG iterations; other arguments
+ * and the plan cache as ttg_chol_dposv_mixed.  No reference counterpart:
+ * the Krylov refinement of the mixed-precision solve (GMRES-IR / CG-IR). */
+int ttg_chol_dposv_krylov(void *stream, int N, int NB, const double *A, int nrhs, const double *B, int ldb,
+                          double *X, int ldx, int prec, double shift, double tol, int max_iter, int *d_info,
+                          int *iters, double *berr);
+
 /* The working factor of the last ttg_chol_dposv_mixed call with the same
  * (N, NB, nrhs, prec), widened to FP64, into L (device, num_tiles(N, NB) *
  * NB * NB doubles, the lower tile-packed layout): L in the lower part, the

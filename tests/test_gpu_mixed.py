@@ -61,3 +61,35 @@ def test_dposv_mixed_nonspd_reports_info(dev, oracle):
     info = torch.zeros(1, dtype=torch.int32, device=dev)
     its, berr = C.dposv_mixed(dA, N, NB, dB, X, prec=1, info=info)
     assert int(info.item()) == NB + 11
+
+
+@pytest.mark.parametrize("prec,kappa,shift", [(3, 1e4, 0.0), (4, 1e4, 0.0), (3, 1e6, 1e-3), (1, 1e4, 0.0)])
+def test_krylov_solve_reaches_tolerance(dev, oracle, prec, kappa, shift):
+    """Preconditioned CG with the low-precision factor (FP16 or BF16 single
+    products, FP32) reaches the FP64 backward error 1e-12, checked by the
+    oracle; at kappa 1e6 the FP16 factor of A itself meets a non-positive
+    pivot, and the factor of A with its diagonal scaled by 1 + 1e-3 serves."""
+    import ttg_amd.chol as C
+
+    N, NB, r = 4096, 512, 4
+    sigma2 = sum(k * k for k in range(-127, 128)) / 255.0 / 127.0 ** 2
+    A0 = oracle.spd_tiles(N, NB, 4 * sigma2 / (kappa - 1), 41)
+    Bh = np.random.default_rng(8).uniform(-1, 1, (N, r))
+    dA = torch.from_numpy(A0).to(dev)
+    dB = torch.from_numpy(np.ascontiguousarray(Bh.T)).to(dev).t()
+    X = torch.zeros(r, N, dtype=torch.float64, device=dev).t()
+    info = torch.zeros(1, dtype=torch.int32, device=dev)
+    print(f"krylov prec={prec} kappa={kappa:g} shift={shift:g}", flush=True)
+    its, be = C.dposv_krylov(dA, N, NB, dB, X, prec=prec, shift=shift, tol=1e-12, max_iter=300, info=info)
+    assert int(info.item()) == 0 and be <= 1e-12 and its >= 1
+    Xh = X.t().cpu().numpy().T
+    assert oracle.solve_backward_error(N, NB, A0, Xh, Bh) <= 1e-12
+
+
+def test_krylov_refuses_bad_shift(dev):
+    import ttg_amd.chol as C
+
+    A = torch.zeros(C.num_tiles(256, 128) * 128 * 128, dtype=torch.float64, device=dev)
+    B = torch.zeros(1, 256, dtype=torch.float64, device=dev).t()
+    with pytest.raises(C.CholError):
+        C.dposv_krylov(A, 256, 128, B, B.clone(), shift=-1.0)

@@ -10,7 +10,10 @@ per device and workload signature:
 * the factorization's schedule knobs (`ttg_chol_set_knob`): panels per
   bulk update, CUs reserved for the diagonal chain, the update's tile-group
   side and the grouped/one-panel schedule, searched by coordinate descent
-  from the defaults.
+  from the defaults;
+* the solve's method: the precision of the factor's trailing updates and
+  the FP64 refinement (iterative refinement or preconditioned CG), the
+  fastest that reaches the tolerance on the caller's data.
 
 Each candidate runs `warmup` untimed and `reps` timed times (HIP events on the
 calling stream); the median wins.  Results go to a JSON cache
@@ -224,3 +227,26 @@ def tune_schedule(run, N, NB, prec, nrhs, reps=2, warmup=1, min_gain=0.01, use_c
     apply_schedule(current)
     _store(key, {"choice": current, "trials": trials})
     return current, trials
+
+
+def tune_method(candidates, make_run, warmup=1, reps=1):
+    """The fastest solve method: candidates is a list of (mode, refine)
+    pairs, make_run(mode, refine) returns a callable that runs one solve
+    and raises when it does not reach the tolerance (or the factorization
+    fails).  Returns ((mode, refine), trials) with trials = [{"mode",
+    "refine", "ms" or "error"}, ...]."""
+    trials = []
+    best, best_ms = None, None
+    for mode, how in candidates:
+        run = make_run(mode, how)
+        try:
+            ms = _time_events(run, warmup, reps)
+        except Exception as e:  # a method that fails to converge is no choice
+            trials.append({"mode": mode, "refine": how, "error": str(e)[:200]})
+            continue
+        trials.append({"mode": mode, "refine": how, "ms": round(ms, 3)})
+        if best_ms is None or ms < best_ms:
+            best, best_ms = (mode, how), ms
+    if best is None:
+        raise RuntimeError(f"no solve method reached the tolerance: {trials}")
+    return best, trials

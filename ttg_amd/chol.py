@@ -66,6 +66,7 @@ SIGNATURES = {
     "ttg_chol_profile_read": (_i, [_i, _p, _p, _p]),
     "ttg_chol_profile_enable": (_i, [_i]),
     "ttg_chol_dposv_mixed": (_i, [_p, _i, _i, _p, _i, _p, _i, _p, _i, _i, _d, _i, _p, _p, _p]),
+    "ttg_chol_dposv_krylov": (_i, [_p, _i, _i, _p, _i, _p, _i, _p, _i, _i, _d, _d, _i, _p, _p, _p]),
     "ttg_chol_sposv_batched": (_i, [_p, _i, _p, _p, _p, _p, _p, _i]),
     "ttg_chol_batched_num_configs": (_i, []),
     "ttg_chol_batched_config": (_i, [_i, _p, _p]),
@@ -573,6 +574,30 @@ def dposv_mixed(A, N, NB, B, X, prec=PREC_FP32, tol=1e-12, max_iter=30, info=Non
     if rc == ENOCONV and not check:
         return it.value, be.value
     _check(rc, "dposv_mixed")
+    return it.value, be.value
+
+
+def dposv_krylov(A, N, NB, B, X, prec=PREC_F16, shift=0.0, tol=1e-12, max_iter=100, info=None, stream=None,
+                 check=True):
+    """A X = B by FP64 preconditioned CG with the low-precision factor of A
+    (diagonal scaled by 1 + shift) as preconditioner; returns (CG
+    iterations, backward error)."""
+    import torch
+
+    _check_tiles(A, N, NB, "A")
+    nrhs = _check_cols(B, N, "B")
+    _check_cols(X, N, "X", nrhs)
+    _check_info(info)
+    _require_cuda(A, B, X, info)
+    if info is None:
+        info = torch.zeros(1, dtype=torch.int32, device=A.device)
+    it = ctypes.c_int()
+    be = ctypes.c_double()
+    rc = lib().ttg_chol_dposv_krylov(_stream(stream), N, NB, _ptr(A), nrhs, _ptr(B), _ld(B), _ptr(X), _ld(X), prec,
+                                     float(shift), tol, max_iter, _ptr(info), ctypes.byref(it), ctypes.byref(be))
+    if rc == ENOCONV and not check:
+        return it.value, be.value
+    _check(rc, "dposv_krylov")
     return it.value, be.value
 
 

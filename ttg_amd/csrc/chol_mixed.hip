@@ -10,6 +10,10 @@
 //   4. Refinement in FP64:  R = B - A X (FP64, on the original A), stop when
 //      max_j ||r_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf) <= tol,
 //      else D = (L L^T)^{-1} R (working precision) and X += D (FP64).
+// Or (ttg_chol_dposv_krylov) preconditioned conjugate gradients in FP64 on
+// A x = b with M = L L^T, the low-precision factor of A (optionally of A
+// with its diagonal scaled by 1 + shift, for factors that would otherwise
+// meet a non-positive pivot), one CG per right-hand side.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,7 +42,8 @@ namespace ttgc {
 // a fixed order, and the four wave sums are added in wave order.
 template <typename T>
 __global__ __launch_bounds__(256) void convert_norm_kernel(int N, int NB, int nt, const double *__restrict__ A,
-                                                           T *__restrict__ W, double *__restrict__ S) {
+                                                           T *__restrict__ W, double *__restrict__ S,
+                                                           double dscale) {
   const long t = blockIdx.x;
   int m, n;
   {
@@ -67,14 +72,14 @@ __global__ __launch_bounds__(256) void convert_norm_kernel(int N, int NB, int nt
     double u = 0.0;
     if (r0 < NB) {
       const double v = a[o + r0];
-      w[o + r0] = (T)v;
+      w[o + r0] = (T)(diag && r0 == c ? v * dscale : v);
       const bool in = r0 < mrows && c < ncols;
       if (in && (!diag || r0 >= c)) ra0 += fabs(v);
       if (in && (!diag || r0 > c)) u += fabs(v);
     }
     if (r1 < NB) {
       const double v = a[o + r1];
-      w[o + r1] = (T)v;
+      w[o + r1] = (T)(diag && r1 == c ? v * dscale : v);
       const bool in = r1 < mrows && c < ncols;
       if (in && (!diag || r1 >= c)) ra1 += fabs(v);
       if (in && (!diag || r1 > c)) u += fabs(v);
@@ -158,6 +163,48 @@ __global__ void from_rows_acc_kernel(int N, int nrhs, const TS *__restrict__ Bt,
     X[i + (long)j * ldx] = v;
 }
 
+// out[j] = sum_i X(i, j) Y(i, j), one workgroup per column, reduced in a
+// fixed order (the same sums on every run)
+__global__ __launch_bounds__(256) void dot_cols_kernel(int N, const double *__restrict__ X, long ldx,
+                                                       const double *__restrict__ Y, long ldy,
+                                                       double *__restrict__ out) {
+  const int j = blockIdx.x;
+  __shared__ double s[256];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < N; i += 256) a += X[i + (long)j * ldx] * Y[i + (long)j * ldy];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = s[0];
+}
+
+// CG step of every column j, with Q = -A P:  a_j = rz_j / (p_j . A p_j),
+// X += a P, R -= a A P  (a = 0 where p_j . A p_j = 0)
+__global__ void cg_step_kernel(int N, int nrhs, const double *__restrict__ rz, const double *__restrict__ pq,
+                               const double *__restrict__ P, double *__restrict__ X, long ldx,
+                               const double *__restrict__ Q, double *__restrict__ R) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long)N * nrhs) return;
+  const int i = (int)(g % N), j = (int)(g / N);
+  const double pap = -pq[j];
+  const double a = pap != 0.0 ? rz[j] / pap : 0.0;
+  X[i + (long)j * ldx] += a * P[g];
+  R[g] += a * Q[g];
+}
+
+// P := Z + (rz_new_j / rz_j) P  (beta = 0 where rz_j = 0)
+__global__ void cg_dir_kernel(int N, int nrhs, const double *__restrict__ rz_new, const double *__restrict__ rz,
+                              const double *__restrict__ Z, double *__restrict__ P) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long)N * nrhs) return;
+  const int j = (int)(g / N);
+  const double b = rz[j] != 0.0 ? rz_new[j] / rz[j] : 0.0;
+  P[g] = Z[g] + b * P[g];
+}
+
 struct MixedPlanBase {
   virtual ~MixedPlanBase() = default;
 };
@@ -170,12 +217,13 @@ struct MixedPlan : MixedPlanBase {
   DevBuf R;      // residual, N x nrhs (FP64)
   DevBuf stats;  // anorm, berr[nrhs]
   DevBuf rows, norm_parts;  // -rowsum of |A|; per-tile |A| partial sums
+  DevBuf cg, zeros;         // CG: P, Q, Z (N x nrhs each) and per-column scalars; a zero N x nrhs
   std::unique_ptr<TiledPlanT<T>> fplan;
   SolvePlan<T> *splan = nullptr;
   std::vector<double> hstats;
 
   ~MixedPlan() override {
-    for (DevBuf *b : {&W, &R, &stats, &rows, &norm_parts})
+    for (DevBuf *b : {&W, &R, &stats, &rows, &norm_parts, &cg, &zeros})
       if (b->ptr) (void)hipFree(b->ptr);
   }
 
@@ -231,6 +279,119 @@ struct MixedPlan : MixedPlanBase {
     return TTG_CHOL_OK;
   }
 
+  // backward error of every column from the residual in `res` (FP64), the
+  // largest over the columns into *e (NaN propagates); host-synchronising
+  int column_berr(hipStream_t s, const double *res, const double *B, long ldb, const double *X, long ldx,
+                  double *e) {
+    hipLaunchKernelGGL(berr_kernel, dim3(nrhs), dim3(256), 0, s, N, res, (long)N, X, ldx, B, ldb, st(), st() + 1);
+    TTGC_HIP(hipGetLastError());
+    TTGC_HIP(hipMemcpyAsync(hstats.data(), st(), sizeof(double) * (1 + nrhs), hipMemcpyDeviceToHost, s));
+    TTGC_HIP(hipStreamSynchronize(s));
+    double m = 0.0;
+    for (int j = 0; j < nrhs; ++j) m = (hstats[1 + j] > m || std::isnan(hstats[1 + j])) ? hstats[1 + j] : m;
+    *e = m;
+    return TTG_CHOL_OK;
+  }
+
+  // steps 1-3: working copy (diagonal scaled by dscale), ||A||_inf, factor,
+  // X = M^{-1} B.  *failed: the factorization met a non-positive pivot.
+  int factor_and_solve(hipStream_t s, const double *A, const double *B, long ldb, double *X, long ldx, int *d_info,
+                       double dscale, bool *failed) {
+    const int nt = (N + NB - 1) / NB;
+    *failed = false;
+    {
+      const long ntiles = (long)nt * (nt + 1) / 2;
+      double *parts = static_cast<double *>(norm_parts.ptr), *rw = static_cast<double *>(rows.ptr);
+      hipLaunchKernelGGL(convert_norm_kernel<T>, dim3((unsigned)ntiles), dim3(256), 0, s, N, NB, nt, A, w(), parts,
+                         dscale);
+      TTGC_HIP(hipGetLastError());
+      hipLaunchKernelGGL(norm_rows_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, N, NB, nt, parts, rw);
+      TTGC_HIP(hipGetLastError());
+      hipLaunchKernelGGL(max_neg_kernel, dim3(1), dim3(1024), 0, s, N, rw, st());
+      TTGC_HIP(hipGetLastError());
+    }
+    int rc = fplan->run(s, d_info);
+    if (rc) return rc;
+    int hinfo = 0;
+    TTGC_HIP(hipMemcpyAsync(&hinfo, d_info, sizeof(int), hipMemcpyDeviceToHost, s));
+    TTGC_HIP(hipStreamSynchronize(s));
+    if (hinfo != 0) {
+      *failed = true;
+      return TTG_CHOL_OK;
+    }
+    rc = splan->prepare(s);
+    if (rc) return rc;
+    return solve(s, B, ldb, X, ldx, false);
+  }
+
+  // Preconditioned CG from X = M^{-1} B: iterations counted in *iters (one
+  // matrix product and one preconditioner solve each).  Convergence is
+  // judged on the recursively updated residual and confirmed on the true
+  // one (R = B - A X); a confirmation that fails restarts CG from the true
+  // residual.
+  int run_pcg(hipStream_t s, const double *A, const double *B, long ldb, double *X, long ldx, double tol,
+              int max_iter, int *d_info, double shift, int *iters, double *berr) {
+    const long n = (long)N * nrhs;
+    TTGC_HIP(cg.reserve(sizeof(double) * (3 * n + 3 * nrhs)));
+    TTGC_HIP(zeros.reserve(sizeof(double) * n));
+    double *P = static_cast<double *>(cg.ptr), *Q = P + n, *Z = Q + n, *rz = Z + n, *rzn = rz + nrhs,
+           *pq = rzn + nrhs;
+    double *zero = static_cast<double *>(zeros.ptr);
+    TTGC_HIP(hipMemsetAsync(zero, 0, sizeof(double) * n, s));
+    bool failed = false;
+    int rc = factor_and_solve(s, A, B, ldb, X, ldx, d_info, 1.0 + shift, &failed);
+    if (rc) return rc;
+    *iters = 0;
+    if (failed) {
+      *berr = INFINITY;
+      return TTG_CHOL_OK;  // caller reads *d_info
+    }
+    const unsigned gv = (unsigned)((n + 255) / 256);
+    double e = INFINITY;
+    int it = 0;
+    bool restart = true;
+    for (;;) {
+      if (restart) {  // true residual, then a fresh search direction
+        rc = residual_fp64(s, N, NB, A, nrhs, X, ldx, B, ldb, r(), N);
+        if (rc) return rc;
+        rc = column_berr(s, r(), B, ldb, X, ldx, &e);
+        if (rc) return rc;
+        if (!(e > tol) || it >= max_iter) break;
+        rc = solve(s, r(), N, Z, N, false);
+        if (rc) return rc;
+        hipLaunchKernelGGL(dot_cols_kernel, dim3(nrhs), dim3(256), 0, s, N, r(), (long)N, Z, (long)N, rz);
+        TTGC_HIP(hipGetLastError());
+        TTGC_HIP(hipMemcpyAsync(P, Z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        restart = false;
+      }
+      // Q = -A P
+      rc = residual_fp64(s, N, NB, A, nrhs, P, N, zero, N, Q, N);
+      if (rc) return rc;
+      hipLaunchKernelGGL(dot_cols_kernel, dim3(nrhs), dim3(256), 0, s, N, P, (long)N, Q, (long)N, pq);
+      TTGC_HIP(hipGetLastError());
+      hipLaunchKernelGGL(cg_step_kernel, dim3(gv), dim3(256), 0, s, N, nrhs, rz, pq, P, X, ldx, Q, r());
+      TTGC_HIP(hipGetLastError());
+      ++it;
+      double er = INFINITY;
+      rc = column_berr(s, r(), B, ldb, X, ldx, &er);
+      if (rc) return rc;
+      if (!(er > tol) || std::isnan(er) || it >= max_iter) {
+        restart = true;  // confirm on the true residual (or stop)
+        continue;
+      }
+      rc = solve(s, r(), N, Z, N, false);
+      if (rc) return rc;
+      hipLaunchKernelGGL(dot_cols_kernel, dim3(nrhs), dim3(256), 0, s, N, r(), (long)N, Z, (long)N, rzn);
+      TTGC_HIP(hipGetLastError());
+      hipLaunchKernelGGL(cg_dir_kernel, dim3(gv), dim3(256), 0, s, N, nrhs, rzn, rz, Z, P);
+      TTGC_HIP(hipGetLastError());
+      std::swap(rz, rzn);
+    }
+    *iters = it;
+    *berr = e;
+    return (e <= tol) ? TTG_CHOL_OK : TTG_CHOL_ENOCONV;
+  }
+
   int run(hipStream_t s, const double *A, const double *B, long ldb, double *X, long ldx, double tol, int max_iter,
           int *d_info, int *iters, double *berr) {
     const int nt = (N + NB - 1) / NB;
@@ -239,7 +400,8 @@ struct MixedPlan : MixedPlanBase {
     {
       const long ntiles = (long)nt * (nt + 1) / 2;
       double *parts = static_cast<double *>(norm_parts.ptr), *rw = static_cast<double *>(rows.ptr);
-      hipLaunchKernelGGL(convert_norm_kernel<T>, dim3((unsigned)ntiles), dim3(256), 0, s, N, NB, nt, A, w(), parts);
+      hipLaunchKernelGGL(convert_norm_kernel<T>, dim3((unsigned)ntiles), dim3(256), 0, s, N, NB, nt, A, w(), parts,
+                         1.0);
       TTGC_HIP(hipGetLastError());
       hipLaunchKernelGGL(norm_rows_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, N, NB, nt, parts, rw);
       TTGC_HIP(hipGetLastError());
@@ -296,61 +458,102 @@ void release_mixed_plans() {
 
 using namespace ttgc;
 
-extern "C" int ttg_chol_dposv_mixed(void *stream, int N, int NB, const double *A, int nrhs, const double *B, int ldb,
-                                    double *X, int ldx, int prec, double tol, int max_iter, int *d_info, int *iters,
-                                    double *berr) {
-  clear_error();
-  if (N <= 0 || NB <= 0 || NB > TTG_CHOL_MAX_TILE || NB % 32 != 0 || nrhs <= 0 || nrhs > 32 ||
-      ldb < std::max(1, N) || ldx < std::max(1, N) || !A || !B || !X || !d_info || !iters || !berr ||
-      max_iter < 0 || !(tol > 0.0)) {
-    set_error("dposv_mixed: bad argument (N=%d NB=%d nrhs=%d ldb=%d ldx=%d)", N, NB, nrhs, ldb, ldx);
-    return TTG_CHOL_EINVAL;
-  }
+namespace {
+
+// The cached plan of (N, NB, nrhs, prec), built on first use; the caller
+// holds g_mixed_mu.
+int mixed_plan(int N, int NB, int nrhs, int prec, MixedPlanBase **out) {
   if (prec != TTG_CHOL_PREC_FP64 && prec != TTG_CHOL_PREC_FP32 && prec != TTG_CHOL_PREC_F16X2 &&
       prec != TTG_CHOL_PREC_F16 && prec != TTG_CHOL_PREC_BF16) {
-    set_error("dposv_mixed: precision %d not supported", prec);
+    set_error("dposv: precision %d not supported", prec);
     return TTG_CHOL_ENOTSUP;
   }
-  hipStream_t s = as_stream(stream);
-  std::lock_guard<std::mutex> lk(g_mixed_mu);
+  if (prec >= TTG_CHOL_PREC_F16X2 && NB % 128 != 0) {
+    set_error("dposv: FP16/BF16 updates need NB %% 128 == 0 (NB=%d)", NB);
+    return TTG_CHOL_ENOTSUP;
+  }
   auto key = std::make_tuple(N, NB, nrhs, prec);
   auto &p = g_mixed[key];
-  if (prec == TTG_CHOL_PREC_FP64) {
-    if (!p) {
+  if (!p) {
+    int rc;
+    if (prec == TTG_CHOL_PREC_FP64) {
       auto *q = new MixedPlan<double>;
       p.reset(q);
       q->N = N;
       q->NB = NB;
       q->nrhs = nrhs;
-      int rc = q->build();
-      if (rc) {
-        g_mixed.erase(key);
-        return rc;
-      }
+      rc = q->build();
+    } else {
+      auto *q = new MixedPlan<float>;
+      p.reset(q);
+      q->N = N;
+      q->NB = NB;
+      q->nrhs = nrhs;
+      q->lowp = prec == TTG_CHOL_PREC_F16X2 ? kLowF16x2
+                : prec == TTG_CHOL_PREC_F16 ? kLowF16
+                : prec == TTG_CHOL_PREC_BF16 ? kLowBF16
+                                              : 0;
+      rc = q->build();
     }
-    return static_cast<MixedPlan<double> *>(p.get())->run(s, A, B, ldb, X, ldx, tol, max_iter, d_info, iters, berr);
-  }
-  if (prec >= TTG_CHOL_PREC_F16X2 && NB % 128 != 0) {
-    set_error("dposv_mixed: FP16/BF16 updates need NB %% 128 == 0 (NB=%d)", NB);
-    return TTG_CHOL_ENOTSUP;
-  }
-  if (!p) {
-    auto *q = new MixedPlan<float>;
-    p.reset(q);
-    q->N = N;
-    q->NB = NB;
-    q->nrhs = nrhs;
-    q->lowp = prec == TTG_CHOL_PREC_F16X2 ? kLowF16x2
-              : prec == TTG_CHOL_PREC_F16 ? kLowF16
-              : prec == TTG_CHOL_PREC_BF16 ? kLowBF16
-                                            : 0;
-    int rc = q->build();
     if (rc) {
       g_mixed.erase(key);
       return rc;
     }
   }
-  return static_cast<MixedPlan<float> *>(p.get())->run(s, A, B, ldb, X, ldx, tol, max_iter, d_info, iters, berr);
+  *out = p.get();
+  return TTG_CHOL_OK;
+}
+
+bool bad_solve_args(const char *name, int N, int NB, const double *A, int nrhs, const double *B, int ldb,
+                    const double *X, int ldx, double tol, int max_iter, const int *d_info, const int *iters,
+                    const double *berr) {
+  if (N <= 0 || NB <= 0 || NB > TTG_CHOL_MAX_TILE || NB % 32 != 0 || nrhs <= 0 || nrhs > 32 ||
+      ldb < std::max(1, N) || ldx < std::max(1, N) || !A || !B || !X || !d_info || !iters || !berr ||
+      max_iter < 0 || !(tol > 0.0)) {
+    set_error("%s: bad argument (N=%d NB=%d nrhs=%d ldb=%d ldx=%d)", name, N, NB, nrhs, ldb, ldx);
+    return true;
+  }
+  return false;
+}
+
+}  // namespace
+
+extern "C" int ttg_chol_dposv_mixed(void *stream, int N, int NB, const double *A, int nrhs, const double *B, int ldb,
+                                    double *X, int ldx, int prec, double tol, int max_iter, int *d_info, int *iters,
+                                    double *berr) {
+  clear_error();
+  if (bad_solve_args("dposv_mixed", N, NB, A, nrhs, B, ldb, X, ldx, tol, max_iter, d_info, iters, berr))
+    return TTG_CHOL_EINVAL;
+  hipStream_t s = as_stream(stream);
+  std::lock_guard<std::mutex> lk(g_mixed_mu);
+  MixedPlanBase *p = nullptr;
+  int rc = mixed_plan(N, NB, nrhs, prec, &p);
+  if (rc) return rc;
+  if (prec == TTG_CHOL_PREC_FP64)
+    return static_cast<MixedPlan<double> *>(p)->run(s, A, B, ldb, X, ldx, tol, max_iter, d_info, iters, berr);
+  return static_cast<MixedPlan<float> *>(p)->run(s, A, B, ldb, X, ldx, tol, max_iter, d_info, iters, berr);
+}
+
+extern "C" int ttg_chol_dposv_krylov(void *stream, int N, int NB, const double *A, int nrhs, const double *B,
+                                     int ldb, double *X, int ldx, int prec, double shift, double tol, int max_iter,
+                                     int *d_info, int *iters, double *berr) {
+  clear_error();
+  if (bad_solve_args("dposv_krylov", N, NB, A, nrhs, B, ldb, X, ldx, tol, max_iter, d_info, iters, berr))
+    return TTG_CHOL_EINVAL;
+  if (!(shift >= 0.0) || shift > 1.0) {
+    set_error("dposv_krylov: shift %g outside [0, 1]", shift);
+    return TTG_CHOL_EINVAL;
+  }
+  hipStream_t s = as_stream(stream);
+  std::lock_guard<std::mutex> lk(g_mixed_mu);
+  MixedPlanBase *p = nullptr;
+  int rc = mixed_plan(N, NB, nrhs, prec, &p);
+  if (rc) return rc;
+  if (prec == TTG_CHOL_PREC_FP64)
+    return static_cast<MixedPlan<double> *>(p)->run_pcg(s, A, B, ldb, X, ldx, tol, max_iter, d_info, shift, iters,
+                                                         berr);
+  return static_cast<MixedPlan<float> *>(p)->run_pcg(s, A, B, ldb, X, ldx, tol, max_iter, d_info, shift, iters,
+                                                      berr);
 }
 
 extern "C" int ttg_chol_dposv_mixed_factor(void *stream, int N, int NB, int nrhs, int prec, double *L) {
