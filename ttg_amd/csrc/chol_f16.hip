@@ -123,13 +123,16 @@ __device__ __forceinline__ void raw_barrier() {
 
 typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
 
-template <int MODE>
+// SPS slabs per LDS stage (one barrier per stage): the single-product modes
+// hold one part per operand, so two slabs per stage fit the LDS of two
+// workgroups per CU and halve the barriers; the split mode keeps one.
+template <int MODE, int SPS>
 __global__ __launch_bounds__(256, 2) void update_f16x2_kernel(const F16UpdOp *__restrict__ ops, int bpo, int nbn,
                                                            const int *__restrict__ d_abort) {
   constexpr int NP = (MODE == kLowF16x2) ? 2 : 1;  // parts per operand: hi, lo
   constexpr int BM = kF16BM, BN = kF16BM;
-  constexpr int A_CH = NP * BM * 4, ST_CH = A_CH + NP * BN * 4;  // 16-byte chunks per stage
-  constexpr int NSEG = ST_CH / 64;                                 // 1 KB (16-row) segments per stage
+  constexpr int A_CH = NP * BM * 4, ST_CH = A_CH + NP * BN * 4;  // 16-byte chunks per slab
+  constexpr int NSEG = ST_CH / 64;                                 // 1 KB (16-row) segments per slab
   constexpr int L = NSEG / 4;                                      // LDS-DMA loads per wave per slab
   constexpr int NCV = 16;                                          // C loads per lane
   if (d_abort && *d_abort) return;
@@ -139,20 +142,20 @@ __global__ __launch_bounds__(256, 2) void update_f16x2_kernel(const F16UpdOp *__
   if (op.lower && bi < bj) return;
   const int i0 = bi * BM, j0 = bj * BN;
   if (i0 >= op.n || j0 >= op.n) return;
-  __shared__ __align__(16) h8_t lds[2][ST_CH];  // [stage][A parts | B parts][row][chunk]
+  __shared__ __align__(16) h8_t lds[2][SPS * ST_CH];  // [stage][slab][A parts | B parts][row][chunk]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int lr = lane & 15, lk = lane >> 4;
   // npan products (panels), each op.k deep: slab s of the run is slab
   // s % nk of panel s / nk (a wave-uniform index)
   const int nk = op.k / kF16BK, npan = op.npan > 1 ? op.npan : 1;
-  const int ns = npan * nk;
+  const int ns = npan * nk;  // slabs; a multiple of SPS (the launcher checks k)
 
   // slab s into stage st: wave w loads segments w, w + 4, ...; lane l of a
   // segment fills physical chunk l & 3 of the segment's row l >> 2 with the
   // logical chunk the swizzle puts there (a permutation within each 64-byte
   // row, so the wave still reads whole 1 KB blocks of the k-slab-major panel)
-  auto issue = [&](int s, int st) {
+  auto issue = [&](int s, int st, int u) {
     const int q = s / nk, ks = s - q * nk;
     const long po = q * op.pstride;
     const int rseg = lane >> 2, p = lane & 3;
@@ -166,7 +169,7 @@ __global__ __launch_bounds__(256, 2) void update_f16x2_kernel(const F16UpdOp *__
       const uint16_t *base = isa ? (part ? op.alo : op.ahi) : (part ? op.blo : op.bhi);
       const h8_t *src = reinterpret_cast<const h8_t *>(base + po) + ((long)ks * op.ld + (isa ? i0 : j0) + row) * 4 +
                         (p ^ ((row >> 1) & 3));
-      const int off = (isa ? 0 : A_CH) + part * BM * 4 + sg * 64;
+      const int off = u * ST_CH + (isa ? 0 : A_CH) + part * BM * 4 + sg * 64;
       __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
                                        (__attribute__((address_space(3))) void *)&lds[st][off], 16, 0, 0);
     }
@@ -180,7 +183,8 @@ __global__ __launch_bounds__(256, 2) void update_f16x2_kernel(const F16UpdOp *__
   // the first slab's loads go out first, then this lane's C values (4
   // consecutive rows of one column per (i, j)), whose latency then hides
   // behind the first slabs instead of delaying them
-  issue(0, 0);
+#pragma unroll
+  for (int u = 0; u < SPS; ++u) issue(u, 0, u);
   f4_t cv[4][4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
@@ -189,19 +193,19 @@ __global__ __launch_bounds__(256, 2) void update_f16x2_kernel(const F16UpdOp *__
     for (int i = 0; i < 4; ++i) cv[i][j] = *gld(reinterpret_cast<const f4_t *>(pc + i * 16));
   }
 
-  auto compute = [&](int st) {
+  auto compute = [&](int st, int u) {
     h8_t ah[NP][4], bh[NP][4];
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int row = wm * 64 + i * 16 + lr;
-        ah[q][i] = lds[st][q * BM * 4 + row * 4 + swz(row, lk)];
+        ah[q][i] = lds[st][u * ST_CH + q * BM * 4 + row * 4 + swz(row, lk)];
       }
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int row = wn * 64 + j * 16 + lr;
-        bh[q][j] = lds[st][A_CH + q * BN * 4 + row * 4 + swz(row, lk)];
+        bh[q][j] = lds[st][u * ST_CH + A_CH + q * BN * 4 + row * 4 + swz(row, lk)];
       }
     }
 #pragma unroll
@@ -238,18 +242,23 @@ __global__ __launch_bounds__(256, 2) void update_f16x2_kernel(const F16UpdOp *__
       }
     }
   };
-  for (int s = 0; s < ns; ++s) {
-    // this wave's loads of slab s are done once at most the C loads issued
-    // after them (slab 0) are outstanding; the barrier makes every wave's
-    // visible and retires all reads of stage (s + 1) & 1 (slab s - 1)
-    if (s == 0)
+  for (int t = 0; t < ns / SPS; ++t) {
+    // this wave's loads of stage t are done once at most the C loads issued
+    // after them (stage 0) are outstanding; the barrier makes every wave's
+    // visible and retires all reads of stage (t + 1) & 1 (stage t - 1)
+    if (t == 0)
       wait_vmcnt<NCV>();
     else
       wait_vmcnt<0>();
     raw_barrier();
-    if (s + 1 < ns) issue(s + 1, (s + 1) & 1);
-    compute(s & 1);
-    if ((s + 1) % nk == 0 && s + 1 < ns) fold(s / nk);  // a panel done, more to come
+    if ((t + 1) * SPS < ns) {
+#pragma unroll
+      for (int u = 0; u < SPS; ++u) issue((t + 1) * SPS + u, (t + 1) & 1, u);
+    }
+#pragma unroll
+    for (int u = 0; u < SPS; ++u) compute(t & 1, u);
+    const int s = (t + 1) * SPS;  // slabs done
+    if (s % nk == 0 && s < ns) fold(s / nk - 1);  // a panel done, more to come (nk is a multiple of SPS)
   }
   fold(npan - 1);
   const bool diag = op.lower && bi == bj;
@@ -290,12 +299,15 @@ hipError_t launch_update_f16x2(hipStream_t s, const F16UpdOp *ops, int nops, int
   const long grid = (long)nops * bpo;
   if (grid > 0x7fffffffL) return hipErrorInvalidConfiguration;
   const dim3 g((unsigned)grid), b(256);
+  // two slabs per stage need every op's depth k to be a multiple of 64
+  // (NB % 128 == 0 in the planned solves)
+  if (NB % (2 * kF16BK) != 0 && mode != kLowF16x2) return hipErrorInvalidValue;
   if (mode == kLowF16x2)
-    hipLaunchKernelGGL(update_f16x2_kernel<kLowF16x2>, g, b, 0, s, ops, bpo, nbn, d_abort);
+    hipLaunchKernelGGL((update_f16x2_kernel<kLowF16x2, 1>), g, b, 0, s, ops, bpo, nbn, d_abort);
   else if (mode == kLowF16)
-    hipLaunchKernelGGL(update_f16x2_kernel<kLowF16>, g, b, 0, s, ops, bpo, nbn, d_abort);
+    hipLaunchKernelGGL((update_f16x2_kernel<kLowF16, 2>), g, b, 0, s, ops, bpo, nbn, d_abort);
   else
-    hipLaunchKernelGGL(update_f16x2_kernel<kLowBF16>, g, b, 0, s, ops, bpo, nbn, d_abort);
+    hipLaunchKernelGGL((update_f16x2_kernel<kLowBF16, 2>), g, b, 0, s, ops, bpo, nbn, d_abort);
   return hipGetLastError();
 }
 
