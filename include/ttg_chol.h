@@ -79,6 +79,10 @@ int ttg_chol_release_plans(void);
  *                                                        [TTG_CHOL_TILE_GROUP, 2]
  *   PAIRED      1: grouped split-FP16 schedule, 0: one panel per update
  *                                                        [TTG_CHOL_PAIRED, 1]
+ *   SWEEP       triangular solves (up to 16 right-hand sides per pass):
+ *               k = 1..8 runs both sweeps as one persistent launch with at
+ *               most k workgroups per CU, 0 as three launches per block
+ *               step replayed from a graph              [TTG_CHOL_SWEEP, 2]
  * value < 0 clears the setting.  ttg_chol_get_knob returns the value a plan
  * built now would use.  No reference counterpart: the reference's schedule
  * is PaRSEC's, steered by the priorities of examples/potrf/potrf.h:743-752;
@@ -87,7 +91,8 @@ int ttg_chol_release_plans(void);
 #define TTG_CHOL_KNOB_POTRF_CUS 1
 #define TTG_CHOL_KNOB_TILE_GROUP 2
 #define TTG_CHOL_KNOB_PAIRED 3
-#define TTG_CHOL_KNOB_COUNT 4
+#define TTG_CHOL_KNOB_SWEEP 4
+#define TTG_CHOL_KNOB_COUNT 5
 int ttg_chol_set_knob(int knob, int value);
 int ttg_chol_get_knob(int knob);
 

@@ -76,7 +76,7 @@ def test_knobs_set_get_and_refuse(dev):
         assert C.get_knob("group") == 4
         C.set_knob("group", None)
         assert C.get_knob("group") == 3
-        for name, bad in (("group", 1), ("group", 9), ("tile_group", 0), ("paired", 2), ("potrf_cus", 65)):
+        for name, bad in (("group", 1), ("group", 9), ("tile_group", 0), ("paired", 2), ("potrf_cus", 65), ("sweep", 9)):
             with pytest.raises(C.CholError):
                 C.set_knob(name, bad)
     finally:

@@ -177,6 +177,50 @@ def test_dpotrs_and_residual_vs_oracle(dev, oracle, N, NB, nrhs):
     assert oracle.solve_backward_error(N, NB, A0, Xg, B) <= 1e-14
 
 
+def _solve_with_sweep(C, fn, L, N, NB, B, dev, sweep):
+    """fn (dpotrs_tiled / spotrs_tiled) on a fresh copy of B with the SWEEP
+    knob set (plans rebuilt), as a host array."""
+    C.set_knob("sweep", sweep)
+    C.release_plans()
+    try:
+        X = torch.from_numpy(np.ascontiguousarray(B.T)).to(dev).t()
+        fn(L, N, NB, X)
+        return X.t().cpu().numpy().T
+    finally:
+        C.set_knob("sweep", None)
+        C.release_plans()
+
+
+@pytest.mark.parametrize("N,NB,nrhs", [(160, 32, 1), (1000, 128, 3), (2048, 512, 16), (1600, 512, 5),
+                                       (3072, 256, 16), (4096, 512, 33)])
+def test_sweep_forms_agree(dev, oracle, N, NB, nrhs):
+    """Both triangular sweeps as one persistent launch (SWEEP = 1, 2 and 4
+    workgroups per CU: the same summation order, so bitwise one result) and
+    as per-step launches (SWEEP = 0): FP64 within the FP64 bar of the oracle's
+    solve; FP32 (the factor rounded to float) within 1e-5 of the oracle's FP64
+    solve with that same rounded factor (A = B B^T/N + I/2, condition ~4)."""
+    import ttg_amd.chol as C
+
+    A0 = oracle.spd_tiles(N, NB, 0.5, 5 + N)
+    L = A0.copy()
+    assert oracle.potrf_tiled(N, NB, L) == 0
+    rng = np.random.default_rng(N + 7 * nrhs)
+    B = np.asfortranarray(rng.uniform(-1, 1, (N, nrhs)))
+    Xref = oracle.potrs_tiled(N, NB, L, B)
+    dL = to_dev(L, dev)
+    got = {sw: _solve_with_sweep(C, C.dpotrs_tiled, dL, N, NB, B, dev, sw) for sw in (0, 1, 2, 4)}
+    for sw, X in got.items():
+        assert np.abs(X - Xref).max() / np.abs(Xref).max() <= TOL, f"FP64 sweep={sw}"
+    assert np.array_equal(got[1], got[2]) and np.array_equal(got[2], got[4])
+    L32 = L.astype(np.float32)
+    Xref32 = oracle.potrs_tiled(N, NB, L32.astype(np.float64), B)
+    dL32 = to_dev(L32, dev)
+    got32 = {sw: _solve_with_sweep(C, C.spotrs_tiled, dL32, N, NB, B, dev, sw) for sw in (0, 1, 2, 4)}
+    for sw, X in got32.items():
+        assert np.abs(X - Xref32).max() / np.abs(Xref32).max() <= 1e-5, f"FP32 sweep={sw}"
+    assert np.array_equal(got32[1], got32[2]) and np.array_equal(got32[2], got32[4])
+
+
 def test_large_factor_solve_backward_error(dev):
     """Size-independent property at a realistic size: factor N=16384, solve,
     and the FP64 residual gives a backward error at the level of eps."""

@@ -167,7 +167,15 @@ def tune_tiled_nb(N, candidates=(256, 512), prec=None, nrhs=16, reps=1, warmup=1
 
 
 # candidate values per schedule knob, searched in this order
-SCHEDULE_SPACE = (("paired", (1, 0)), ("group", (3, 2, 4)), ("potrf_cus", (8, 0, 16)), ("tile_group", (2, 1, 4)))
+SCHEDULE_SPACE = (
+    ("paired", (1, 0)),
+    ("group", (3, 2, 4)),
+    ("potrf_cus", (8, 0, 16)),
+    ("tile_group", (2, 1, 4)),
+    # workgroups per CU of the persistent sweeps (each value gives bitwise
+    # the same solution; 0, the per-step launches, sums in another order)
+    ("sweep", (2, 1, 4)),
+)
 
 
 def schedule_signature(N, NB, prec, nrhs):

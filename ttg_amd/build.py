@@ -21,6 +21,7 @@ SOURCES = [
     "chol_f16.hip",
     "chol_batched.hip",
     "chol_poinv.hip",
+    "chol_sweep.hip",
     "chol_check.hip", "chol_tileops.hip",
 ]
 

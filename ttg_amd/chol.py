@@ -422,6 +422,14 @@ def dpotrs_tiled(tiles, N, NB, B, stream=None):
     _check(lib().ttg_chol_dpotrs_tiled(_stream(stream), N, NB, _ptr(tiles), nrhs, _ptr(B), _ld(B)), "dpotrs_tiled")
 
 
+def spotrs_tiled(tiles, N, NB, B, stream=None):
+    """B := (L L^T)^{-1} B with an FP32 factor (lower float tiles), FP64 B."""
+    _check_tiles(tiles, N, NB, dtype="f32")
+    nrhs = _check_cols(B, N, "B")
+    _require_cuda(tiles, B)
+    _check(lib().ttg_chol_spotrs_tiled(_stream(stream), N, NB, _ptr(tiles), nrhs, _ptr(B), _ld(B)), "spotrs_tiled")
+
+
 def dresidual_tiled(tiles, N, NB, X, B, R, stream=None):
     _check_tiles(tiles, N, NB)
     nrhs = _check_cols(B, N, "B")
@@ -490,7 +498,7 @@ def release_plans():
 
 
 # schedule knobs of the planned factorizations (include/ttg_chol.h)
-KNOBS = {"group": 0, "potrf_cus": 1, "tile_group": 2, "paired": 3}
+KNOBS = {"group": 0, "potrf_cus": 1, "tile_group": 2, "paired": 3, "sweep": 4}
 
 
 def set_knob(name, value):

@@ -538,7 +538,7 @@ using namespace ttgc;
 
 namespace {
 // process-wide schedule knobs; -1: not set (environment or default)
-int g_knobs[TTG_CHOL_KNOB_COUNT] = {-1, -1, -1, -1};
+int g_knobs[TTG_CHOL_KNOB_COUNT] = {-1, -1, -1, -1, -1};
 std::mutex g_knob_mu;
 }  // namespace
 
@@ -565,7 +565,7 @@ int ttg_chol_num_devices(void) {
 
 int ttg_chol_set_knob(int knob, int value) {
   clear_error();
-  static const int lo[TTG_CHOL_KNOB_COUNT] = {2, 0, 1, 0}, hi[TTG_CHOL_KNOB_COUNT] = {8, 64, 8, 1};
+  static const int lo[TTG_CHOL_KNOB_COUNT] = {2, 0, 1, 0, 0}, hi[TTG_CHOL_KNOB_COUNT] = {8, 64, 8, 1, 8};
   if (knob < 0 || knob >= TTG_CHOL_KNOB_COUNT || (value >= 0 && (value < lo[knob] || value > hi[knob]))) {
     set_error("set_knob: knob %d cannot take %d", knob, value);
     return TTG_CHOL_EINVAL;
@@ -582,6 +582,7 @@ int ttg_chol_get_knob(int knob) {
     case TTG_CHOL_KNOB_POTRF_CUS: return plan_knob(knob, "TTG_CHOL_POTRF_CUS", 8);
     case TTG_CHOL_KNOB_TILE_GROUP: return plan_knob(knob, "TTG_CHOL_TILE_GROUP", 2);
     case TTG_CHOL_KNOB_PAIRED: return plan_knob(knob, "TTG_CHOL_PAIRED", 1);
+    case TTG_CHOL_KNOB_SWEEP: return plan_knob(knob, "TTG_CHOL_SWEEP", 2);
     default: set_error("get_knob: no knob %d", knob); return TTG_CHOL_EINVAL;
   }
 }

@@ -279,6 +279,20 @@ struct MixedPlan : MixedPlanBase {
     return TTG_CHOL_OK;
   }
 
+  // Synchronise s; an error if the last persistent sweep gave up (the
+  // solves run before every synchronisation of the refinement loops).
+  int sync_checked(hipStream_t s) {
+    int failed = 0;
+    if (const int *fw = splan->fail_word())
+      TTGC_HIP(hipMemcpyAsync(&failed, fw, sizeof(int), hipMemcpyDeviceToHost, s));
+    TTGC_HIP(hipStreamSynchronize(s));
+    if (failed) {
+      set_error("triangular sweep: a workgroup waited past the time limit");
+      return TTG_CHOL_EHIP;
+    }
+    return TTG_CHOL_OK;
+  }
+
   // backward error of every column from the residual in `res` (FP64), the
   // largest over the columns into *e (NaN propagates); host-synchronising
   int column_berr(hipStream_t s, const double *res, const double *B, long ldb, const double *X, long ldx,
@@ -286,7 +300,8 @@ struct MixedPlan : MixedPlanBase {
     hipLaunchKernelGGL(berr_kernel, dim3(nrhs), dim3(256), 0, s, N, res, (long)N, X, ldx, B, ldb, st(), st() + 1);
     TTGC_HIP(hipGetLastError());
     TTGC_HIP(hipMemcpyAsync(hstats.data(), st(), sizeof(double) * (1 + nrhs), hipMemcpyDeviceToHost, s));
-    TTGC_HIP(hipStreamSynchronize(s));
+    int rc = sync_checked(s);
+    if (rc) return rc;
     double m = 0.0;
     for (int j = 0; j < nrhs; ++j) m = (hstats[1 + j] > m || std::isnan(hstats[1 + j])) ? hstats[1 + j] : m;
     *e = m;
@@ -433,7 +448,8 @@ struct MixedPlan : MixedPlanBase {
       hipLaunchKernelGGL(berr_kernel, dim3(nrhs), dim3(256), 0, s, N, r(), (long)N, X, ldx, B, ldb, st(), st() + 1);
       TTGC_HIP(hipGetLastError());
       TTGC_HIP(hipMemcpyAsync(hstats.data(), st(), sizeof(double) * (1 + nrhs), hipMemcpyDeviceToHost, s));
-      TTGC_HIP(hipStreamSynchronize(s));
+      rc = sync_checked(s);
+      if (rc) return rc;
       e = 0.0;
       for (int j = 0; j < nrhs; ++j) e = (hstats[1 + j] > e || std::isnan(hstats[1 + j])) ? hstats[1 + j] : e;
       if (e <= tol || it >= max_iter) break;

@@ -30,6 +30,7 @@
 #include "chol_internal.hpp"
 #include "chol_kernels.hpp"
 #include "chol_solve.hpp"
+#include "chol_sweep.hpp"
 #include "chol_tiled.hpp"
 
 namespace ttgc {
@@ -190,6 +191,8 @@ static hipError_t launch_reduce(hipStream_t s, const UpdOp<T> &op, const T *P) {
 
 template <typename T>
 int SolvePlan<T>::build() {
+  const int wgs = plan_knob(TTG_CHOL_KNOB_SWEEP, "TTG_CHOL_SWEEP", 2);
+  persist = wgs > 0 && NB <= kSweepMaxNB && NB % kSweepRows == 0;
   NT = (N + NB - 1) / NB;
   Np = NT * NB;
   dstride = ((NB + 31) / 32) * 32 * 32;
@@ -235,19 +238,53 @@ int SolvePlan<T>::build() {
       upd.push_back(UpdOp<T>{rows(i), tile(i, m), xrows(m), nrhs, dim(m), dim(i), nrhs, NB, nrhs, 0});
     bw_upd_cnt.push_back((int)upd.size() - bw_upd_off.back());
   }
+  // the persistent sweeps' first off-diagonal products (prepare() forms
+  // them): P_m = Linv_m L_{m,m-1} and R_i = L_{i+1,i} Linv_i
+  std::vector<UpdOp<T>> pr;
+  if (persist && NT > 1) {
+    TTGC_HIP(d_pr.reserve(sizeof(T) * (size_t)2 * (NT - 1) * NB * NB));
+    T *pbase = static_cast<T *>(d_pr.ptr), *rbase = pbase + (long)(NT - 1) * NB * NB;
+    for (int m = 1; m < NT; ++m)  // C = Linv_m B^T, B(c, k) = L_{m,m-1}(k, c)
+      pr.push_back(UpdOp<T>{li(m), tile(m, m - 1), pbase + (long)(m - 1) * NB * NB, dim(m), dim(m - 1), dim(m), NB,
+                            NB, NB, 0});
+    n_pr = (int)pr.size();
+    for (int i = 0; i + 1 < NT; ++i)  // C = L_{i+1,i} B^T, B(c, k) = Linv_i(k, c)
+      pr.push_back(
+          UpdOp<T>{tile(i + 1, i), li(i), rbase + (long)i * NB * NB, dim(i + 1), dim(i), dim(i), NB, NB, NB, 0});
+  }
   auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
   o_dj = 0;
   o_inv = align(dj.size() * sizeof(PotrfOpT<T>));
   o_upd = align(o_inv + inv.size() * sizeof(TrsmOp<T>));
-  const size_t total = align(o_upd + upd.size() * sizeof(UpdOp<T>));
+  o_pr = align(o_upd + upd.size() * sizeof(UpdOp<T>));
+  const size_t total = align(o_pr + pr.size() * sizeof(UpdOp<T>));
   TTGC_HIP(d_ops.reserve(total));
   std::vector<char> host(total);
   memcpy(host.data() + o_dj, dj.data(), dj.size() * sizeof(PotrfOpT<T>));
   memcpy(host.data() + o_inv, inv.data(), inv.size() * sizeof(TrsmOp<T>));
   if (!upd.empty()) memcpy(host.data() + o_upd, upd.data(), upd.size() * sizeof(UpdOp<T>));
+  if (!pr.empty()) memcpy(host.data() + o_pr, pr.data(), pr.size() * sizeof(UpdOp<T>));
   TTGC_HIP(hipMemcpy(d_ops.ptr, host.data(), total, hipMemcpyHostToDevice));
   host_ops.assign(upd.begin(), upd.end());
   TTGC_HIP(hipStreamCreateWithFlags(&s_main, hipStreamNonBlocking));
+  if (persist) {
+    int dev = 0, ncu = 0, per_cu = 0, khz = 0;
+    TTGC_HIP(hipGetDevice(&dev));
+    TTGC_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    TTGC_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    TTGC_HIP(sweep_occupancy<T>(&per_cu));
+    // every workgroup resident at once (and at least NB/32 of them: the
+    // sweep's progress condition)
+    sweep_grid = std::min(wgs, per_cu) * ncu;
+    if (sweep_grid < NB / kSweepRows) {
+      set_error("solve plan: %d resident sweep workgroups, %d needed", sweep_grid, NB / kSweepRows);
+      return TTG_CHOL_ENOTSUP;
+    }
+    sweep_timeout = (long long)(khz > 0 ? khz : 100000) * 1000LL * 10;  // 10 s
+    const size_t nc = (size_t)sweep_counter_ints(NT);
+    TTGC_HIP(d_ctr.reserve(sizeof(int) * nc));
+    TTGC_HIP(hipMemset(d_ctr.ptr, 0, sizeof(int) * nc));
+  }
   return TTG_CHOL_OK;
 }
 
@@ -259,6 +296,11 @@ int SolvePlan<T>::prepare(hipStream_t s) {
   hipLaunchKernelGGL(identity_tiles_kernel<T>, dim3(64, NT), dim3(256), 0, s, N, NB, static_cast<T *>(d_linv.ptr));
   TTGC_HIP(hipGetLastError());
   TTGC_HIP(launch_trsm<T>(s, inv_ops(), NT, tnone, NB, nullptr, false));
+  if (persist && n_pr > 0) {
+    UpdOp<T> unone{};
+    const bool fast = NB % 128 == 0 && N % NB == 0;
+    TTGC_HIP(launch_update<T>(s, pr_ops(), 2 * n_pr, unone, NB, NB, T(1), T(0), fast, nullptr, true));
+  }
   return TTG_CHOL_OK;
 }
 
@@ -272,6 +314,29 @@ int SolvePlan<T>::prepare(hipStream_t s) {
 // GPU and a directly launched step is bound by the host's launch calls.
 template <typename T>
 int SolvePlan<T>::enqueue_sweeps(hipStream_t s) {
+  if (persist) {
+    // passes of up to 16 columns: a column's arithmetic does not depend on
+    // how many columns the solve has
+    for (int j0 = 0; j0 < nrhs; j0 += kSweepMaxRhs) {
+      const T *pm = static_cast<const T *>(d_pr.ptr);
+      SweepArgs<T> a{tiles,
+                     static_cast<const T *>(d_linv.ptr),
+                     pm,
+                     pm + (long)(NT - 1) * NB * NB,
+                     bt() + j0,
+                     static_cast<T *>(d_xt.ptr) + j0,
+                     static_cast<int *>(d_ctr.ptr),
+                     N,
+                     NB,
+                     NT,
+                     std::min(kSweepMaxRhs, nrhs - j0),
+                     nrhs,
+                     sweep_full<T>(N, NB) ? 1 : 0,
+                     sweep_timeout};
+      TTGC_HIP(launch_sweep<T>(s, a, sweep_grid, j0 == 0));
+    }
+    return TTG_CHOL_OK;
+  }
   UpdOp<T> unone{};
   const T m1 = T(-1), p1 = T(1);
   const bool fast = NB % 128 == 0 && N % NB == 0;
@@ -298,7 +363,7 @@ SolvePlan<T>::~SolvePlan() {
   if (gexec) (void)hipGraphExecDestroy(gexec);
   if (graph) (void)hipGraphDestroy(graph);
   if (s_main) (void)hipStreamDestroy(s_main);
-  for (DevBuf *b : {&d_ops, &d_bt, &d_xt, &d_dinv, &d_linv, &d_part})
+  for (DevBuf *b : {&d_ops, &d_bt, &d_xt, &d_dinv, &d_linv, &d_part, &d_ctr, &d_pr})
     if (b->ptr) (void)hipFree(b->ptr);
 }
 
@@ -588,10 +653,11 @@ int ttg_chol_dpotrs_tiled(void *stream, int N, int NB, const double *tiles, int 
     return TTG_CHOL_EINVAL;
   }
   if (N == 0 || nrhs == 0) return TTG_CHOL_OK;
-  if (nrhs > 32) {
-    // the sweeps use the skinny (<= 32 rows) update shape: split wide solves
-    for (int j0 = 0; j0 < nrhs; j0 += 32) {
-      int rc = ttg_chol_dpotrs_tiled(stream, N, NB, tiles, std::min(32, nrhs - j0), B + (long)j0 * ldb, ldb);
+  if (nrhs > kSweepMaxRhs) {
+    // the sweeps take up to 16 right-hand sides per pass: split wide solves
+    for (int j0 = 0; j0 < nrhs; j0 += kSweepMaxRhs) {
+      int rc = ttg_chol_dpotrs_tiled(stream, N, NB, tiles, std::min(kSweepMaxRhs, nrhs - j0), B + (long)j0 * ldb,
+                                     ldb);
       if (rc != TTG_CHOL_OK) return rc;
     }
     return TTG_CHOL_OK;
@@ -618,9 +684,10 @@ int ttg_chol_spotrs_tiled(void *stream, int N, int NB, const float *tiles, int n
     return TTG_CHOL_EINVAL;
   }
   if (N == 0 || nrhs == 0) return TTG_CHOL_OK;
-  if (nrhs > 32) {
-    for (int j0 = 0; j0 < nrhs; j0 += 32) {
-      int rc = ttg_chol_spotrs_tiled(stream, N, NB, tiles, std::min(32, nrhs - j0), B + (long)j0 * ldb, ldb);
+  if (nrhs > kSweepMaxRhs) {
+    for (int j0 = 0; j0 < nrhs; j0 += kSweepMaxRhs) {
+      int rc = ttg_chol_spotrs_tiled(stream, N, NB, tiles, std::min(kSweepMaxRhs, nrhs - j0), B + (long)j0 * ldb,
+                                     ldb);
       if (rc != TTG_CHOL_OK) return rc;
     }
     return TTG_CHOL_OK;

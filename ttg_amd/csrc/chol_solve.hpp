@@ -7,6 +7,7 @@
 
 #include "chol_internal.hpp"
 #include "chol_kernels.hpp"
+#include "chol_sweep.hpp"
 
 namespace ttgc {
 
@@ -25,6 +26,15 @@ struct SolvePlan {
   size_t o_dj = 0, o_inv = 0, o_upd = 0;
   std::vector<int> fw_app, fw_upd_off, fw_upd_cnt, bw_app, bw_upd_off, bw_upd_cnt;
   std::vector<UpdOp<T>> host_ops;  // host copy of the sweep descriptors (the applies read theirs from it)
+  // persistent sweeps (the SWEEP knob > 0): both sweeps as one launch of
+  // sweep_grid workgroups per 16 right-hand sides, counters in d_ctr
+  bool persist = false;
+  int sweep_grid = 0;
+  long long sweep_timeout = 0;
+  DevBuf d_ctr;
+  DevBuf d_pr;  // P_m (NT-1 tiles) then R_i (NT-1 tiles), from prepare()
+  size_t o_pr = 0;  // offset of the P and R descriptors in d_ops
+  int n_pr = 0;     // P descriptors (the R descriptors follow)
   hipStream_t s_main = nullptr;  // the stream the sweep graph is captured on
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -37,7 +47,14 @@ struct SolvePlan {
   PotrfOpT<T> *dj_ops() { return reinterpret_cast<PotrfOpT<T> *>(static_cast<char *>(d_ops.ptr) + o_dj); }
   TrsmOp<T> *inv_ops() { return reinterpret_cast<TrsmOp<T> *>(static_cast<char *>(d_ops.ptr) + o_inv); }
   UpdOp<T> *upd_ops() { return reinterpret_cast<UpdOp<T> *>(static_cast<char *>(d_ops.ptr) + o_upd); }
+  UpdOp<T> *pr_ops() { return reinterpret_cast<UpdOp<T> *>(static_cast<char *>(d_ops.ptr) + o_pr); }
   T *bt() { return static_cast<T *>(d_bt.ptr); }
+  // The failure word of the persistent sweeps (nullptr when the plan does
+  // not use them): nonzero after a solve in which a launch gave up (time
+  // limit; the solve's results are then not valid); the next solve clears it.
+  const int *fail_word() const {
+    return persist ? static_cast<const int *>(d_ctr.ptr) + kSweepFail * kSweepStride : nullptr;
+  }
 };
 
 template <typename T>
