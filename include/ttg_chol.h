@@ -222,14 +222,20 @@ int ttg_chol_dgemm_tn(void *stream, int m, int n, int k, double alpha, const dou
  * operations above issued by the calling thread on `stream` are recorded
  * instead of launched; group_end launches them as one batched kernel per
  * operation kind (PLGSY tiles, POTRF, TRSM, SYRK+GEMM), in that order.
- * Operations in one group must be independent (no tile written by one op
- * and read or written by another).  This is how the task runtime fuses every ready device task
- * of a scheduling round into a few launches (no counterpart in the
- * reference, which issues one vendor call per task). */
+ * A call that reads memory an earlier call of the group writes, or writes
+ * memory an earlier call reads or writes (by the address ranges of their
+ * blocks), makes the group launch what it holds first and go on empty, so
+ * dependent calls keep their program order.  This is how the task runtime
+ * fuses every ready device task of a scheduling round into a few launches
+ * (no counterpart in the reference, which issues one vendor call per task). */
 int ttg_chol_group_begin(void *stream);
 int ttg_chol_group_end(void *stream);
-/* Number of kernel launches issued by the last group_end on this thread. */
+/* Number of kernel launches issued by the last group on this thread (all
+ * its parts). */
 int ttg_chol_group_last_launches(void);
+/* Number of times the last group on this thread launched early because a
+ * call depended on an earlier one.  No reference counterpart. */
+int ttg_chol_group_last_splits(void);
 
 /* ---------------------------------------------------------------- */
 /* whole-matrix drivers (lower tile-packed layout)                   */
@@ -311,7 +317,9 @@ int ttg_chol_dpotri_tiled(void *stream, int N, int NB, double *tiles, int *d_inf
  * nrhs <= 32) with a factorization in working precision `prec` followed by
  * FP64 iterative refinement until the normwise backward error
  * max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf) <= tol or
- * max_iter refinement steps.  The residual r = b - A x and the update
+ * max_iter refinement steps; it stops earlier when that backward error is
+ * NaN or two steps in a row did not halve it (a diverging or stagnating
+ * refinement; TTG_CHOL_ENOCONV unless tol was met).  The residual r = b - A x and the update
  * x += d run in FP64 on the device; the diagonal POTRF computes in FP64.
  * Synchronizes the host once per refinement step (to test convergence).
  * *iters = refinement steps taken, *berr = final backward error;

@@ -136,6 +136,7 @@ def test_chain_benchmark(flows):
     assert r.returncode == 0
     assert f"Graph {flows} is connected." in r.stdout
     assert "# of tasks = 500" in r.stdout
+    assert "# of tasks run = 501" in r.stdout  # keys 0 .. 500: the chain ran to its end
     avg = float(r.stdout.strip().splitlines()[-1].split("avg")[-1])
     assert 0.0 < avg < 10000.0
 

@@ -63,6 +63,30 @@ def test_dposv_mixed_nonspd_reports_info(dev, oracle):
     assert int(info.item()) == NB + 11
 
 
+def test_refinement_stops_when_stagnating(dev, oracle):
+    """An unreachable tolerance (1e-300) with 30 allowed steps: the FP32
+    refinement reaches the FP64 floor within a few steps, and the loop then
+    stops after two steps that do not halve the backward error (ENOCONV)
+    instead of running all 30 FP64 residuals and solves."""
+    import ttg_amd.chol as C
+
+    N, NB, r = 2048, 512, 2
+    A0 = oracle.spd_tiles(N, NB, 0.5, 13)
+    Bh = np.random.default_rng(2).uniform(-1, 1, (N, r))
+    dA = to_dev(A0, dev)
+    dB = torch.from_numpy(np.ascontiguousarray(Bh.T)).to(dev).t()
+    X = torch.zeros(r, N, dtype=torch.float64, device=dev).t()
+    info = torch.zeros(1, dtype=torch.int32, device=dev)
+    with pytest.raises(C.CholError):
+        C.dposv_mixed(dA, N, NB, dB, X, prec=C.PREC_FP32, tol=1e-300, max_iter=30, info=info)
+    its, berr = C.dposv_mixed(dA, N, NB, dB, X, prec=C.PREC_FP32, tol=1e-300, max_iter=30, info=info, check=False)
+    assert int(info.item()) == 0
+    assert 2 <= its <= 8, its
+    Xh = X.t().cpu().numpy().T
+    assert oracle.solve_backward_error(N, NB, A0, Xh, Bh) <= 1e-15
+    assert berr <= 1e-15
+
+
 @pytest.mark.parametrize("prec,kappa,shift", [(3, 1e4, 0.0), (4, 1e4, 0.0), (3, 1e6, 1e-3), (1, 1e4, 0.0)])
 def test_krylov_solve_reaches_tolerance(dev, oracle, prec, kappa, shift):
     """Preconditioned CG with the low-precision factor (FP16 or BF16 single

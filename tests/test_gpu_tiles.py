@@ -180,10 +180,51 @@ def test_group_launches_match_single_launches(dev):
             with C.group():
                 body()
             assert C.group_last_launches() == 5  # potrf, dinv+trsm, two (alpha, beta) update batches
+            assert C.group_last_splits() == 0  # independent calls: one part
         else:
             body()
         torch.cuda.synchronize()
         return [x.cpu() for x in a + b + c] + [info.cpu()]
+
+    for x, y in zip(run(False), run(True)):
+        assert torch.equal(x, y)
+
+
+def test_group_keeps_dependent_calls_in_order(dev):
+    """A task body that issues dependent calls inside one group -- SYRK into a
+    tile, then POTRF of it, then TRSM of a panel by the new factor, then GEMM
+    from that panel -- gets them in program order: the group launches what it
+    holds whenever a call reads or writes what an earlier one writes (or
+    writes what one reads), so the result is bitwise the one of separate
+    launches, and each dependent step split the group once."""
+    import ttg_amd.chol as C
+
+    rng = np.random.default_rng(21)
+    n = 256
+    A0 = spd(n, rng)
+    W0 = rng.uniform(-1, 1, (n, n)) * 0.05
+    P0 = rng.uniform(-1, 1, (n, n))
+    G0 = rng.uniform(-1, 1, (n, n))
+
+    def run(grouped):
+        a, w, p, g = (dev_colmajor(x, dev) for x in (A0, W0, P0, G0))
+        info = torch.zeros(1, dtype=torch.int32, device=dev)
+
+        def body():
+            C.dsyrk_ln(w, a)          # a -= w w^T (lower)
+            C.dpotrf_tile(a, info)    # factor the updated tile
+            C.dtrsm_rlt(a, p)         # p := p a^{-T}
+            C.dgemm_nt(p, p, g)       # g -= p p^T
+
+        if grouped:
+            with C.group():
+                body()
+            assert C.group_last_splits() == 3
+        else:
+            body()
+        torch.cuda.synchronize()
+        assert int(info.item()) == 0
+        return [x.cpu() for x in (a, p, g)]
 
     for x, y in zip(run(False), run(True)):
         assert torch.equal(x, y)

@@ -34,6 +34,7 @@ SIGNATURES = {
     "ttg_chol_group_begin": (_i, [_p]),
     "ttg_chol_group_end": (_i, [_p]),
     "ttg_chol_group_last_launches": (_i, []),
+    "ttg_chol_group_last_splits": (_i, []),
     "ttg_chol_dpotrf_tiled": (_i, [_p, _i, _i, _p, _p, _u]),
     "ttg_chol_dpotrs_tiled": (_i, [_p, _i, _i, _p, _i, _p, _i]),
     "ttg_chol_dresidual_tiled": (_i, [_p, _i, _i, _p, _i, _p, _i, _p, _i, _p, _i]),
@@ -384,6 +385,12 @@ class group:
 
 def group_last_launches():
     return lib().ttg_chol_group_last_launches()
+
+
+def group_last_splits():
+    """How often the last group launched early for a call that depended on
+    an earlier call of the group."""
+    return lib().ttg_chol_group_last_splits()
 
 
 # ---------------------------------------------------------------- tiled

@@ -12,6 +12,7 @@
 
 #include "chol_internal.hpp"
 #include "chol_kernels.hpp"
+#include "chol_overlap.hpp"
 #include "chol_tileops.hpp"
 
 namespace ttgc {
@@ -158,7 +159,9 @@ struct Group {
   };
   std::vector<Write> writes;  // outputs of the POINV tile operations
   std::vector<PlgsyOp> plgsy;
-  int last_launches = 0;
+  Footprints fp;       // what the recorded calls read and write
+  int last_launches = 0;  // launches of the last group (all its parts)
+  int splits = 0;         // parts run early because a call depended on an earlier one
 
   void clear() {
     potrf.clear();
@@ -183,6 +186,7 @@ struct Group {
     norm_after.clear();
     writes.clear();
     plgsy.clear();
+    fp.clear();
   }
   // a work tile of m x n doubles (leading dimension m), as an offset
   double *work_tile(int m, int n) {
@@ -219,12 +223,6 @@ bool group_defer_nrm2(hipStream_t s, int n, const double *x, double *r) {
   for (auto &w : g.writes) written = written || overlaps(x, n, w.p, w.m, w.n, w.ld);
   if (written) g.norm_after.push_back(Group::Norm{n, x, r});
   return written;
-}
-
-bool group_record_plgsy(hipStream_t s, const PlgsyOp &op) {
-  if (!grouping(s)) return false;
-  g_group.plgsy.push_back(op);
-  return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -342,10 +340,7 @@ static int run_group(Group &g, hipStream_t s) {
   for (auto &d : g.trsm_f_dinv) dinv_bytes += sizeof(float) * dinv_doubles(d.n);
   const size_t dinv_off = place(dinv_bytes);
   const size_t work_off = place(g.work);
-  if (off == 0) {
-    g.last_launches = 0;
-    return TTG_CHOL_OK;
-  }
+  if (off == 0) return TTG_CHOL_OK;
   TTGC_HIP(w.ops.reserve(off));
   char *base = static_cast<char *>(w.ops.ptr);
   char *work = base + work_off;
@@ -528,8 +523,38 @@ static int run_group(Group &g, hipStream_t s) {
     ++launches;
   }
   for (auto &nm : g.norm_after) TTGC_HIP(launch_nrm2(s, nm.n, nm.x, nm.r));
-  g.last_launches = launches;
+  g.last_launches += launches;
   return TTG_CHOL_OK;
+}
+
+// A call about to be recorded into the open group on s, with the given reads
+// and writes: if it reads what a recorded call writes, or writes what one
+// reads or writes (a task body that issues dependent calls, e.g. SYRK and
+// then POTRF on one tile), the group so far runs first, so that stream order
+// keeps the two in program order, and recording goes on in an empty group.
+template <size_t NI, size_t NO>
+static int group_admit(hipStream_t s, const Span (&ins)[NI], const Span (&outs)[NO]) {
+  Group &g = g_group;
+  if (g.fp.conflicts(ins, outs)) {
+    ++g.splits;
+    const int rc = run_group(g, s);
+    g.clear();
+    if (rc != TTG_CHOL_OK) return rc;
+  }
+  g.fp.add(ins, outs);
+  return TTG_CHOL_OK;
+}
+template <typename T>
+static Span blk(const T *p, long m, long n, long ld) {
+  return block_span(p, m, n, ld, sizeof(T));
+}
+
+bool group_record_plgsy(hipStream_t s, const PlgsyOp &op) {
+  if (!grouping(s)) return false;
+  const Span none[1], outs[1] = {block_span(op.A, op.m, op.n, op.lda, sizeof(double))};
+  if (group_admit(s, none, outs) != TTG_CHOL_OK) return false;  // the caller launches it alone
+  g_group.plgsy.push_back(op);
+  return true;
 }
 
 }  // namespace ttgc
@@ -612,6 +637,8 @@ int ttg_chol_dpotrf_tile(void *stream, int n, double *A, int lda, int *d_info, i
     op.n = 0;  // the kernel stores 0 into *d_info
   }
   if (grouping(s)) {
+    const Span none[1], outs[1] = {blk(A, n, n, lda)};
+    if (int rc = group_admit(s, none, outs)) return rc;
     g_group.potrf.push_back(op);
     return TTG_CHOL_OK;
   }
@@ -631,6 +658,8 @@ int ttg_chol_dtrsm_rlt(void *stream, int m, int n, const double *L, int ldl, dou
   TrsmOp<double> op{L, B, nullptr, m, n, ldl, ldb};
   PotrfOp dj{const_cast<double *>(L), nullptr, nullptr, n, ldl, 0, 0};
   if (grouping(s)) {
+    const Span ins[1] = {blk(L, n, n, ldl)}, outs[1] = {blk(B, m, n, ldb)};
+    if (int rc = group_admit(s, ins, outs)) return rc;
     g_group.trsm.push_back(op);
     g_group.trsm_dinv.push_back(dj);
     g_group.trsm_rel.push_back(0);
@@ -699,6 +728,8 @@ static int update_entry(const char *name, hipStream_t s, int m, int n, int k, do
   }
   UpdOp<double> op{A, B, C, m, n, k, lda, ldb, ldc, lower};
   if (grouping(s)) {
+    const Span ins[2] = {blk(A, m, k, lda), blk(B, n, k, ldb)}, outs[1] = {blk(C, m, n, ldc)};
+    if (int rc = group_admit(s, ins, outs)) return rc;
     g_group.add_upd(UpdKey{alpha, beta, false, false}, op, 0);
     return TTG_CHOL_OK;
   }
@@ -734,6 +765,8 @@ int ttg_chol_spotrf_tile(void *stream, int n, float *A, int lda, int *d_info, in
   };
   if (n == 0 && !d_info) return TTG_CHOL_OK;
   if (grouping(s)) {
+    const Span none[1], outs[1] = {blk(A, n, n, lda)};
+    if (int rc = group_admit(s, none, outs)) return rc;
     rec(g_group);
     return TTG_CHOL_OK;
   }
@@ -754,6 +787,8 @@ int ttg_chol_strsm_rlt(void *stream, int m, int n, const float *L, int ldl, floa
     g.trsm_f_dinv.push_back(PotrfOpT<float>{const_cast<float *>(L), nullptr, nullptr, n, ldl, 0, 0, nullptr});
   };
   if (grouping(s)) {
+    const Span ins[1] = {blk(L, n, n, ldl)}, outs[1] = {blk(B, m, n, ldb)};
+    if (int rc = group_admit(s, ins, outs)) return rc;
     rec(g_group);
     return TTG_CHOL_OK;
   }
@@ -772,6 +807,8 @@ static int update_f_entry(const char *name, hipStream_t s, int m, int n, int k, 
   const UpdOp<float> op{A, B, C, m, n, k, lda, ldb, ldc, lower};
   const UpdKey key{alpha, beta, false, false};
   if (grouping(s)) {
+    const Span ins[2] = {blk(A, m, k, lda), blk(B, n, k, ldb)}, outs[1] = {blk(C, m, n, ldc)};
+    if (int rc = group_admit(s, ins, outs)) return rc;
     g_group.upd_f[key].push_back(op);
     return TTG_CHOL_OK;
   }
@@ -810,6 +847,8 @@ int ttg_chol_dtrtri_tile(void *stream, int n, double *A, int lda, int *d_info, i
     return run_alone(s, [&](Group &g) { g.zero.push_back(z); });
   }
   if (grouping(s)) {
+    const Span none[1], outs[1] = {blk(A, n, n, lda)};
+    if (int rc = group_admit(s, none, outs)) return rc;
     rec_trtri(g_group, n, A, lda, d_info, info_offset);
     return TTG_CHOL_OK;
   }
@@ -831,6 +870,8 @@ int ttg_chol_dtrsm_lln(void *stream, int m, int n, double alpha, const double *L
   if (m == 0 || n == 0) return TTG_CHOL_OK;
   hipStream_t s = as_stream(stream);
   if (grouping(s)) {
+    const Span ins[1] = {blk(L, m, m, ldl)}, outs[1] = {blk(B, m, n, ldb)};
+    if (int rc = group_admit(s, ins, outs)) return rc;
     rec_trsm_lln(g_group, m, n, alpha, L, ldl, B, ldb);
     return TTG_CHOL_OK;
   }
@@ -843,6 +884,8 @@ int ttg_chol_dtrsm_rln(void *stream, int m, int n, double alpha, const double *L
   if (m == 0 || n == 0) return TTG_CHOL_OK;
   hipStream_t s = as_stream(stream);
   if (grouping(s)) {
+    const Span ins[1] = {blk(L, n, n, ldl)}, outs[1] = {blk(B, m, n, ldb)};
+    if (int rc = group_admit(s, ins, outs)) return rc;
     rec_trsm_rln(g_group, m, n, alpha, L, ldl, B, ldb);
     return TTG_CHOL_OK;
   }
@@ -858,6 +901,8 @@ int ttg_chol_dlauum_tile(void *stream, int n, double *A, int lda) {
   if (n == 0) return TTG_CHOL_OK;
   hipStream_t s = as_stream(stream);
   if (grouping(s)) {
+    const Span none[1], outs[1] = {blk(A, n, n, lda)};
+    if (int rc = group_admit(s, none, outs)) return rc;
     rec_lauum(g_group, n, A, lda);
     return TTG_CHOL_OK;
   }
@@ -870,6 +915,8 @@ int ttg_chol_dtrmm_llt(void *stream, int m, int n, double alpha, const double *L
   if (m == 0 || n == 0) return TTG_CHOL_OK;
   hipStream_t s = as_stream(stream);
   if (grouping(s)) {
+    const Span ins[1] = {blk(L, m, m, ldl)}, outs[1] = {blk(B, m, n, ldb)};
+    if (int rc = group_admit(s, ins, outs)) return rc;
     rec_trmm_llt(g_group, m, n, alpha, L, ldl, B, ldb);
     return TTG_CHOL_OK;
   }
@@ -892,6 +939,9 @@ static int update_tb_entry(const char *name, hipStream_t s, bool ta, int m, int 
   UpdOp<double> op{A, B, C, m, n, k, lda, ldb, ldc, lower};
   const UpdKey key{alpha, beta, ta, true};
   if (grouping(s)) {
+    const Span ins[2] = {ta ? blk(A, k, m, lda) : blk(A, m, k, lda), blk(B, k, n, ldb)},
+               outs[1] = {blk(C, m, n, ldc)};
+    if (int rc = group_admit(s, ins, outs)) return rc;
     g_group.add_upd(key, op, 0);
     return TTG_CHOL_OK;
   }
@@ -925,10 +975,14 @@ int ttg_chol_group_begin(void *stream) {
   g_group.active = true;
   g_group.stream = as_stream(stream);
   g_group.clear();
+  g_group.last_launches = 0;
+  g_group.splits = 0;
   return TTG_CHOL_OK;
 }
 
 int ttg_chol_group_last_launches(void) { return g_group.last_launches; }
+
+int ttg_chol_group_last_splits(void) { return g_group.splits; }
 
 int ttg_chol_group_end(void *stream) {
   clear_error();

@@ -439,9 +439,11 @@ struct MixedPlan : MixedPlanBase {
     if (rc) return rc;
     rc = solve(s, B, ldb, X, ldx, false);
     if (rc) return rc;
-    // 4. refinement
-    double e = INFINITY;
-    int it = 0;
+    // 4. refinement; it stops early (ENOCONV) on a NaN backward error or
+    // after two steps in a row that did not halve it (a diverging or
+    // stagnating refinement would otherwise run all max_iter steps)
+    double e = INFINITY, e_prev = INFINITY;
+    int it = 0, slow = 0;
     for (;; ++it) {
       rc = residual_fp64(s, N, NB, A, nrhs, X, ldx, B, ldb, r(), N);
       if (rc) return rc;
@@ -452,7 +454,10 @@ struct MixedPlan : MixedPlanBase {
       if (rc) return rc;
       e = 0.0;
       for (int j = 0; j < nrhs; ++j) e = (hstats[1 + j] > e || std::isnan(hstats[1 + j])) ? hstats[1 + j] : e;
-      if (e <= tol || it >= max_iter) break;
+      if (e <= tol || it >= max_iter || std::isnan(e)) break;
+      slow = (it > 0 && !(e < 0.5 * e_prev)) ? slow + 1 : 0;
+      if (slow >= 2) break;
+      e_prev = e;
       rc = solve(s, r(), N, X, ldx, true);
       if (rc) return rc;
     }

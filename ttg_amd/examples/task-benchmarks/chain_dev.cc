@@ -6,6 +6,8 @@
 // the reference's lines are printed:
 //   # of tasks = <n>
 //   time elapsed (microseconds) = <t>, avg <t / n>
+// and, between them, "# of tasks run = <count>" from the runtime's own
+// count of the timed run (n + 1 when the chain runs to its end).
 // Usage: chain_ttg_dev [F] [-n <tasks>]   (defaults: F = 0, 100000 tasks)
 // No task launches a kernel, so the time is the runtime's own: a scheduling
 // round per task (its launch group and completion event), the buffer
@@ -53,16 +55,20 @@ struct KeyChain {
   }
 };
 
+long g_tasks_run = 0;  // tasks the runtime ran in the timed run
+
 template <typename Init>
 double run_twice(Init &init) {
   init->invoke();
   ttg::execute();
   ttg::fence();
+  const long before = ttg::detail::Runtime::get().tasks_run;
   const auto t0 = std::chrono::high_resolution_clock::now();
   init->invoke();
   ttg::execute();
   ttg::fence();
   const auto t1 = std::chrono::high_resolution_clock::now();
+  g_tasks_run = ttg::detail::Runtime::get().tasks_run - before;
   return std::chrono::duration_cast<std::chrono::microseconds>(t1 - t0).count();
 }
 
@@ -112,6 +118,9 @@ int main(int argc, char **argv) {
   if (us >= 0.0) {
     // the key-only chain runs tasks 0..n, the value chains likewise
     std::cout << "# of tasks = " << g_ntasks << std::endl;
+    // the chain's own count of the tasks that ran (keys 0 .. n; the init
+    // body runs at invoke(), outside the runtime)
+    std::cout << "# of tasks run = " << g_tasks_run << std::endl;
     std::cout << "time elapsed (microseconds) = " << us << ", avg " << us / (double)g_ntasks << std::endl;
   }
   ttg::finalize();
