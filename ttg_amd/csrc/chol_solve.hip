@@ -238,19 +238,22 @@ int SolvePlan<T>::build() {
       upd.push_back(UpdOp<T>{rows(i), tile(i, m), xrows(m), nrhs, dim(m), dim(i), nrhs, NB, nrhs, 0});
     bw_upd_cnt.push_back((int)upd.size() - bw_upd_off.back());
   }
-  // the persistent sweeps' first off-diagonal products (prepare() forms
-  // them): P_m = Linv_m L_{m,m-1} and R_i = L_{i+1,i} Linv_i
+  // the persistent sweeps' folded off-diagonal products (prepare() forms
+  // them): P_{m,j} = Linv_m L_{m,m-j} and R_{i,j} = L_{i+j,i} Linv_i
   std::vector<UpdOp<T>> pr;
   if (persist && NT > 1) {
-    TTGC_HIP(d_pr.reserve(sizeof(T) * (size_t)2 * (NT - 1) * NB * NB));
-    T *pbase = static_cast<T *>(d_pr.ptr), *rbase = pbase + (long)(NT - 1) * NB * NB;
-    for (int m = 1; m < NT; ++m)  // C = Linv_m B^T, B(c, k) = L_{m,m-1}(k, c)
-      pr.push_back(UpdOp<T>{li(m), tile(m, m - 1), pbase + (long)(m - 1) * NB * NB, dim(m), dim(m - 1), dim(m), NB,
-                            NB, NB, 0});
+    const long per = (long)(NT - 1) * NB * NB;  // tiles of one j and direction
+    TTGC_HIP(d_pr.reserve(sizeof(T) * (size_t)2 * kSweepDepth * per));
+    T *pbase = static_cast<T *>(d_pr.ptr), *rbase = pbase + kSweepDepth * per;
+    for (int j = 1; j <= kSweepDepth; ++j)
+      for (int m = j; m < NT; ++m)  // C = Linv_m B^T, B(c, k) = L_{m,m-j}(k, c)
+        pr.push_back(UpdOp<T>{li(m), tile(m, m - j), pbase + (j - 1) * per + (long)(m - j) * NB * NB, dim(m),
+                              dim(m - j), dim(m), NB, NB, NB, 0});
     n_pr = (int)pr.size();
-    for (int i = 0; i + 1 < NT; ++i)  // C = L_{i+1,i} B^T, B(c, k) = Linv_i(k, c)
-      pr.push_back(
-          UpdOp<T>{tile(i + 1, i), li(i), rbase + (long)i * NB * NB, dim(i + 1), dim(i), dim(i), NB, NB, NB, 0});
+    for (int j = 1; j <= kSweepDepth; ++j)
+      for (int i = 0; i + j < NT; ++i)  // C = L_{i+j,i} B^T, B(c, k) = Linv_i(k, c)
+        pr.push_back(UpdOp<T>{tile(i + j, i), li(i), rbase + (j - 1) * per + (long)i * NB * NB, dim(i + j), dim(i),
+                              dim(i), NB, NB, NB, 0});
   }
   auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
   o_dj = 0;
@@ -264,6 +267,7 @@ int SolvePlan<T>::build() {
   memcpy(host.data() + o_inv, inv.data(), inv.size() * sizeof(TrsmOp<T>));
   if (!upd.empty()) memcpy(host.data() + o_upd, upd.data(), upd.size() * sizeof(UpdOp<T>));
   if (!pr.empty()) memcpy(host.data() + o_pr, pr.data(), pr.size() * sizeof(UpdOp<T>));
+  n_pr_all = pr.size();
   TTGC_HIP(hipMemcpy(d_ops.ptr, host.data(), total, hipMemcpyHostToDevice));
   host_ops.assign(upd.begin(), upd.end());
   TTGC_HIP(hipStreamCreateWithFlags(&s_main, hipStreamNonBlocking));
@@ -296,10 +300,10 @@ int SolvePlan<T>::prepare(hipStream_t s) {
   hipLaunchKernelGGL(identity_tiles_kernel<T>, dim3(64, NT), dim3(256), 0, s, N, NB, static_cast<T *>(d_linv.ptr));
   TTGC_HIP(hipGetLastError());
   TTGC_HIP(launch_trsm<T>(s, inv_ops(), NT, tnone, NB, nullptr, false));
-  if (persist && n_pr > 0) {
+  if (persist && n_pr_all > 0) {
     UpdOp<T> unone{};
     const bool fast = NB % 128 == 0 && N % NB == 0;
-    TTGC_HIP(launch_update<T>(s, pr_ops(), 2 * n_pr, unone, NB, NB, T(1), T(0), fast, nullptr, true));
+    TTGC_HIP(launch_update<T>(s, pr_ops(), (int)n_pr_all, unone, NB, NB, T(1), T(0), fast, nullptr, true));
   }
   return TTG_CHOL_OK;
 }
@@ -322,7 +326,7 @@ int SolvePlan<T>::enqueue_sweeps(hipStream_t s) {
       SweepArgs<T> a{tiles,
                      static_cast<const T *>(d_linv.ptr),
                      pm,
-                     pm + (long)(NT - 1) * NB * NB,
+                     pm + (long)kSweepDepth * (NT - 1) * NB * NB,
                      bt() + j0,
                      static_cast<T *>(d_xt.ptr) + j0,
                      static_cast<int *>(d_ctr.ptr),

@@ -32,9 +32,10 @@ struct SolvePlan {
   int sweep_grid = 0;
   long long sweep_timeout = 0;
   DevBuf d_ctr;
-  DevBuf d_pr;  // P_m (NT-1 tiles) then R_i (NT-1 tiles), from prepare()
-  size_t o_pr = 0;  // offset of the P and R descriptors in d_ops
+  DevBuf d_pr;  // the folded products P_{m,j}, then R_{i,j} (kSweepDepth (NT-1) tiles each), from prepare()
+  size_t o_pr = 0;  // offset of their descriptors in d_ops
   int n_pr = 0;     // P descriptors (the R descriptors follow)
+  size_t n_pr_all = 0;
   hipStream_t s_main = nullptr;  // the stream the sweep graph is captured on
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;

@@ -4,21 +4,25 @@
 //
 // The right-hand sides (up to 16 per launch) are held transposed, row i of
 // the block vector at i * ld.  With Linv_k = L_kk^{-1} and the products of
-// the first off-diagonal tiles P_m = Linv_m L_{m,m-1}, R_i = L_{i+1,i} Linv_i
-// (both from the plan's prepare step), the sweeps are
-//   forward   y_m = d_m - P_m y_{m-1},  d_m = Linv_m (b_m - sum_{i<m-1} L_mi y_i)
-//   backward  x_i = e_i - R_i^T x_{i+1}, e_i = Linv_i^T (y_i - sum_{m>i+1} L_mi^T x_m)
+// the first D = kSweepDepth off-diagonal tiles of every block row and column
+// P_{m,j} = Linv_m L_{m,m-j}, R_{i,j} = L_{i+j,i} Linv_i (j = 1 .. D, from
+// the plan's prepare step), the sweeps are
+//   forward   y_m = d_m - sum_j P_{m,j} y_{m-j},
+//             d_m = Linv_m (b_m - sum_{i<m-D} L_mi y_i)
+//   backward  x_i = e_i - sum_j R_{i,j}^T x_{i+j},
+//             e_i = Linv_i^T (y_i - sum_{m>i+D} L_mi^T x_m)
 // so one block step of the critical chain is a single product of a
-// prefetched tile with the block just finished, while d_m (e_i) is formed
-// one step earlier.  Work is cut into units of 32 rows of one block row:
+// prefetched tile with the block just finished, while d_m (e_i), which
+// gathers the whole block, is formed D steps earlier.  Work is cut into
+// units of 32 rows of one block row:
 //   forward  unit (m, s), m = 0 .. NT-1 (y in xt; c_m in place in bt)
-//     phase A   c_m[s] = b_m[s] - sum_{i<m-1} L_mi[s, :] y_i
+//     phase A   c_m[s] = b_m[s] - sum_{i<m-D} L_mi[s, :] y_i
 //     phase B   d_m[s] = Linv_m[s, :] c_m          (all of c_m)
-//     phase C   y_m[s] = d_m[s] - P_m[s, :] y_{m-1}
+//     phase C   y_m[s] = d_m[s] - sum_j P_{m,j}[s, :] y_{m-j}
 //   backward unit (i, s), i = NT-1 .. 0 (x in bt; y updated in place in xt)
-//     phase A   c_i[s] = y_i[s] - sum_{m>i+1} L_mi[:, s]^T x_m
+//     phase A   c_i[s] = y_i[s] - sum_{m>i+D} L_mi[:, s]^T x_m
 //     phase B   e_i[s] = Linv_i[:, s]^T c_i
-//     phase C   x_i[s] = e_i[s] - R_i[:, s]^T x_{i+1}
+//     phase C   x_i[s] = e_i[s] - sum_j R_{i,j}[:, s]^T x_{i+j}
 // Workgroups take units from a device counter in this order.  A unit reads
 // only blocks finished by units with lower indices, except that phase B
 // waits for the phase A of its own block's other units; so as long as NB/32
@@ -57,7 +61,9 @@
 // Timing hook (no code unless a timing harness defines it): event k of
 // unit u.
 #ifndef TTGC_SWEEP_TRACE
-#define TTGC_SWEEP_TRACE(u, k)
+#define TTGC_SWEEP_TRACE(u, k) \
+  do {                         \
+  } while (0)
 #endif
 
 namespace ttgc {
@@ -91,6 +97,7 @@ __device__ __forceinline__ P opaque(P x) {
 template <typename T, bool FULL>
 __global__ __launch_bounds__(SweepCfg<T>::NW * 64) void sweep_kernel(SweepArgs<T> a) {
   constexpr int NW = SweepCfg<T>::NW;
+  constexpr int D = kSweepDepth;
   constexpr int KW = kSweepMaxNB / NW;  // k per wave
   static_assert(KW == sweep_wave_k<T>(), "k per wave");
   constexpr int NKS = KW / 4;           // rows form: k steps per wave
@@ -339,10 +346,11 @@ __global__ __launch_bounds__(SweepCfg<T>::NW * 64) void sweep_kernel(SweepArgs<T
     if (u < NT * S) {
       // ---------------- forward unit (m, s) ----------------
       const int m = u / S, s = u % S, r0 = s * kSweepRows, dm = dim(m);
+      const int jmax = min(D, m);  // folded tiles L_{m,m-j}, j = 1 .. jmax
       T *bm = a.bt + (long)m * NB * ld;
       load_pre_rows(bm, r0);  // b_m: written before the launch
       zero();
-      for (int i = 0; i + 1 < m; ++i) {
+      for (int i = 0; i < m - D; ++i) {
         const int di = dim(i);
         if (!rows_step(tile(m, i), r0, dm, di, a.xt + (long)i * NB * ld,
                        [&] { return wait_ge(fw_done, (i + 1) * S, known_fw); }))
@@ -361,29 +369,35 @@ __global__ __launch_bounds__(SweepCfg<T>::NW * 64) void sweep_kernel(SweepArgs<T
           }))
         return;
       reduce();  // d_m
-      if (m > 0) {
+      if (jmax > 0) {
         keep_acc();
         zero();
-        if (!rows_step(a.pm + (long)(m - 1) * NB * NB, r0, dm, dim(m - 1), a.xt + (long)(m - 1) * NB * ld, [&] {
-              const bool r = wait_ge(fw_done, m * S, known_fw);
-              TTGC_SWEEP_TRACE(u, 0);
-              return r;
-            }))
-          return;
+        // acc = sum_j P_{m,j} y_{m-j}, the farthest first: only the last
+        // product (j = 1) waits for the block just finished
+        for (int j = jmax; j >= 1; --j) {
+          if (!rows_step(a.pm + ((long)(j - 1) * (NT - 1) + (m - j)) * NB * NB, r0, dm, dim(m - j),
+                         a.xt + (long)(m - j) * NB * ld, [&] {
+                           const bool r = wait_ge(fw_done, (m - j + 1) * S, known_fw);
+                           if (j == 1) TTGC_SWEEP_TRACE(u, 0);
+                           return r;
+                         }))
+            return;
+        }
         reduce();
       }
-      store_rows(a.xt + (long)m * NB * ld, r0, m > 0);  // y_m = d_m - P_m y_{m-1}
+      store_rows(a.xt + (long)m * NB * ld, r0, jmax > 0);  // y_m = d_m - sum_j P_{m,j} y_{m-j}
       signal(fw_done);
       TTGC_SWEEP_TRACE(u, 3);
     } else {
       // ---------------- backward unit (i, s) ----------------
       const int v = u - NT * S;
       const int i = NT - 1 - v / S, s = v % S, c0 = s * kSweepRows, di = dim(i);
+      const int jmax = min(D, NT - 1 - i);  // folded tiles L_{i+j,i}, j = 1 .. jmax
       T *yi = a.xt + (long)i * NB * ld;
       if (!wait_ge(fw_done, NT * S, known_fw)) return;
       load_pre_cols(yi, c0);  // y_i: final since the forward sweep ended
       zero();
-      for (int m = NT - 1; m > i + 1; --m) {
+      for (int m = NT - 1; m > i + D; --m) {
         const int dm = dim(m);
         if (!cols_step(tile(m, i), c0, dm, di, a.bt + (long)m * NB * ld,
                        [&] { return wait_ge(bw_done, (NT - m) * S, known_bw); }))
@@ -402,18 +416,21 @@ __global__ __launch_bounds__(SweepCfg<T>::NW * 64) void sweep_kernel(SweepArgs<T
           }))
         return;
       reduce();  // e_i
-      if (i < NT - 1) {
+      if (jmax > 0) {
         keep_acc();
         zero();
-        if (!cols_step(a.rm + (long)i * NB * NB, c0, dim(i + 1), di, a.bt + (long)(i + 1) * NB * ld, [&] {
-              const bool r = wait_ge(bw_done, (NT - 1 - i) * S, known_bw);
-              TTGC_SWEEP_TRACE(u, 0);
-              return r;
-            }))
-          return;
+        for (int j = jmax; j >= 1; --j) {
+          if (!cols_step(a.rm + ((long)(j - 1) * (NT - 1) + i) * NB * NB, c0, dim(i + j), di,
+                         a.bt + (long)(i + j) * NB * ld, [&] {
+                           const bool r = wait_ge(bw_done, (NT - i - j) * S, known_bw);
+                           if (j == 1) TTGC_SWEEP_TRACE(u, 0);
+                           return r;
+                         }))
+            return;
+        }
         reduce();
       }
-      store_cols(a.bt + (long)i * NB * ld, c0, i < NT - 1);  // x_i = e_i - R_i^T x_{i+1}
+      store_cols(a.bt + (long)i * NB * ld, c0, jmax > 0);  // x_i = e_i - sum_j R_{i,j}^T x_{i+j}
       signal(bw_done);
       TTGC_SWEEP_TRACE(u, 3);
     }

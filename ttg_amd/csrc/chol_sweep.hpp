@@ -6,6 +6,9 @@
 namespace ttgc {
 
 constexpr int kSweepRows = 32;     // rows of one work unit
+// off-diagonal tiles folded per block step (2 measured no faster: the extra
+// slack goes to the whole-block phase, which no longer limits)
+constexpr int kSweepDepth = 1;
 constexpr int kSweepMaxNB = 512;   // largest tile the kernel's registers cover
 constexpr int kSweepMaxRhs = 16;   // right-hand sides per launch (one MFMA column block)
 constexpr int kSweepStride = 64;   // ints between two counters (256 bytes)
@@ -31,8 +34,8 @@ template <typename T>
 struct SweepArgs {
   const T *tiles;  // lower tile-packed factor
   const T *linv;   // NT inverses of the diagonal tiles (NB x NB, column-major)
-  const T *pm;     // P_m = Linv_m L_{m,m-1} at (m - 1) NB^2, m = 1 .. NT-1
-  const T *rm;     // R_i = L_{i+1,i} Linv_i at i NB^2, i = 0 .. NT-2
+  const T *pm;     // P_{m,j} = Linv_m L_{m,m-j} at ((j-1)(NT-1) + m-j) NB^2, j = 1 .. kSweepDepth
+  const T *rm;     // R_{i,j} = L_{i+j,i} Linv_i at ((j-1)(NT-1) + i) NB^2
   T *bt;           // Np rows of nrhs values, row i at i * ld: b in, x out
   T *xt;           // Np rows (same layout): y (work)
   int *ctr;        // sweep_counter_ints(NT) ints (zero before the first launch)
