@@ -20,6 +20,7 @@ calling stream); the median wins.  Results go to a JSON cache
 ($TTG_AMD_TUNING, default ~/.cache/ttg_amd/tuning.json) keyed by the device
 name and the workload signature, so later runs reuse them without measuring.
 """
+import ctypes
 import json
 import os
 import statistics
@@ -87,22 +88,35 @@ def _time_events(fn, warmup, reps, reset=None):
 
 def synthetic_batch(sizes, seed=0, device="cuda"):
     """FP32 SPD matrices B B^T / n + I (B uniform in [-1, 1]) packed back to
-    back (column-major, lda = n) and uniform right-hand sides, on the device."""
+    back (column-major, lda = n) and uniform right-hand sides, on the device.
+    The products run on the library's own FP32 update kernel, all matrices in
+    one launch group (no vendor BLAS)."""
     import torch
 
+    import ttg_amd.chol as C
+
     g = torch.Generator(device=device).manual_seed(seed)
-    total = sum(int(n) * int(n) for n in sizes)
-    A = torch.empty(total, dtype=torch.float32, device=device)
-    b = torch.empty(sum(int(n) for n in sizes), dtype=torch.float32, device=device)
-    oa = ob = 0
-    for n in sizes:
-        n = int(n)
-        B = torch.rand(n, n, generator=g, device=device, dtype=torch.float32) * 2 - 1
-        M = B @ B.t() / n + torch.eye(n, device=device, dtype=torch.float32)
-        A[oa:oa + n * n] = M.reshape(-1)
-        b[ob:ob + n] = torch.rand(n, generator=g, device=device, dtype=torch.float32) * 2 - 1
+    sizes = [int(n) for n in sizes]
+    total = sum(n * n for n in sizes)
+    Bs = torch.rand(total, generator=g, device=device, dtype=torch.float32) * 2 - 1
+    b = torch.rand(sum(sizes), generator=g, device=device, dtype=torch.float32) * 2 - 1
+    A = torch.zeros(total, dtype=torch.float32, device=device)
+    diag, oa = [], 0
+    for n in sizes:  # A := I
+        diag.append(torch.arange(n, dtype=torch.int64) * (n + 1) + oa)
         oa += n * n
-        ob += n
+    A[torch.cat(diag).to(device)] = 1.0
+    fp = ctypes.c_float
+    lib = C.lib()
+    st = torch.cuda.current_stream(device)
+    sp = ctypes.c_void_p(st.cuda_stream)
+    with C.group(st):
+        oa = 0
+        for n in sizes:  # A += (1/n) B B^T
+            pa, pb = A.data_ptr() + 4 * oa, Bs.data_ptr() + 4 * oa
+            C._check(lib.ttg_chol_sgemm_nt(sp, n, n, n, fp(1.0 / n), pb, n, pb, n, fp(1.0), pa, n), "sgemm_nt")
+            oa += n * n
+    del Bs
     return A, b
 
 
