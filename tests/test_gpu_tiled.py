@@ -152,7 +152,8 @@ def test_dpotrf_tiled_info_nonspd(dev, oracle, N, NB, k, local):
     assert int(info.item()) == 0
 
 
-@pytest.mark.parametrize("N,NB,nrhs", [(160, 32, 1), (1000, 128, 3), (2048, 512, 16), (1536, 256, 40)])
+@pytest.mark.parametrize("N,NB,nrhs", [(160, 32, 1), (1000, 128, 3), (2048, 512, 16), (1536, 256, 40),
+                                       (2600, 512, 20), (1024, 512, 1)])
 def test_dpotrs_and_residual_vs_oracle(dev, oracle, N, NB, nrhs):
     import ttg_amd.chol as C
 

@@ -186,6 +186,43 @@ __device__ __forceinline__ int xcd_remap(int b, int G) {
   return x * q + (x < r ? x : r) + pos;
 }
 
+// s_waitcnt vmcnt(N), expcnt and lgkmcnt not waited for (gfx9 encoding:
+// vmcnt bits [3:0] and [15:14], expcnt [6:4], lgkmcnt [11:8])
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+// s_waitcnt lgkmcnt(0) only: this wave's LDS reads and writes are done
+__device__ __forceinline__ void wait_lgkmcnt0() { __builtin_amdgcn_s_waitcnt(0xc07f); }
+
+// s_barrier without the fence of __syncthreads (which would wait for every
+// load in flight); the empty asm keeps the compiler from moving memory
+// operations across it
+__device__ __forceinline__ void raw_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// LDS-DMA loads (HBM -> LDS, no VGPRs: lane l's 4 or 16 bytes land at the
+// wave's LDS base + 4 l or 16 l) written as inline asm, so that the compiler
+// does not track them: its tracking makes every LDS read that it cannot
+// prove distinct from an LDS-DMA target wait for all LDS DMA in flight,
+// which drains a multi-stage ring.  Every wait for these loads is therefore
+// explicit (wait_vmcnt; they count on vmcnt in issue order with the other
+// vector memory operations).  M0 holds the LDS base; one wait state between
+// the write of M0 and the load.
+__device__ __forceinline__ void lds_dma_dword(const void *g, const void *l) {
+  const unsigned la = (unsigned)(uintptr_t)(const __attribute__((address_space(3))) void *)l;
+  asm volatile("s_nop 0\n\tglobal_load_lds_dword %0, off" ::"v"(g), "{m0}"(la) : "memory");
+}
+__device__ __forceinline__ void lds_dma_dwordx4(const void *g, const void *l) {
+  const unsigned la = (unsigned)(uintptr_t)(const __attribute__((address_space(3))) void *)l;
+  asm volatile("s_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "{m0}"(la) : "memory");
+}
+
 // Make this workgroup's global stores visible to its own later global loads
 // (the POTRF, TRSM and batched kernels re-read data they wrote).  All waves
 // of a workgroup run on one CU and share its vector L1, so a workgroup-scope

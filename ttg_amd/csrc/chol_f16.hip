@@ -104,23 +104,6 @@ constexpr int kF16BM = 128, kF16BK = 32;  // LDS rows of 32 halves = 4 chunks of
 // 128-byte bank window (rows alternate halves, the XOR walks the chunks).
 __device__ __forceinline__ int swz(int row, int chunk) { return chunk ^ ((row >> 1) & 3); }
 
-// s_waitcnt vmcnt(N), expcnt and lgkmcnt not waited for (gfx9 encoding:
-// vmcnt bits [3:0] and [15:14], expcnt [6:4], lgkmcnt [11:8])
-template <int N>
-__device__ __forceinline__ void wait_vmcnt() {
-  static_assert(N >= 0 && N < 64, "vmcnt range");
-  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
-}
-
-// s_barrier without the fence of __syncthreads (which would wait for every
-// load in flight); the empty asm keeps the compiler from moving memory
-// operations across it
-__device__ __forceinline__ void raw_barrier() {
-  asm volatile("" ::: "memory");
-  __builtin_amdgcn_s_barrier();
-  asm volatile("" ::: "memory");
-}
-
 typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
 
 // SPS slabs per LDS stage (one barrier per stage): the single-product modes

@@ -449,12 +449,22 @@ __global__ void residual_rows_kernel(int N, int Np, const double *__restrict__ X
 // both on FP64 MFMA from 64 x 64 blocks of A staged in LDS (each element of A
 // read from HBM once), stored row-major to the tile's slots W[t][0|1][NB][16].
 // Wave w owns rows w*16..w*16+15 of every 64-row block for P and of every
-// 64-column block for Q.
-__global__ __launch_bounds__(256) void residual_tile_kernel(int N, int NB, int nt, const double *__restrict__ tiles,
+// 64-column block for Q.  The general form: any NB, diagonal and ragged
+// tiles.  rest = 0: workgroup b takes tile b of the lower tiles; rest = 1:
+// only the tiles residual_strip_kernel leaves (diagonal tiles b < nt, then
+// the last block row's when N is not a multiple of NB).
+__global__ __launch_bounds__(256) void residual_tile_kernel(int N, int NB, int nt, int rest,
+                                                            const double *__restrict__ tiles,
                                                             const double *__restrict__ Xr, double *__restrict__ W) {
-  const long t = blockIdx.x;
   int m, n;
-  {
+  long t;
+  if (rest) {
+    const int b = blockIdx.x;
+    m = b < nt ? b : nt - 1;
+    n = b < nt ? b : b - nt;
+    t = tile_index(m, n, nt);
+  } else {
+    t = blockIdx.x;
     int lo = 0, hi = nt - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) / 2;
@@ -576,6 +586,166 @@ __global__ __launch_bounds__(256) void residual_tile_kernel(int N, int NB, int n
   }
 }
 
+// The same products for the full off-diagonal tiles of NB = 512 (all but
+// the diagonal tiles and a ragged last block row), written for HBM
+// streaming.  A tile is read as 64 blocks of 16 columns x 256 rows (strip s
+// = block / 2, half h = block % 2): each column of a block is 2 KB
+// contiguous in HBM.  The blocks go HBM -> LDS by LDS-DMA loads through a
+// ring of 4 stages (3 blocks in flight), one barrier per block.  Each of
+// the 8 waves takes 32 rows of a block for both products: P (rows 256 h +
+// 32 w + 16 i + (0..15), i = 0, 1; 4 k-slices of the strip's columns) and
+// Q (the strip's 16 columns; 8 k-slices of its 32 rows, whose X_m operands
+// stay in registers for the whole tile).  The operands of the next block
+// are read into registers while this block's MFMAs run.  P accumulates
+// over the columns in the order of residual_tile_kernel (bitwise the same
+// P); Q of a strip is the sum of the 8 waves' partials in wave order.
+constexpr int kStripNB = 512;
+constexpr int kStripLd = 257;                          // LDS column stride of a block (doubles)
+constexpr int kStripXn = 16 * kStripLd;                // X_n rows of the strip (16 x 16)
+constexpr int kStripStage = kStripXn + 16 * kRhsChunk;  // doubles per stage
+constexpr int kStripLoads = 17;                        // LDS-DMA loads per wave per block
+constexpr int kStripBlocks = 2 * (kStripNB / 16);      // 64 blocks per tile
+
+struct StripOps {
+  double a[20];  // P operands of A [i][ks] (8), Q operands of A [ks] (8), X_n operands [ks] (4)
+};
+
+__global__ __launch_bounds__(512) void residual_strip_kernel(int nt, int nfull, const double *__restrict__ tiles,
+                                                             const double *__restrict__ Xr, double *__restrict__ W) {
+  // workgroup b -> full off-diagonal tile (m, n), n < m < nfull, column by column
+  const int b = blockIdx.x;
+  int n, m;
+  {
+    int lo = 0, hi = nfull - 2;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) / 2;
+      if (mid * (nfull - 1) - mid * (mid - 1) / 2 <= b)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    n = lo;
+    m = n + 1 + (b - (n * (nfull - 1) - n * (n - 1) / 2));
+  }
+  const long t = tile_index(m, n, nt);
+  const double *A = tiles + t * (long)kStripNB * kStripNB;
+  double *W0 = W + t * 2L * kRhsChunk * kStripNB;
+  double *W1 = W0 + (long)kRhsChunk * kStripNB;
+  const double *Xn = Xr + (long)n * kStripNB * kRhsChunk;
+  const double *Xm = Xr + (long)m * kStripNB * kRhsChunk;
+  __shared__ __align__(16) double lds[4][kStripStage];
+  __shared__ __align__(16) double qpart[8][256];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lr = lane & 15, lk = lane >> 4;
+  // block j into stage st: wave w loads rows 32 w .. 32 w + 31 of the 16
+  // columns (one 4-byte word per lane, 256 bytes per load) and rows 16 s +
+  // 2 w, + 1 of X_n
+  auto issue = [&](int j, int st) {
+    const int s = j >> 1, h = j & 1;
+    double *sb = lds[st];
+    const char *src =
+        reinterpret_cast<const char *>(A + (long)(16 * s) * kStripNB + 256 * h + 32 * w + (lane >> 1)) + (lane & 1) * 4;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) lds_dma_dword(src + (long)c * kStripNB * 8, sb + c * kStripLd + 32 * w);
+    lds_dma_dword(reinterpret_cast<const char *>(Xn + (long)(16 * s + 2 * w) * kRhsChunk) + lane * 4,
+                  sb + kStripXn + 32 * w);
+  };
+  auto read = [&](int st, StripOps &o) {
+    const double *sb = lds[st];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) o.a[4 * i + ks] = sb[(4 * ks + lk) * kStripLd + 32 * w + 16 * i + lr];
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) o.a[8 + ks] = sb[lr * kStripLd + 32 * w + 4 * ks + lk];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) o.a[16 + ks] = sb[kStripXn + (4 * ks + lk) * kRhsChunk + lr];
+  };
+  d4_t pacc[2][2];  // [h][i]
+  d4_t qacc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int i = 0; i < 2; ++i) pacc[h][i] = d4_t{0.0, 0.0, 0.0, 0.0};
+  double xq[2][8];  // X_m rows 256 h + 32 w + 4 ks + lk
+  auto mma = [&](int h, const StripOps &o) {
+    auto body = [&](d4_t &c0, d4_t &c1, const double(&x)[8]) {
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        c0 = Mfma<double>::mma(o.a[ks], o.a[16 + ks], c0);
+        c1 = Mfma<double>::mma(o.a[4 + ks], o.a[16 + ks], c1);
+        qacc = Mfma<double>::mma(o.a[8 + 2 * ks], x[2 * ks], qacc);
+        qacc = Mfma<double>::mma(o.a[9 + 2 * ks], x[2 * ks + 1], qacc);
+      }
+    };
+    if (h == 0)
+      body(pacc[0][0], pacc[0][1], xq[0]);
+    else
+      body(pacc[1][0], pacc[1][1], xq[1]);
+  };
+  // Q of strip s: the 8 partials summed in wave order, 32 elements per wave
+  auto q_sum = [&](int s) {
+    if (lane < 32) {
+      const int e = 32 * w + lane, pl = e & 63, pr = e >> 6;
+      double v = 0.0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v += qpart[q][pl * 4 + pr];
+      W1[(long)(16 * s + Mfma<double>::drow(pl >> 4, pr)) * kRhsChunk + (pl & 15)] = v;
+    }
+  };
+  // blocks 0 .. 3 in flight, then this wave's X_m operands straight into
+  // registers; one wait for all
+  for (int j = 0; j < 4; ++j) issue(j, j);
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) xq[h][ks] = Xm[(long)(256 * h + 32 * w + 4 * ks + lk) * kRhsChunk + lr];
+  wait_vmcnt<0>();
+  raw_barrier();
+  StripOps ops[2];
+  read(0, ops[0]);
+  // Block j (operands in cur): wait for this wave's reads of block j's stage
+  // and for its loads of block j + 1 (blocks j + 2 and j + 3 may be
+  // outstanding; the W1 stores of q_sum go out before each iteration's
+  // loads, so at least those 34 loads are younger than block j + 1's), then
+  // one barrier publishes block j + 1 and frees block j's stage for j + 4.
+  auto step = [&](int j, StripOps &cur, StripOps &nxt) {
+    wait_lgkmcnt0();
+    if (j + 3 < kStripBlocks)
+      wait_vmcnt<2 * kStripLoads>();
+    else if (j + 2 < kStripBlocks)
+      wait_vmcnt<kStripLoads>();
+    else
+      wait_vmcnt<0>();
+    raw_barrier();
+    const int h = j & 1;
+    if (j >= 2 && h == 0) q_sum((j >> 1) - 1);
+    if (j + 4 < kStripBlocks) issue(j + 4, j & 3);
+    if (j + 1 < kStripBlocks) read((j + 1) & 3, nxt);
+    mma(h, cur);
+    if (h == 1) {  // the strip ends: this wave's Q partial
+#pragma unroll
+      for (int r = 0; r < 4; ++r) qpart[w][lane * 4 + r] = qacc[r];
+      qacc = d4_t{0.0, 0.0, 0.0, 0.0};
+    }
+  };
+  for (int j = 0; j < kStripBlocks; j += 2) {
+    step(j, ops[0], ops[1]);
+    step(j + 1, ops[1], ops[0]);
+  }
+  wait_lgkmcnt0();
+  raw_barrier();
+  q_sum(kStripBlocks / 2 - 1);
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        W0[(long)(256 * h + 32 * w + 16 * i + Mfma<double>::drow(lk, e)) * kRhsChunk + lr] = pacc[h][i][e];
+}
+
 // R(i, j) = B(i, j) - sum_{n <= m} P(m, n)[r][j] - sum_{n' >= m} Q(n', m)[r][j],
 // m = i / NB, r = i % NB: a fixed summation order (deterministic)
 __global__ void residual_reduce_kernel(int N, int NB, int nt, const double *__restrict__ W, int nr,
@@ -625,6 +795,8 @@ int residual_fp64(hipStream_t s, int N, int NB, const double *tiles, int nrhs, c
     ws = p.get();
   }
   const size_t wsz = (size_t)ntiles * 2 * kRhsChunk * NB;
+  // block rows of full NB = 512 tiles (the strip kernel's), 0 otherwise
+  const int nfull = NB == kStripNB ? N / NB : 0;
   TTGC_HIP(ws->reserve(sizeof(double) * (wsz + (size_t)Np * kRhsChunk)));
   double *W = static_cast<double *>(ws->ptr);
   double *Xr = W + wsz;
@@ -633,7 +805,16 @@ int residual_fp64(hipStream_t s, int N, int NB, const double *tiles, int nrhs, c
     hipLaunchKernelGGL(residual_rows_kernel, dim3((unsigned)((Np * kRhsChunk + 255) / 256)), dim3(256), 0, s, N,
                        (int)Np, X + (long)j0 * ldx, ldx, nr, Xr);
     TTGC_HIP(hipGetLastError());
-    hipLaunchKernelGGL(residual_tile_kernel, dim3((unsigned)ntiles), dim3(256), 0, s, N, NB, nt, tiles, Xr, W);
+    if (nfull >= 2) {
+      // full off-diagonal tiles on the strip kernel, the others on the general one
+      hipLaunchKernelGGL(residual_strip_kernel, dim3((unsigned)(nfull * (nfull - 1) / 2)), dim3(512), 0, s, nt, nfull,
+                         tiles, Xr, W);
+      TTGC_HIP(hipGetLastError());
+      hipLaunchKernelGGL(residual_tile_kernel, dim3((unsigned)(nt + (nfull < nt ? nt - 1 : 0))), dim3(256), 0, s, N,
+                         NB, nt, 1, tiles, Xr, W);
+    } else {
+      hipLaunchKernelGGL(residual_tile_kernel, dim3((unsigned)ntiles), dim3(256), 0, s, N, NB, nt, 0, tiles, Xr, W);
+    }
     TTGC_HIP(hipGetLastError());
     const long n = (long)N * nr;
     hipLaunchKernelGGL(residual_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, N, NB, nt, W, nr,
