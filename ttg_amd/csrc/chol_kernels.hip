@@ -36,14 +36,15 @@ constexpr int kMaxTile = 512;                 // largest single-tile POTRF/TRSM
 constexpr int kPotrfWaves = 8;                          // two waves per SIMD
 constexpr int kRowBlocks = kMaxTile / 16 / kPotrfWaves;  // 16-row blocks per wave
 
-template <typename TS>
+template <typename TS, typename TC>
 __global__ __launch_bounds__(kPotrfWaves * 64) void potrf_kernel(const PotrfOpT<TS> *__restrict__ ops, PotrfOpT<TS> one,
                                                     const int *__restrict__ d_abort) {
+  using acc_t = typename Mfma<TC>::acc_t;
   if (d_abort && *d_abort) return;
   const PotrfOpT<TS> op = ops ? ops[blockIdx.x] : one;
-  __shared__ double Ds[kPB][kPB + 1];
-  __shared__ double Di[kPB][kPB + 1];
-  __shared__ double Xs[kPotrfWaves][16][kPB + 1];  // per-wave staging of one 16 x 32 block
+  __shared__ TC Ds[kPB][kPB + 1];
+  __shared__ TC Di[kPB][kPB + 1];
+  __shared__ TC Xs[kPotrfWaves][16][kPB + 1];  // per-wave staging of one 16 x 32 block
   __shared__ int fail_s;
   // the diagonal tiles are the critical path of the factorization: issue
   // ahead of co-resident trailing-update waves
@@ -53,15 +54,16 @@ __global__ __launch_bounds__(kPotrfWaves * 64) void potrf_kernel(const PotrfOpT<
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: lets the row-block tests branch on scalars
   const int lr = lane & 15, lk = lane >> 4;
   const int n = __builtin_amdgcn_readfirstlane(op.n);
-  // working FP64 matrix: the tile itself, or for float tiles an FP64
-  // workspace that receives the factor columns as they are finished (the
-  // float tile is read once, at the initialisation of each block column, and
-  // written once, rounded, next to the FP64 copy)
+  // working matrix in the compute type: the tile itself, or for float
+  // tiles factored in FP64 an FP64 workspace that receives the factor columns
+  // as they are finished (the float tile is read once, at the
+  // initialisation of each block column, and written once, rounded, next to
+  // the FP64 copy)
   const long lda = __builtin_amdgcn_readfirstlane(op.lda);
-  double *__restrict__ W;
+  TC *__restrict__ W;
   long ldw;
-  if constexpr (sizeof(TS) == 8) {
-    W = reinterpret_cast<double *>(op.A);
+  if constexpr (std::is_same_v<TS, TC>) {
+    W = op.A;
     ldw = lda;
   } else {
     W = op.pw;
@@ -69,9 +71,9 @@ __global__ __launch_bounds__(kPotrfWaves * 64) void potrf_kernel(const PotrfOpT<
   }
   typedef __attribute__((address_space(1))) const TS gts;
   const gts *Ag = (const gts *)op.A;
-  auto store = [&](long r, long c, double v) {
+  auto store = [&](long r, long c, TC v) {
     gst(W)[r + c * ldw] = v;
-    if constexpr (sizeof(TS) != 8) gst(op.A)[r + c * lda] = (TS)v;
+    if constexpr (!std::is_same_v<TS, TC>) gst(op.A)[r + c * lda] = (TS)v;
   };
   int fail = 0;
   const int nblk = (n + kPB - 1) / kPB;
@@ -80,9 +82,9 @@ __global__ __launch_bounds__(kPotrfWaves * 64) void potrf_kernel(const PotrfOpT<
     const int c0 = jb * kPB, bs = min(kPB, n - c0);
     const int R = n - c0;           // rows of the block column (diagonal block included)
     const int nrb = (R + 15) / 16;  // 16-row blocks
-    d4_t acc[kRowBlocks][2];
-    typedef __attribute__((address_space(1))) const double gdouble;
-    const gdouble *Wg = (const gdouble *)W;  // tiles and workspace are global memory
+    acc_t acc[kRowBlocks][2];
+    typedef __attribute__((address_space(1))) const TC gtc;
+    const gtc *Wg = (const gtc *)W;  // tiles and workspace are global memory
     const int tact = (nrb - wave + kPotrfWaves - 1) / kPotrfWaves;  // active row blocks of this wave (uniform)
     // initialise with MINUS the current column block (the GEMM below adds
     // L(rows, k) L(c0+cols, k) and the sign is restored after it); loads from
@@ -96,20 +98,20 @@ __global__ __launch_bounds__(kPotrfWaves * 64) void potrf_kernel(const PotrfOpT<
           const int col = c0 + jj * 16 + lr;
 #pragma unroll
           for (int r = 0; r < 4; ++r)
-            acc[t][jj][r] = (double)Ag[min(c0 + q * 16 + Mfma<double>::drow(lk, r), n - 1) + (long)min(col, n - 1) * lda];
+            acc[t][jj][r] = (TC)Ag[min(c0 + q * 16 + Mfma<TC>::drow(lk, r), n - 1) + (long)min(col, n - 1) * lda];
         }
 #pragma unroll
         for (int jj = 0; jj < 2; ++jj) {
           const int col = c0 + jj * 16 + lr;
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int row = c0 + q * 16 + Mfma<double>::drow(lk, r);
-            acc[t][jj][r] = (row < n && col < n && row >= col) ? -acc[t][jj][r] : 0.0;
+            const int row = c0 + q * 16 + Mfma<TC>::drow(lk, r);
+            acc[t][jj][r] = (row < n && col < n && row >= col) ? -acc[t][jj][r] : TC(0);
           }
         }
       } else {
-        acc[t][0] = d4_t{0.0, 0.0, 0.0, 0.0};
-        acc[t][1] = d4_t{0.0, 0.0, 0.0, 0.0};
+        acc[t][0] = acc_t{TC(0), TC(0), TC(0), TC(0)};
+        acc[t][1] = acc_t{TC(0), TC(0), TC(0), TC(0)};
       }
     }
     // left-looking update with the columns [0, c0), streamed from L2 with the
@@ -129,9 +131,9 @@ __global__ __launch_bounds__(kPotrfWaves * 64) void potrf_kernel(const PotrfOpT<
         // operand slots in flight: more when few row blocks leave little
         // MFMA work per k-step to cover the load latency
         constexpr int SL = TA >= 2 ? 2 : 4;
-        double bv[SL][2], av[SL][TA];
+        TC bv[SL][2], av[SL][TA];
         auto load = [&](int sl, int k) {
-          const gdouble *Wk = Wg + (long)k * ldw;
+          const gtc *Wk = Wg + (long)k * ldw;
           bv[sl][0] = Wk[ob0];
           bv[sl][1] = Wk[ob1];
 #pragma unroll
@@ -140,8 +142,8 @@ __global__ __launch_bounds__(kPotrfWaves * 64) void potrf_kernel(const PotrfOpT<
         auto mma = [&](int sl) {
 #pragma unroll
           for (int t = 0; t < TA; ++t) {
-            acc[t][0] = Mfma<double>::mma(av[sl][t], bv[sl][0], acc[t][0]);
-            acc[t][1] = Mfma<double>::mma(av[sl][t], bv[sl][1], acc[t][1]);
+            acc[t][0] = Mfma<TC>::mma(av[sl][t], bv[sl][0], acc[t][0]);
+            acc[t][1] = Mfma<TC>::mma(av[sl][t], bv[sl][1], acc[t][1]);
           }
         };
 #pragma unroll
@@ -173,10 +175,10 @@ __global__ __launch_bounds__(kPotrfWaves * 64) void potrf_kernel(const PotrfOpT<
       for (int jj = 0; jj < 2; ++jj)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int rl = wave * 16 + Mfma<double>::drow(lk, r), cl = jj * 16 + lr;
-          double v = acc[0][jj][r];
-          if (rl >= bs || cl >= bs) v = (rl == cl) ? 1.0 : 0.0;  // identity padding
-          else if (rl < cl) v = 0.0;
+          const int rl = wave * 16 + Mfma<TC>::drow(lk, r), cl = jj * 16 + lr;
+          TC v = acc[0][jj][r];
+          if (rl >= bs || cl >= bs) v = (rl == cl) ? TC(1) : TC(0);  // identity padding
+          else if (rl < cl) v = TC(0);
           Ds[rl][cl] = v;
         }
     }
@@ -206,21 +208,21 @@ __global__ __launch_bounds__(kPotrfWaves * 64) void potrf_kernel(const PotrfOpT<
 #pragma unroll
         for (int jj = 0; jj < 2; ++jj)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) Xs[wave][Mfma<double>::drow(lk, r)][jj * 16 + lr] = acc[t][jj][r];
+          for (int r = 0; r < 4; ++r) Xs[wave][Mfma<TC>::drow(lk, r)][jj * 16 + lr] = acc[t][jj][r];
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        d4_t x0 = {0, 0, 0, 0}, x1 = {0, 0, 0, 0};
+        acc_t x0 = {TC(0), TC(0), TC(0), TC(0)}, x1 = {TC(0), TC(0), TC(0), TC(0)};
 #pragma unroll
         for (int ks = 0; ks < kPB / 4; ++ks) {
-          const double av = Xs[wave][lr][ks * 4 + lk];
-          x0 = Mfma<double>::mma(av, Di[lr][ks * 4 + lk], x0);
-          x1 = Mfma<double>::mma(av, Di[16 + lr][ks * 4 + lk], x1);
+          const TC av = Xs[wave][lr][ks * 4 + lk];
+          x0 = Mfma<TC>::mma(av, Di[lr][ks * 4 + lk], x0);
+          x1 = Mfma<TC>::mma(av, Di[16 + lr][ks * 4 + lk], x1);
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int row = c0 + q * 16 + Mfma<double>::drow(lk, r);
+          const int row = c0 + q * 16 + Mfma<TC>::drow(lk, r);
           if (row < n) {
             if (lr < bs) store(row, c0 + lr, x0[r]);
             if (16 + lr < bs) store(row, c0 + 16 + lr, x1[r]);
@@ -507,9 +509,15 @@ __global__ __launch_bounds__(256, 2) void gen_gemm_kernel(const UpdOp<T> *__rest
 // ---------------------------------------------------------------------------
 template <typename TS>
 hipError_t launch_potrf(hipStream_t s, const PotrfOpT<TS> *ops, int nops, const PotrfOpT<TS> &one,
-                        const int *d_abort) {
+                        const int *d_abort, bool compute_ts) {
   if (nops <= 0) return hipSuccess;
-  hipLaunchKernelGGL(potrf_kernel<TS>, dim3(nops), dim3(kPotrfWaves * 64), 0, s, ops, one, d_abort);
+  if constexpr (std::is_same_v<TS, float>) {
+    if (compute_ts) {
+      hipLaunchKernelGGL((potrf_kernel<float, float>), dim3(nops), dim3(kPotrfWaves * 64), 0, s, ops, one, d_abort);
+      return hipGetLastError();
+    }
+  }
+  hipLaunchKernelGGL((potrf_kernel<TS, double>), dim3(nops), dim3(kPotrfWaves * 64), 0, s, ops, one, d_abort);
   return hipGetLastError();
 }
 
@@ -521,9 +529,9 @@ hipError_t launch_dinv(hipStream_t s, const PotrfOpT<TS> *ops, int nops, const P
 }
 
 template hipError_t launch_potrf<double>(hipStream_t, const PotrfOpT<double> *, int, const PotrfOpT<double> &,
-                                         const int *);
+                                         const int *, bool);
 template hipError_t launch_potrf<float>(hipStream_t, const PotrfOpT<float> *, int, const PotrfOpT<float> &,
-                                        const int *);
+                                        const int *, bool);
 template hipError_t launch_dinv<double>(hipStream_t, const PotrfOpT<double> *, int, const PotrfOpT<double> &);
 template hipError_t launch_dinv<float>(hipStream_t, const PotrfOpT<float> *, int, const PotrfOpT<float> &);
 

@@ -46,9 +46,12 @@ struct UpdOp {
 // argument) is used and nops must be 1.  `d_abort` (optional device int):
 // when nonzero at kernel start, the kernel does nothing (a failed POTRF
 // stops the rest of the factorization without a host round trip).
+// POTRF computes in FP64 (float tiles through the FP64 workspace pw), or,
+// for float tiles with compute_ts, in FP32 on the tile itself (pw unused):
+// for factorizations whose trailing updates carry larger errors anyway.
 template <typename TS>
 hipError_t launch_potrf(hipStream_t s, const PotrfOpT<TS> *ops, int nops, const PotrfOpT<TS> &one,
-                        const int *d_abort);
+                        const int *d_abort, bool compute_ts = false);
 // inverses of the 32 x 32 diagonal blocks of an already factored tile
 template <typename TS>
 hipError_t launch_dinv(hipStream_t s, const PotrfOpT<TS> *ops, int nops, const PotrfOpT<TS> &one);

@@ -47,6 +47,11 @@ struct TiledPlanT {
   T *tiles = nullptr;
   unsigned flags = 0;
   int lowp = 0;  // float tiles only: 0 FP32 updates, else kLowF16x2 / kLowF16 / kLowBF16
+  // The diagonal tiles are factored in FP64, except under the one-product
+  // FP16 / BF16 updates, whose operand rounding (2^-11 / 2^-8 of each row's
+  // largest entry) dwarfs an FP32 factorization's: there POTRF runs in FP32
+  // on the tile itself (FP32 MFMA, half the bytes of its left-looking reads).
+  bool potrf_in_ts() const { return lowp == kLowF16 || lowp == kLowBF16; }
   DevBuf d_ops, d_dinv, d_info, d_pw;
   DevBuf d_panel, d_scale, d_f16ops;  // split-FP16 panels (two slots, 2G when grouped), scales, descriptors
   std::vector<int> split_off, split_cnt;
@@ -303,7 +308,7 @@ struct TiledPlanT {
     };
     auto potrf = [&](int k) -> int {
       const int slot = prof ? prof_begin(sd) : -1;
-      TTGC_HIP(launch_potrf<T>(sd, potrf_ops() + k, 1, pnone, info));
+      TTGC_HIP(launch_potrf<T>(sd, potrf_ops() + k, 1, pnone, info, potrf_in_ts()));
       if (prof) prof_end(slot, sd, PROF_POTRF, b3 / 3.0);
       return dep(ev_diag, sd, s_look);
     };
@@ -385,7 +390,7 @@ struct TiledPlanT {
     };
     auto potrf = [&](int k) -> int {
       const int slot = prof ? prof_begin(sd) : -1;
-      TTGC_HIP(launch_potrf<T>(sd, potrf_ops() + k, 1, pnone, info));
+      TTGC_HIP(launch_potrf<T>(sd, potrf_ops() + k, 1, pnone, info, potrf_in_ts()));
       if (prof) prof_end(slot, sd, PROF_POTRF, b3 / 3.0);
       if (rsv) {  // TRSM(k) follows on the panel stream
         TTGC_HIP(hipEventRecord(ev_diag, s_diag));
