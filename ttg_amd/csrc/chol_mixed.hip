@@ -41,10 +41,10 @@ namespace ttgc {
 // t + 256 (NB <= 512); each wave reduces its rows of a column by shuffles in
 // a fixed order, and the four wave sums are added in wave order.
 template <typename T>
-__global__ __launch_bounds__(256) void convert_norm_kernel(int N, int NB, int nt, const double *__restrict__ A,
+__global__ __launch_bounds__(256) void convert_norm_kernel(int N, int NB, int nt, long t0, const double *__restrict__ A,
                                                            T *__restrict__ W, double *__restrict__ S,
                                                            double dscale) {
-  const long t = blockIdx.x;
+  const long t = t0 + blockIdx.x;
   int m, n;
   {
     int lo = 0, hi = nt - 1;
@@ -308,24 +308,56 @@ struct MixedPlan : MixedPlanBase {
     return TTG_CHOL_OK;
   }
 
+  // Working copy of A (diagonal scaled by dscale) with the tiles' row and
+  // column sums of |A|, -||A||_inf into st()[0], and the factorization of
+  // the copy.  The tiles of the columns the first panel chain works on are
+  // converted first, on s; the others, and the norm, go to the plan's bulk
+  // stream, where they run beside that chain.
+  int convert_and_factor(hipStream_t s, const double *A, double dscale, int *d_info) {
+    const int nt = (N + NB - 1) / NB;
+    const long ntiles = (long)nt * (nt + 1) / 2;
+    const int c = fplan->first_cols();
+    const long t1 = (long)c * nt - (long)c * (c - 1) / 2;  // tiles of columns < c
+    double *parts = static_cast<double *>(norm_parts.ptr), *rw = static_cast<double *>(rows.ptr);
+    T *wk = w();
+    double *stat = st();
+    const int n = N, nb = NB;
+    auto convert = [=](hipStream_t q, long a, long b) -> int {
+      if (b <= a) return TTG_CHOL_OK;
+      hipLaunchKernelGGL(convert_norm_kernel<T>, dim3((unsigned)(b - a)), dim3(256), 0, q, n, nb, nt, a, A, wk,
+                         parts, dscale);
+      TTGC_HIP(hipGetLastError());
+      return TTG_CHOL_OK;
+    };
+    auto norm = [=](hipStream_t q) -> int {
+      hipLaunchKernelGGL(norm_rows_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, q, n, nb, nt, parts, rw);
+      TTGC_HIP(hipGetLastError());
+      hipLaunchKernelGGL(max_neg_kernel, dim3(1), dim3(1024), 0, q, n, rw, stat);
+      TTGC_HIP(hipGetLastError());
+      return TTG_CHOL_OK;
+    };
+    int rc = convert(s, 0, t1);
+    if (rc) return rc;
+    if (t1 < ntiles) {
+      fplan->pre_bulk = [=](hipStream_t q) -> int {
+        const int r = convert(q, t1, ntiles);
+        return r ? r : norm(q);
+      };
+    } else {
+      rc = norm(s);
+      if (rc) return rc;
+    }
+    rc = fplan->run(s, d_info);
+    fplan->pre_bulk = nullptr;
+    return rc;
+  }
+
   // steps 1-3: working copy (diagonal scaled by dscale), ||A||_inf, factor,
   // X = M^{-1} B.  *failed: the factorization met a non-positive pivot.
   int factor_and_solve(hipStream_t s, const double *A, const double *B, long ldb, double *X, long ldx, int *d_info,
                        double dscale, bool *failed) {
-    const int nt = (N + NB - 1) / NB;
     *failed = false;
-    {
-      const long ntiles = (long)nt * (nt + 1) / 2;
-      double *parts = static_cast<double *>(norm_parts.ptr), *rw = static_cast<double *>(rows.ptr);
-      hipLaunchKernelGGL(convert_norm_kernel<T>, dim3((unsigned)ntiles), dim3(256), 0, s, N, NB, nt, A, w(), parts,
-                         dscale);
-      TTGC_HIP(hipGetLastError());
-      hipLaunchKernelGGL(norm_rows_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, N, NB, nt, parts, rw);
-      TTGC_HIP(hipGetLastError());
-      hipLaunchKernelGGL(max_neg_kernel, dim3(1), dim3(1024), 0, s, N, rw, st());
-      TTGC_HIP(hipGetLastError());
-    }
-    int rc = fplan->run(s, d_info);
+    int rc = convert_and_factor(s, A, dscale, d_info);
     if (rc) return rc;
     int hinfo = 0;
     TTGC_HIP(hipMemcpyAsync(&hinfo, d_info, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -409,22 +441,9 @@ struct MixedPlan : MixedPlanBase {
 
   int run(hipStream_t s, const double *A, const double *B, long ldb, double *X, long ldx, double tol, int max_iter,
           int *d_info, int *iters, double *berr) {
-    const int nt = (N + NB - 1) / NB;
-    // 1. working copy (padding is set later by the factorization's prepare
-    //    step) and ||A||_inf in one pass over A
-    {
-      const long ntiles = (long)nt * (nt + 1) / 2;
-      double *parts = static_cast<double *>(norm_parts.ptr), *rw = static_cast<double *>(rows.ptr);
-      hipLaunchKernelGGL(convert_norm_kernel<T>, dim3((unsigned)ntiles), dim3(256), 0, s, N, NB, nt, A, w(), parts,
-                         1.0);
-      TTGC_HIP(hipGetLastError());
-      hipLaunchKernelGGL(norm_rows_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, N, NB, nt, parts, rw);
-      TTGC_HIP(hipGetLastError());
-      hipLaunchKernelGGL(max_neg_kernel, dim3(1), dim3(1024), 0, s, N, rw, st());
-      TTGC_HIP(hipGetLastError());
-    }
-    // 2. factor
-    int rc = fplan->run(s, d_info);
+    // 1-2. working copy (padding is set later by the factorization's prepare
+    //      step), ||A||_inf in the same pass over A, factor
+    int rc = convert_and_factor(s, A, 1.0, d_info);
     if (rc) return rc;
     int hinfo = 0;
     TTGC_HIP(hipMemcpyAsync(&hinfo, d_info, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -1,7 +1,8 @@
 // Plan of the tiled factorization (the examples/potrf task graph with the
 // launches grouped per panel step), templated on the storage type T of the
-// factor tiles: double (plain FP64) or float (mixed precision: FP64 POTRF
-// arithmetic on the diagonal tiles, FP32 TRSM and trailing updates).
+// factor tiles: double (plain FP64) or float (mixed precision: FP32 TRSM and
+// trailing updates, or FP16 / BF16 updates; the diagonal tiles in FP64, or in
+// FP32 under the one-product FP16 / BF16 updates).
 //
 // Step k of the graph is POTRF(k), TRSM(m,k) for m > k, and the updates
 // SYRK(k,m) / GEMM(m,n,k) for k < n <= m.  Every TRSM of a step is one
@@ -28,6 +29,7 @@
 
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <vector>
 
 #include "chol_f16.hpp"
@@ -61,6 +63,16 @@ struct TiledPlanT {
   // columns (bulkA) and of all further columns (bulkB)
   bool paired = false;
   int G = 2;
+  // Optional work the caller's first step leaves to the plan: enqueued on the
+  // bulk stream before anything there touches the tiles of columns >=
+  // first_cols() (the grouped schedule: after the panel chain of the first
+  // group has been released, so that the two overlap; otherwise at the
+  // start).  Set by the caller for one run.
+  std::function<int(hipStream_t)> pre_bulk;
+  // columns the caller must have ready before run(): the first group's in
+  // the grouped schedule with full tiles, else all of them
+  // (the graph-replayed schedule, used without reserved CUs, takes all)
+  int first_cols() const { return (paired && N % NB == 0 && s_diag) ? std::min(G, NT) : NT; }
   int TG = 2;  // side of the square tile groups of the bulk updates (TTG_CHOL_TILE_GROUP)
   std::vector<int> pe_off, pe_cnt, pa_off, pa_cnt, pb_off, pb_cnt;  // eager per step; bulkA, bulkB per group
   hipEvent_t ev_bulk = nullptr;
@@ -294,7 +306,7 @@ struct TiledPlanT {
   // bulkB(g-1) overlaps the whole panel chain of group g; it writes columns
   // >= k0 + G and reads the panels of group g-1, which the chain of group g
   // neither writes nor splits into.
-  int enqueue_paired(int *info, bool prof) {
+  int enqueue_paired(int *info, bool prof, bool hook) {
     const double b3 = (double)NB * NB * NB;
     PotrfOpT<T> pnone{};
     TrsmOp<T> tnone{};
@@ -330,6 +342,8 @@ struct TiledPlanT {
       return TTG_CHOL_OK;
     };
     if (int rc = dep(ev_col, s_main, sd)) return rc;
+    if (hook && pre_bulk)
+      if (int rc = pre_bulk(s_main)) return rc;
     for (int g = 0, k0 = 0; k0 < NT; ++g, k0 += G) {
       const int ge = std::min(k0 + G, NT);
       if (g > 0) TTGC_HIP(hipStreamWaitEvent(sd, ev_bulk, 0));  // the group's columns complete (bulkA(g-1))
@@ -366,7 +380,10 @@ struct TiledPlanT {
   // Enqueue the whole factorization on s_main (and s_look), writing *info.
   // With `prof`, every launch is bracketed by timing events.
   int enqueue(int *info, bool prof) {
-    if (paired) return enqueue_paired(info, prof);
+    const bool late = paired && N % NB == 0;  // the grouped schedule runs the hook after the chain's release
+    if (pre_bulk && !late)
+      if (int rc = pre_bulk(s_main)) return rc;
+    if (paired) return enqueue_paired(info, prof, late);
     const bool fast = NB % 128 == 0;
     const T m1 = T(-1), p1 = T(1);
     const double b3 = (double)NB * NB * NB;
@@ -448,7 +465,16 @@ struct TiledPlanT {
     if (rc) return rc;
     const bool prof = (flags & TTG_CHOL_F_PROFILE) != 0 || prof_enabled();
     if ((flags & TTG_CHOL_F_GRAPH) && !prof && !s_diag) {
+      if (pre_bulk) {  // never captured: it runs before the replay, on the caller's stream
+        std::function<int(hipStream_t)> hook;
+        hook.swap(pre_bulk);
+        rc = hook(user);
+        hook.swap(pre_bulk);
+        if (rc) return rc;
+      }
       if (!gexec || bound_info != info) {
+        std::function<int(hipStream_t)> hook;  // empty while the schedule is captured
+        hook.swap(pre_bulk);
         if (gexec) {
           TTGC_HIP(hipGraphExecDestroy(gexec));
           gexec = nullptr;
@@ -461,6 +487,7 @@ struct TiledPlanT {
         rc = enqueue(info, false);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(s_main, &g);
+        hook.swap(pre_bulk);
         if (rc != TTG_CHOL_OK) {
           if (g) (void)hipGraphDestroy(g);
           return rc;
