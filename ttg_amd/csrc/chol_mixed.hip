@@ -106,17 +106,24 @@ __global__ void norm_rows_kernel(int N, int NB, int nt, const double *__restrict
   rows[i] = -s;
 }
 
-// Per right-hand side j: berr_j = max|R_j| / (anorm * max|X_j| + max|B_j|).
-__global__ __launch_bounds__(256) void berr_kernel(int N, const double *__restrict__ R, long ldr,
-                                                   const double *__restrict__ X, long ldx,
-                                                   const double *__restrict__ B, long ldb,
-                                                   const double *__restrict__ anorm, double *__restrict__ out) {
-  const int j = blockIdx.x;
+// Per-column reductions over N rows, spread over kRedParts workgroups per
+// column (rows in kRedParts contiguous chunks) and combined per column in
+// chunk order by a second launch: fixed summation order, and the whole GPU
+// on a pass that one workgroup per column left latency-bound.
+constexpr int kRedParts = 32;
+
+// max |R_j|, max |X_j|, max |B_j| of chunk g of column j (NaN in R propagates)
+__global__ __launch_bounds__(256) void berr_part_kernel(int N, const double *__restrict__ R, long ldr,
+                                                        const double *__restrict__ X, long ldx,
+                                                        const double *__restrict__ B, long ldb,
+                                                        double *__restrict__ part) {
+  const int g = blockIdx.x, j = blockIdx.y;
+  const int chunk = (N + kRedParts - 1) / kRedParts, i0 = g * chunk, i1 = min(N, i0 + chunk);
   __shared__ double sr[256], sx[256], sb[256];
   double r = 0.0, x = 0.0, b = 0.0;
-  for (int i = threadIdx.x; i < N; i += 256) {
+  for (int i = i0 + threadIdx.x; i < i1; i += 256) {
     const double rv = fabs(R[i + (long)j * ldr]);
-    r = (rv > r || rv != rv) ? rv : r;  // NaN propagates
+    r = (rv > r || rv != rv) ? rv : r;
     x = fmax(x, fabs(X[i + (long)j * ldx]));
     b = fmax(b, fabs(B[i + (long)j * ldb]));
   }
@@ -133,7 +140,27 @@ __global__ __launch_bounds__(256) void berr_kernel(int N, const double *__restri
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[j] = sr[0] / (anorm[0] * sx[0] + sb[0]);
+  if (threadIdx.x == 0) {
+    double *o = part + 3L * ((long)j * kRedParts + g);
+    o[0] = sr[0];
+    o[1] = sx[0];
+    o[2] = sb[0];
+  }
+}
+
+// Per right-hand side j: berr_j = max|R_j| / (anorm * max|X_j| + max|B_j|).
+__global__ void berr_final_kernel(int nrhs, const double *__restrict__ part, const double *__restrict__ anorm,
+                                  double *__restrict__ out) {
+  for (int j = threadIdx.x; j < nrhs; j += blockDim.x) {
+    double r = 0.0, x = 0.0, b = 0.0;
+    for (int g = 0; g < kRedParts; ++g) {
+      const double *o = part + 3L * ((long)j * kRedParts + g);
+      r = (o[0] > r || o[0] != o[0]) ? o[0] : r;
+      x = fmax(x, o[1]);
+      b = fmax(b, o[2]);
+    }
+    out[j] = r / (anorm[0] * x + b);
+  }
 }
 
 // anorm = max_i (-rowsum_i) where rowsum = 0 - |A| 1
@@ -163,22 +190,31 @@ __global__ void from_rows_acc_kernel(int N, int nrhs, const TS *__restrict__ Bt,
     X[i + (long)j * ldx] = v;
 }
 
-// out[j] = sum_i X(i, j) Y(i, j), one workgroup per column, reduced in a
-// fixed order (the same sums on every run)
-__global__ __launch_bounds__(256) void dot_cols_kernel(int N, const double *__restrict__ X, long ldx,
+// sum over chunk g of column j of X(i, j) Y(i, j), reduced in a fixed order
+__global__ __launch_bounds__(256) void dot_part_kernel(int N, const double *__restrict__ X, long ldx,
                                                        const double *__restrict__ Y, long ldy,
-                                                       double *__restrict__ out) {
-  const int j = blockIdx.x;
-  __shared__ double s[256];
+                                                       double *__restrict__ part) {
+  const int g = blockIdx.x, j = blockIdx.y;
+  const int chunk = (N + kRedParts - 1) / kRedParts, i0 = g * chunk, i1 = min(N, i0 + chunk);
+  __shared__ double sm[256];
   double a = 0.0;
-  for (int i = threadIdx.x; i < N; i += 256) a += X[i + (long)j * ldx] * Y[i + (long)j * ldy];
-  s[threadIdx.x] = a;
+  for (int i = i0 + threadIdx.x; i < i1; i += 256) a += X[i + (long)j * ldx] * Y[i + (long)j * ldy];
+  sm[threadIdx.x] = a;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[j] = s[0];
+  if (threadIdx.x == 0) part[(long)j * kRedParts + g] = sm[0];
+}
+
+// out[j] = sum of column j's chunk sums, in chunk order
+__global__ void dot_final_kernel(int nrhs, const double *__restrict__ part, double *__restrict__ out) {
+  for (int j = threadIdx.x; j < nrhs; j += blockDim.x) {
+    double a = 0.0;
+    for (int g = 0; g < kRedParts; ++g) a += part[(long)j * kRedParts + g];
+    out[j] = a;
+  }
 }
 
 // CG step of every column j, with Q = -A P:  a_j = rz_j / (p_j . A p_j),
@@ -215,7 +251,7 @@ struct MixedPlan : MixedPlanBase {
   int lowp = 0;  // 0: FP32 updates, else kLowF16x2 / kLowF16 / kLowBF16
   DevBuf W;      // working factor tiles (T)
   DevBuf R;      // residual, N x nrhs (FP64)
-  DevBuf stats;  // anorm, berr[nrhs]
+  DevBuf stats;  // anorm, berr[nrhs], then the per-chunk partials of the column reductions
   DevBuf rows, norm_parts;  // -rowsum of |A|; per-tile |A| partial sums
   DevBuf cg, zeros;         // CG: P, Q, Z (N x nrhs each) and per-column scalars; a zero N x nrhs
   std::unique_ptr<TiledPlanT<T>> fplan;
@@ -230,13 +266,14 @@ struct MixedPlan : MixedPlanBase {
   T *w() { return static_cast<T *>(W.ptr); }
   double *r() { return static_cast<double *>(R.ptr); }
   double *st() { return static_cast<double *>(stats.ptr); }
+  double *red_part() { return st() + 1 + nrhs; }
 
   int build() {
     const int nt = (N + NB - 1) / NB;
     const size_t ntile = (size_t)nt * (nt + 1) / 2 * NB * NB;
     TTGC_HIP(W.reserve(sizeof(T) * ntile));
     TTGC_HIP(R.reserve(sizeof(double) * (size_t)N * nrhs));
-    TTGC_HIP(stats.reserve(sizeof(double) * (1 + nrhs)));
+    TTGC_HIP(stats.reserve(sizeof(double) * (1 + nrhs + 3 * (size_t)kRedParts * nrhs)));
     TTGC_HIP(rows.reserve(sizeof(double) * (size_t)N));
     TTGC_HIP(norm_parts.reserve(sizeof(double) * (size_t)nt * (nt + 1) / 2 * 2 * NB));
     fplan.reset(new TiledPlanT<T>);
@@ -293,14 +330,33 @@ struct MixedPlan : MixedPlanBase {
     return TTG_CHOL_OK;
   }
 
+  // st()[1 + j] = backward error of column j (N rows; R, X and B column-major)
+  int berr_launch(hipStream_t s, const double *R, long ldr, const double *X, long ldx, const double *B, long ldb) {
+    double *part = red_part();
+    hipLaunchKernelGGL(berr_part_kernel, dim3(kRedParts, nrhs), dim3(256), 0, s, N, R, ldr, X, ldx, B, ldb, part);
+    TTGC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(berr_final_kernel, dim3(1), dim3(256), 0, s, nrhs, part, st(), st() + 1);
+    TTGC_HIP(hipGetLastError());
+    return TTG_CHOL_OK;
+  }
+  // out[j] = sum_i X(i, j) Y(i, j) over the N rows (fixed order)
+  int dot_cols(hipStream_t s, const double *X, long ldx, const double *Y, long ldy, double *out) {
+    double *part = red_part();
+    hipLaunchKernelGGL(dot_part_kernel, dim3(kRedParts, nrhs), dim3(256), 0, s, N, X, ldx, Y, ldy, part);
+    TTGC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(256), 0, s, nrhs, part, out);
+    TTGC_HIP(hipGetLastError());
+    return TTG_CHOL_OK;
+  }
+
   // backward error of every column from the residual in `res` (FP64), the
   // largest over the columns into *e (NaN propagates); host-synchronising
   int column_berr(hipStream_t s, const double *res, const double *B, long ldb, const double *X, long ldx,
                   double *e) {
-    hipLaunchKernelGGL(berr_kernel, dim3(nrhs), dim3(256), 0, s, N, res, (long)N, X, ldx, B, ldb, st(), st() + 1);
-    TTGC_HIP(hipGetLastError());
+    int rc = berr_launch(s, res, (long)N, X, ldx, B, ldb);
+    if (rc) return rc;
     TTGC_HIP(hipMemcpyAsync(hstats.data(), st(), sizeof(double) * (1 + nrhs), hipMemcpyDeviceToHost, s));
-    int rc = sync_checked(s);
+    rc = sync_checked(s);
     if (rc) return rc;
     double m = 0.0;
     for (int j = 0; j < nrhs; ++j) m = (hstats[1 + j] > m || std::isnan(hstats[1 + j])) ? hstats[1 + j] : m;
@@ -406,16 +462,16 @@ struct MixedPlan : MixedPlanBase {
         if (!(e > tol) || it >= max_iter) break;
         rc = solve(s, r(), N, Z, N, false);
         if (rc) return rc;
-        hipLaunchKernelGGL(dot_cols_kernel, dim3(nrhs), dim3(256), 0, s, N, r(), (long)N, Z, (long)N, rz);
-        TTGC_HIP(hipGetLastError());
+        rc = dot_cols(s, r(), (long)N, Z, (long)N, rz);
+        if (rc) return rc;
         TTGC_HIP(hipMemcpyAsync(P, Z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         restart = false;
       }
       // Q = -A P
       rc = residual_fp64(s, N, NB, A, nrhs, P, N, zero, N, Q, N);
       if (rc) return rc;
-      hipLaunchKernelGGL(dot_cols_kernel, dim3(nrhs), dim3(256), 0, s, N, P, (long)N, Q, (long)N, pq);
-      TTGC_HIP(hipGetLastError());
+      rc = dot_cols(s, P, (long)N, Q, (long)N, pq);
+      if (rc) return rc;
       hipLaunchKernelGGL(cg_step_kernel, dim3(gv), dim3(256), 0, s, N, nrhs, rz, pq, P, X, ldx, Q, r());
       TTGC_HIP(hipGetLastError());
       ++it;
@@ -428,8 +484,8 @@ struct MixedPlan : MixedPlanBase {
       }
       rc = solve(s, r(), N, Z, N, false);
       if (rc) return rc;
-      hipLaunchKernelGGL(dot_cols_kernel, dim3(nrhs), dim3(256), 0, s, N, r(), (long)N, Z, (long)N, rzn);
-      TTGC_HIP(hipGetLastError());
+      rc = dot_cols(s, r(), (long)N, Z, (long)N, rzn);
+      if (rc) return rc;
       hipLaunchKernelGGL(cg_dir_kernel, dim3(gv), dim3(256), 0, s, N, nrhs, rzn, rz, Z, P);
       TTGC_HIP(hipGetLastError());
       std::swap(rz, rzn);
@@ -466,8 +522,8 @@ struct MixedPlan : MixedPlanBase {
     for (;; ++it) {
       rc = residual_fp64(s, N, NB, A, nrhs, X, ldx, B, ldb, r(), N);
       if (rc) return rc;
-      hipLaunchKernelGGL(berr_kernel, dim3(nrhs), dim3(256), 0, s, N, r(), (long)N, X, ldx, B, ldb, st(), st() + 1);
-      TTGC_HIP(hipGetLastError());
+      rc = berr_launch(s, r(), (long)N, X, ldx, B, ldb);
+      if (rc) return rc;
       TTGC_HIP(hipMemcpyAsync(hstats.data(), st(), sizeof(double) * (1 + nrhs), hipMemcpyDeviceToHost, s));
       rc = sync_checked(s);
       if (rc) return rc;
