@@ -126,7 +126,9 @@ __global__ __launch_bounds__(256, 2) void update_f16x2_kernel(const F16UpdOp *__
   const int i0 = bi * BM, j0 = bj * BN;
   if (i0 >= op.n || j0 >= op.n) return;
   __shared__ __align__(16) h8_t lds[2][SPS * ST_CH];  // [stage][slab][A parts | B parts][row][chunk]
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ __align__(16) float scl[2][BM + BN];       // [panel & 1][row factors | column factors]
+  // the wave index as a scalar, so that everything derived from it stays in SGPRs
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave >> 1, wn = wave & 1;
   const int lr = lane & 15, lk = lane >> 4;
   // npan products (panels), each op.k deep: slab s of the run is slab
@@ -134,27 +136,38 @@ __global__ __launch_bounds__(256, 2) void update_f16x2_kernel(const F16UpdOp *__
   const int nk = op.k / kF16BK, npan = op.npan > 1 ? op.npan : 1;
   const int ns = npan * nk;  // slabs; a multiple of SPS (the launcher checks k)
 
-  // slab s into stage st: wave w loads segments w, w + 4, ...; lane l of a
-  // segment fills physical chunk l & 3 of the segment's row l >> 2 with the
-  // logical chunk the swizzle puts there (a permutation within each 64-byte
-  // row, so the wave still reads whole 1 KB blocks of the k-slab-major panel)
+  // slab s into stage st: per part, wave w loads the 1 KB (16-row) segments
+  // w and w + 4 of A and of B; lane l fills physical chunk l & 3 of the
+  // segment's row l >> 2 with the logical chunk the swizzle puts there (a
+  // permutation within each 64-byte row, so a wave still reads whole 1 KB
+  // blocks of the k-slab-major panel).  The lane offsets are fixed for the
+  // block; the slab's base is scalar.
+  static_assert(L == 4 * NP, "two A and two B segments per wave and part");
+  const int row0 = wave * 16 + (lane >> 2);  // segment w + 4 adds 64 rows (4 KB), same swizzle
+  const unsigned chk = (unsigned)((lane & 3) ^ ((row0 >> 1) & 3));
+  const unsigned voffA = ((unsigned)(i0 + row0) * 4 + chk) * 16, voffB = ((unsigned)(j0 + row0) * 4 + chk) * 16;
   auto issue = [&](int s, int st, int u) {
     const int q = s / nk, ks = s - q * nk;
-    const long po = q * op.pstride;
-    const int rseg = lane >> 2, p = lane & 3;
+    const long eo = q * op.pstride + (long)ks * op.ld * kF16BK;  // halves
+    const h8_t *l0 = &lds[st][u * ST_CH];
 #pragma unroll
-    for (int i = 0; i < L; ++i) {
-      const int g = wave + 4 * i;
-      const bool isa = g < NP * BM / 16;
-      const int h = isa ? g : g - NP * BM / 16;
-      const int part = h / (BM / 16), sg = h % (BM / 16);
-      const int row = sg * 16 + rseg;
-      const uint16_t *base = isa ? (part ? op.alo : op.ahi) : (part ? op.blo : op.bhi);
-      const h8_t *src = reinterpret_cast<const h8_t *>(base + po) + ((long)ks * op.ld + (isa ? i0 : j0) + row) * 4 +
-                        (p ^ ((row >> 1) & 3));
-      const int off = u * ST_CH + (isa ? 0 : A_CH) + part * BM * 4 + sg * 64;
-      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
-                                       (__attribute__((address_space(3))) void *)&lds[st][off], 16, 0, 0);
+    for (int part = 0; part < NP; ++part) {
+      const char *ba = reinterpret_cast<const char *>((part ? op.alo : op.ahi) + eo);
+      const char *bb = reinterpret_cast<const char *>((part ? op.blo : op.bhi) + eo);
+      lds_dma_dwordx4_s(ba, voffA, l0 + part * BM * 4 + wave * 64);
+      lds_dma_dwordx4_s(ba + 4096, voffA, l0 + part * BM * 4 + (wave + 4) * 64);
+      lds_dma_dwordx4_s(bb, voffB, l0 + A_CH + part * BN * 4 + wave * 64);
+      lds_dma_dwordx4_s(bb + 4096, voffB, l0 + A_CH + part * BN * 4 + (wave + 4) * 64);
+    }
+  };
+  // panel q's row and column factors into slot q & 1, by wave 0 (lanes 0-31
+  // the BM row factors, lanes 32-63 the BN column factors), one panel ahead
+  // of the fold that reads them
+  auto issue_scl = [&](int q) {
+    if (wave == 0) {
+      const long so = q * op.sstride;
+      const float *src = lane < 32 ? op.ra + so + i0 + 4 * lane : op.rb + so + j0 + 4 * (lane - 32);
+      lds_dma_dwordx4(src, &scl[q & 1][0]);
     }
   };
 
@@ -163,9 +176,10 @@ __global__ __launch_bounds__(256, 2) void update_f16x2_kernel(const F16UpdOp *__
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
-  // the first slab's loads go out first, then this lane's C values (4
-  // consecutive rows of one column per (i, j)), whose latency then hides
-  // behind the first slabs instead of delaying them
+  // the first stage's loads (and panel 0's factors) go out first, then this
+  // lane's C values (4 consecutive rows of one column per (i, j)), whose
+  // latency then hides behind the first slabs instead of delaying them
+  issue_scl(0);
 #pragma unroll
   for (int u = 0; u < SPS; ++u) issue(u, 0, u);
   f4_t cv[4][4];
@@ -210,14 +224,12 @@ __global__ __launch_bounds__(256, 2) void update_f16x2_kernel(const F16UpdOp *__
   // cv -= acc * (row factor x column factor) of panel q: the products with
   // powers of two are exact, so this is the one rounding of an FP32 update
   auto fold = [&](int q) {
-    const long so = q * op.sstride;
     f4_t rrow[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-      rrow[i] = *gld(reinterpret_cast<const f4_t *>(op.ra + so + i0 + wm * 64 + i * 16 + 4 * lk));
+    for (int i = 0; i < 4; ++i) rrow[i] = *reinterpret_cast<const f4_t *>(&scl[q & 1][wm * 64 + i * 16 + 4 * lk]);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const float rc = gld(op.rb + so)[j0 + wn * 64 + j * 16 + lr];
+      const float rc = scl[q & 1][BM + wn * 64 + j * 16 + lr];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         cv[i][j] = cv[i][j] - acc[i][j] * (rrow[i] * rc);
@@ -228,13 +240,15 @@ __global__ __launch_bounds__(256, 2) void update_f16x2_kernel(const F16UpdOp *__
   for (int t = 0; t < ns / SPS; ++t) {
     // this wave's loads of stage t are done once at most the C loads issued
     // after them (stage 0) are outstanding; the barrier makes every wave's
-    // visible and retires all reads of stage (t + 1) & 1 (stage t - 1)
+    // visible and retires all reads of stage (t + 1) & 1 (stage t - 1) and,
+    // a panel after its fold, of factor slot q & 1
     if (t == 0)
       wait_vmcnt<NCV>();
     else
       wait_vmcnt<0>();
     raw_barrier();
     if ((t + 1) * SPS < ns) {
+      if (((t + 1) * SPS) % nk == 0) issue_scl((t + 1) * SPS / nk);
 #pragma unroll
       for (int u = 0; u < SPS; ++u) issue((t + 1) * SPS + u, (t + 1) & 1, u);
     }
