@@ -263,13 +263,21 @@ __device__ __forceinline__ void lds_dma_dwordx4(const void *g, const void *l) {
 // scalar, where per-lane 64-bit products would each cost several VALU
 // cycles.  The base is made uniform by readfirstlane (which returns int: the
 // halves are widened as unsigned, so a low word >= 2^31 is not sign-extended).
-__device__ __forceinline__ void lds_dma_dwordx4_s(const void *sbase, unsigned voff, const void *l) {
-  const unsigned la = (unsigned)(uintptr_t)(const __attribute__((address_space(3))) void *)l;
-  const unsigned long long b = (unsigned long long)(uintptr_t)sbase;
+__device__ __forceinline__ unsigned long long uniform_addr(const void *p) {
+  const unsigned long long b = (unsigned long long)(uintptr_t)p;
   const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
   const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
-  const unsigned long long bu = ((unsigned long long)hi << 32) | (unsigned long long)lo;
-  asm volatile("s_nop 0\n\tglobal_load_lds_dwordx4 %0, %1" ::"v"(voff), "s"(bu), "{m0}"(la) : "memory");
+  return ((unsigned long long)hi << 32) | (unsigned long long)lo;
+}
+__device__ __forceinline__ void lds_dma_dwordx4_s(const void *sbase, unsigned voff, const void *l) {
+  const unsigned la = (unsigned)(uintptr_t)(const __attribute__((address_space(3))) void *)l;
+  asm volatile("s_nop 0\n\tglobal_load_lds_dwordx4 %0, %1" ::"v"(voff), "s"(uniform_addr(sbase)), "{m0}"(la)
+               : "memory");
+}
+__device__ __forceinline__ void lds_dma_dword_s(const void *sbase, unsigned voff, const void *l) {
+  const unsigned la = (unsigned)(uintptr_t)(const __attribute__((address_space(3))) void *)l;
+  asm volatile("s_nop 0\n\tglobal_load_lds_dword %0, %1" ::"v"(voff), "s"(uniform_addr(sbase)), "{m0}"(la)
+               : "memory");
 }
 
 // Make this workgroup's global stores visible to its own later global loads

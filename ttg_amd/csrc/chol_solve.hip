@@ -640,16 +640,15 @@ __global__ __launch_bounds__(512) void residual_strip_kernel(int nt, int nfull, 
   const int lr = lane & 15, lk = lane >> 4;
   // block j into stage st: wave w loads rows 32 w .. 32 w + 31 of the 16
   // columns (one 4-byte word per lane, 256 bytes per load) and rows 16 s +
-  // 2 w, + 1 of X_n
+  // 2 w, + 1 of X_n; the bases are scalar, lane l loads bytes 4 l .. 4 l + 3
   auto issue = [&](int j, int st) {
     const int s = j >> 1, h = j & 1;
     double *sb = lds[st];
-    const char *src =
-        reinterpret_cast<const char *>(A + (long)(16 * s) * kStripNB + 256 * h + 32 * w + (lane >> 1)) + (lane & 1) * 4;
+    const char *src = reinterpret_cast<const char *>(A + (long)(16 * s) * kStripNB + 256 * h + 32 * w);
+    const unsigned voff = 4u * (unsigned)lane;
 #pragma unroll
-    for (int c = 0; c < 16; ++c) lds_dma_dword(src + (long)c * kStripNB * 8, sb + c * kStripLd + 32 * w);
-    lds_dma_dword(reinterpret_cast<const char *>(Xn + (long)(16 * s + 2 * w) * kRhsChunk) + lane * 4,
-                  sb + kStripXn + 32 * w);
+    for (int c = 0; c < 16; ++c) lds_dma_dword_s(src + (long)c * kStripNB * 8, voff, sb + c * kStripLd + 32 * w);
+    lds_dma_dword_s(Xn + (long)(16 * s + 2 * w) * kRhsChunk, voff, sb + kStripXn + 32 * w);
   };
   auto read = [&](int st, StripOps &o) {
     const double *sb = lds[st];
