@@ -129,8 +129,9 @@ __global__ __launch_bounds__(kPotrfWaves * 64) void potrf_kernel(const PotrfOpT<
 #pragma unroll
         for (int t = 0; t < TA; ++t) oa[t] = min(c0 + (wave + kPotrfWaves * t) * 16 + lr, n - 1);
         // operand slots in flight: more when few row blocks leave little
-        // MFMA work per k-step to cover the load latency
-        constexpr int SL = TA >= 2 ? 2 : 4;
+        // MFMA work per k-step to cover the load latency; 4 for FP32 (one
+        // register per operand; 8 would spill)
+        constexpr int SL = sizeof(TC) == 4 ? 4 : (TA >= 2 ? 2 : 4);
         TC bv[SL][2], av[SL][TA];
         auto load = [&](int sl, int k) {
           const gtc *Wk = Wg + (long)k * ldw;
