@@ -90,11 +90,14 @@ __global__ __launch_bounds__(256) void split_kernel(const SplitOp *__restrict__ 
 // workgroups per CU.  The 32-k slabs of the A and B panel rows go from HBM
 // straight into LDS with gfx950's LDS-DMA loads (global_load_lds_dwordx4:
 // each lane's 16 bytes land at the wave's LDS base + 16 lane), so no VGPRs
-// stage them and no ds_write runs; two LDS stages, one barrier per slab.
-// The compiler does not track LDS-DMA completion, so every wait is explicit:
-// before the barrier of slab s each wave waits (vmcnt) for its own loads of
-// slab s, and the barrier then publishes all of them and retires every
-// wave's reads of the stage refilled next.
+// stage them and no ds_write runs; two LDS stages, one barrier per stage.
+// The loads are inline asm (the compiler does not track them) with the
+// slab's base in SGPRs and fixed 32-bit lane offsets: the per-lane 64-bit
+// address arithmetic of the plain form, queued behind the wave's MFMAs,
+// took most of the time of the k loop.  Every wait is explicit: before the
+// barrier of stage t each wave waits (vmcnt) for its own loads of stage t,
+// and the barrier then publishes all of them and retires every wave's reads
+// of the stage refilled next.
 // ---------------------------------------------------------------------------
 constexpr int kF16BM = 128, kF16BK = 32;  // LDS rows of 32 halves = 4 chunks of 16 bytes
 
